@@ -1,0 +1,129 @@
+"""ctypes binding of libgmfa.so (declarations in include/gmfa.h).
+
+The library is built in-tree by ``graphmambaformer_amd.build`` (or
+``__graft_entry__.build()``).  There is no fallback: if the shared object is
+missing or fails to load, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (load torch's HIP runtime before libgmfa binds to it)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgmfa.so")
+
+GMFA_OK = 0
+GMFA_E_INVALID = -1
+GMFA_E_RUNTIME = -2
+GMFA_E_UNSUPPORTED = -3
+
+SW_TRACEBACK = 1
+SW_NMASK = 2
+SW_ROWKEYS = 4
+
+OP_I, OP_D, OP_S, OP_EQ, OP_X = 1, 2, 4, 7, 8
+OP_CHAR = {OP_I: "I", OP_D: "D", OP_S: "S", OP_EQ: "=", OP_X: "X"}
+
+_P = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+
+
+class SwParams(ctypes.Structure):
+    _fields_ = [("match", _i32), ("mismatch", _i32), ("gap_open", _i32),
+                ("gap_extend", _i32), ("half_band", _i32), ("flags", ctypes.c_uint32),
+                ("cells_per_lane", _i32), ("lanes_per_pair", _i32)]
+
+
+class SwBatch(ctypes.Structure):
+    _fields_ = [("n_pairs", _i32), ("max_qlen", _i32), ("qseq", _P), ("tseq", _P),
+                ("qmask", _P), ("tmask", _P), ("q_words", _i64), ("t_words", _i64),
+                ("q_off", _P), ("t_off", _P), ("qlen", _P), ("tlen", _P),
+                ("band_offset", _P), ("order", _P)]
+
+
+class SwOut(ctypes.Structure):
+    _fields_ = [("score", _P), ("query_end", _P), ("target_end", _P), ("tb", _P),
+                ("tb_off", _P), ("rowkey", _P), ("rowkey_off", _P), ("work_counter", _P)]
+
+
+class SwTile(ctypes.Structure):
+    _fields_ = [("cells_per_lane", _i32), ("lanes_per_pair", _i32), ("lanes_stored", _i32),
+                ("half_band", _i32)]
+
+
+# Every exported symbol with its ctypes signature (must match include/gmfa.h).
+SIGNATURES = {
+    "gmfa_last_error": (ctypes.c_char_p, []),
+    "gmfa_abi_version": (ctypes.c_int, []),
+    "gmfa_device_info": (ctypes.c_int, [_P, _P, ctypes.c_char_p, _i32]),
+    "gmfa_sw_resolve_tile": (ctypes.c_int, [_i32, _i32, _i32, ctypes.POINTER(SwTile)]),
+    "gmfa_sw_list_tiles": (ctypes.c_int, [_P, _P, _i32]),
+    "gmfa_sw_tb_bytes": (_i64, [_i32, ctypes.POINTER(SwTile)]),
+    "gmfa_pack_codes": (ctypes.c_int, [_P, _i64, _i64, _P, _P, _i64, _P, _P, _P, _P]),
+    "gmfa_sw_forward": (ctypes.c_int, [ctypes.POINTER(SwBatch), ctypes.POINTER(SwParams),
+                                       ctypes.POINTER(SwOut), _P]),
+    "gmfa_sw_traceback": (ctypes.c_int, [ctypes.POINTER(SwBatch), ctypes.POINTER(SwParams),
+                                         _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "gmfa_chain_dp_f64": (ctypes.c_int, [_P, _P, _P, _P, _i32, _i32, _i64, _P, _P, _P, _P,
+                                         _P]),
+    "gmfa_chain_dp_i32": (ctypes.c_int, [_P, _P, _P, _P, _i32, _i32, _i64, _i32, _i32, _P,
+                                         _P, _P]),
+    "gmfa_chain_backtrack": (_i64, [_i64, _P, _P, _i64, ctypes.c_double, _P, _P, _P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class LibraryMissing(RuntimeError):
+    """libgmfa.so is not built / not loadable."""
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise LibraryMissing(
+                    f"{LIB_PATH} is missing: build it with `python -m graphmambaformer_amd.build`")
+            try:
+                L = ctypes.CDLL(LIB_PATH)
+            except OSError as exc:
+                raise LibraryMissing(f"cannot load {LIB_PATH}: {exc}") from exc
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            if L.gmfa_abi_version() != 1:
+                raise LibraryMissing("libgmfa ABI version mismatch")
+            _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc >= 0:
+        return rc
+    msg = lib().gmfa_last_error().decode(errors="replace")
+    if what:
+        msg = f"{what}: {msg}"
+    if rc == GMFA_E_INVALID:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(t) -> int | None:
+    """Device/host address of a tensor (None for None)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_of(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream

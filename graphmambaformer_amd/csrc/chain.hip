@@ -1,0 +1,210 @@
+// Anchor-chaining DP for MI355X (gfx950).
+//
+// Reference semantics: graphmambaformer/alignment/chaining.py:129-180
+// (chain_dp_numpy) of sarakh1999/GraphMambaFormer:
+//   f[i] = max(w_i, max_{j in [i-L, i), ok} (f[j] + min(dq, dr, w_i)) - pen(|dr-dq|))
+//   ok  <=> 0 < dq, dr <= max_gap;  ties -> smallest j;  taken only if > w_i.
+//
+// One wavefront per read (segment).  The DP is sequential in i, so the
+// parallelism is the lookback window: lane l holds predecessor j = i-64+l
+// (read_end, ref_end, f) in registers; each step evaluates all candidates,
+// reduces (max score, first j) across the wave with DPP, writes f[i] and
+// parent[i], and slides the window one lane left (DPP wave_shl:1) with anchor
+// i entering at lane 63.  Lookback is therefore capped at 64.
+//
+// The float64 profile reproduces numpy exactly: the penalty comes from a
+// caller-supplied table computed with the reference's own numpy expression
+// (chaining.py:120-126) and the score is formed as (f[j] + adv) - pen in two
+// IEEE double operations, the order numpy evaluates them in.
+#include "common.h"
+
+namespace gmfa {
+
+struct ChainArgs {
+    const int64_t *__restrict__ read_end;
+    const int64_t *__restrict__ ref_end;
+    const double *__restrict__ wd;
+    const int32_t *__restrict__ wi;
+    const int64_t *__restrict__ seg_off;
+    int32_t n_segments, L;
+    int64_t max_gap;
+    const double *__restrict__ lut;
+    const double *__restrict__ bonus;
+    int32_t go, ge;
+    double *fd;
+    int32_t *fi;
+    int64_t *parent;
+};
+
+__device__ __forceinline__ int64_t shl_i64(int64_t old, int64_t v)
+{
+    const int lo = from_next_lane((int)(uint32_t)old, (int)(uint32_t)v);
+    const int hi = from_next_lane((int)(uint32_t)((uint64_t)old >> 32), (int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+__device__ __forceinline__ double shl_f64(double old, double v)
+{
+    return __longlong_as_double(shl_i64(__double_as_longlong(old), __double_as_longlong(v)));
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double old, double v)
+{
+    const int64_t o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const int lo = dpp<CTRL, ROW_MASK>((int)(uint32_t)o, (int)(uint32_t)x);
+    const int hi = dpp<CTRL, ROW_MASK>((int)(uint32_t)((uint64_t)o >> 32), (int)(uint32_t)((uint64_t)x >> 32));
+    return __longlong_as_double((int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// max over the 64 lanes, result valid in lane 63
+__device__ __forceinline__ double wave_max_to63(double x)
+{
+    const double ninf = -__builtin_inf();
+    x = fmax(x, dpp_f64<kRowShr1>(ninf, x));
+    x = fmax(x, dpp_f64<kRowShr2>(ninf, x));
+    x = fmax(x, dpp_f64<kRowShr4>(ninf, x));
+    x = fmax(x, dpp_f64<kRowShr8>(ninf, x));
+    x = fmax(x, dpp_f64<kRowBcast15, 0xa>(ninf, x));
+    x = fmax(x, dpp_f64<kRowBcast31, 0xc>(ninf, x));
+    return x;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const int64_t x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)x, l);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), l);
+    return __longlong_as_double((int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+template <bool F64, bool BONUS>
+__global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
+{
+    const int seg = blockIdx.x;
+    if (seg >= a.n_segments) return;
+    const int lane = (int)threadIdx.x;
+    const int64_t base = a.seg_off[seg];
+    const int n = (int)(a.seg_off[seg + 1] - base);
+    const int L = a.L;
+    const int64_t G = a.max_gap;
+
+    int64_t wre = 0, wfe = 0;
+    double wfd = 0.0;
+    int wfi = 0;
+    for (int i = 0; i < n; ++i) {
+        const int64_t re_i = a.read_end[base + i];
+        const int64_t fe_i = a.ref_end[base + i];
+        const int lo = i - L > 0 ? i - L : 0;
+        const int nvalid = i - lo;  // predecessors lo..i-1 sit in lanes 64-nvalid..63
+        const int j = i - 64 + lane;
+        const int64_t dq = re_i - wre;
+        const int64_t dr = fe_i - wfe;
+        const bool ok = (lane >= 64 - nvalid) && dq > 0 && dr > 0 && dq <= G && dr <= G;
+        const int64_t mn = dq < dr ? dq : dr;
+        int64_t g = dr - dq;
+        g = g < 0 ? -g : g;
+        const uint64_t okmask = __builtin_amdgcn_ballot_w64(ok);
+        int par = -1;
+        if constexpr (F64) {
+            const double w_i = a.wd[base + i];
+            double s = -__builtin_inf();
+            if (ok) {
+                const double adv = (double)mn < w_i ? (double)mn : w_i;
+                const double t1 = __dadd_rn(wfd, adv);
+                s = __dsub_rn(t1, a.lut[g]);
+                if (BONUS) s = __dadd_rn(s, a.bonus[(base + i) * L + (L - nvalid) + (j - lo)]);
+            }
+            const double best = readlane_f64(wave_max_to63(s), 63);
+            double fi = w_i;
+            if (okmask != 0ull) {
+                const uint64_t hit = __builtin_amdgcn_ballot_w64(ok && s == best);
+                const int bl = (int)__builtin_ctzll(hit);
+                if (best > w_i) { fi = best; par = i - 64 + bl; }
+            }
+            if (lane == 0) { a.fd[base + i] = fi; a.parent[base + i] = par; }
+            wfd = shl_f64(fi, wfd);
+        } else {
+            const int w_i = a.wi[base + i];
+            int key = INT_MIN;
+            if (ok) {
+                const int adv = (int)(mn < (int64_t)w_i ? mn : (int64_t)w_i);
+                const int pen = g > 0 ? a.go + a.ge * (int)g : 0;
+                const int s = wfi + adv - pen;
+                key = (int)(((uint32_t)s << 6) | (uint32_t)(63 - lane));
+            }
+            key = seg_scan_max<64>(key);
+            key = __builtin_amdgcn_readlane(key, 63);
+            int fi = w_i;
+            if (okmask != 0ull) {
+                const int best = key >> 6;
+                if (best > w_i) { fi = best; par = i - 64 + (63 - (key & 63)); }
+            }
+            if (lane == 0) { a.fi[base + i] = fi; a.parent[base + i] = par; }
+            wfi = from_next_lane(fi, wfi);
+        }
+        wre = shl_i64(re_i, wre);
+        wfe = shl_i64(fe_i, wfe);
+    }
+}
+
+}  // namespace gmfa
+
+using namespace gmfa;
+
+static int chain_common_checks(const int64_t *read_end, const int64_t *ref_end,
+                               const int64_t *seg_off, int32_t n_segments, int32_t L,
+                               int64_t max_gap)
+{
+    GMFA_REQUIRE(n_segments >= 0, "negative segment count");
+    GMFA_REQUIRE(L >= 1 && L <= 64, "max_lookback %d outside [1, 64]", L);
+    GMFA_REQUIRE(max_gap >= 0, "negative max_gap");
+    if (n_segments == 0) return GMFA_OK;
+    GMFA_REQUIRE(read_end && ref_end && seg_off, "NULL pointer");
+    return GMFA_OK;
+}
+
+extern "C" int gmfa_chain_dp_f64(const int64_t *read_end, const int64_t *ref_end,
+                                 const double *weight, const int64_t *seg_off,
+                                 int32_t n_segments, int32_t max_lookback, int64_t max_gap,
+                                 const double *pen_lut, const double *bonus, double *f,
+                                 int64_t *parent, void *stream)
+{
+    int rc = chain_common_checks(read_end, ref_end, seg_off, n_segments, max_lookback, max_gap);
+    if (rc != GMFA_OK || n_segments == 0) return rc;
+    GMFA_REQUIRE(weight && pen_lut && f && parent, "NULL pointer");
+    GMFA_REQUIRE(max_gap >= 1, "max_gap must be >= 1 for the penalty table");
+    ChainArgs a{};
+    a.read_end = read_end; a.ref_end = ref_end; a.wd = weight; a.seg_off = seg_off;
+    a.n_segments = n_segments; a.L = max_lookback; a.max_gap = max_gap;
+    a.lut = pen_lut; a.bonus = bonus; a.fd = f; a.parent = parent;
+    if (bonus)
+        hipLaunchKernelGGL((chain_dp_kernel<true, true>), dim3(n_segments), dim3(64), 0,
+                           (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL((chain_dp_kernel<true, false>), dim3(n_segments), dim3(64), 0,
+                           (hipStream_t)stream, a);
+    GMFA_CHECK_HIP(hipGetLastError());
+    return GMFA_OK;
+}
+
+extern "C" int gmfa_chain_dp_i32(const int64_t *read_end, const int64_t *ref_end,
+                                 const int32_t *weight, const int64_t *seg_off,
+                                 int32_t n_segments, int32_t max_lookback, int64_t max_gap,
+                                 int32_t gap_open, int32_t gap_extend, int32_t *f,
+                                 int64_t *parent, void *stream)
+{
+    int rc = chain_common_checks(read_end, ref_end, seg_off, n_segments, max_lookback, max_gap);
+    if (rc != GMFA_OK || n_segments == 0) return rc;
+    GMFA_REQUIRE(weight && f && parent, "NULL pointer");
+    GMFA_REQUIRE(gap_open >= 0 && gap_extend >= 0 && (int64_t)gap_extend * max_gap < (1 << 22),
+                 "integer chaining penalties out of range");
+    ChainArgs a{};
+    a.read_end = read_end; a.ref_end = ref_end; a.wi = weight; a.seg_off = seg_off;
+    a.n_segments = n_segments; a.L = max_lookback; a.max_gap = max_gap;
+    a.go = gap_open; a.ge = gap_extend; a.fi = f; a.parent = parent;
+    hipLaunchKernelGGL((chain_dp_kernel<false, false>), dim3(n_segments), dim3(64), 0,
+                       (hipStream_t)stream, a);
+    GMFA_CHECK_HIP(hipGetLastError());
+    return GMFA_OK;
+}

@@ -1,0 +1,402 @@
+"""Low-level host API over libgmfa: packing, SW forward, traceback walk, chaining DP.
+
+Everything here works on torch tensors resident on one HIP device and calls
+the C ABI through ctypes; outputs are allocated by the caller side (torch), so
+the library owns no device memory.  The drop-in modules in
+``graphmambaformer_amd.alignment`` are built on these functions.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import SwBatch, SwOut, SwParams, SwTile, check, lib, ptr, stream_of
+
+# ----------------------------------------------------------------------------
+# 2-bit packing
+# ----------------------------------------------------------------------------
+
+
+def _pack_rows(codes2: np.ndarray, nmask: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """codes2 / nmask: (words*16,) uint8 -> (words,) uint32 each."""
+    c = codes2.reshape(-1, 16).astype(np.uint32)
+    m = nmask.reshape(-1, 16).astype(np.uint32)
+    sh = (np.arange(16, dtype=np.uint32) * 2)[None, :]
+    return (c << sh).sum(axis=1, dtype=np.uint64).astype(np.uint32), \
+        (m << sh).sum(axis=1, dtype=np.uint64).astype(np.uint32)
+
+
+def pack_host(seqs: Sequence[np.ndarray], alphabet: str = "ref"):
+    """Pack sequences back to back, each starting on a 16-base word.
+
+    alphabet "ref": codes of the reference encoder (A,C,G,T = 1..4, N = 5,
+    negative = padding; seeding.py:46-80); "2bit": codes 0..3, no N.
+    Returns (words uint32, masks uint32, base offsets int64, has_n).
+    """
+    lens = np.array([len(s) for s in seqs], dtype=np.int64)
+    nwords = (lens + 15) // 16
+    woff = np.zeros(len(seqs) + 1, dtype=np.int64)
+    np.cumsum(nwords, out=woff[1:])
+    total = int(woff[-1]) if len(seqs) else 0
+    total = max(total, 1)
+    c2 = np.zeros(total * 16, dtype=np.uint8)
+    nm = np.ones(total * 16, dtype=np.uint8)  # padding never matches
+    has_n = False
+    for k, s in enumerate(seqs):
+        s = np.asarray(s)
+        o = int(woff[k]) * 16
+        if alphabet == "2bit":
+            c2[o:o + len(s)] = s
+            nm[o:o + len(s)] = 0
+        else:
+            s16 = s.astype(np.int16)
+            bad = (s16 == 0) | (s16 > 5)
+            if bad.any():
+                raise ValueError("base codes must be negative (padding), 1..4 (ACGT) or 5 (N)")
+            ok = (s16 >= 1) & (s16 <= 4)
+            c2[o:o + len(s)] = np.where(ok, s16 - 1, 0)
+            nm[o:o + len(s)] = ~ok
+            has_n = has_n or bool((~ok).any())
+    words, masks = _pack_rows(c2, nm)
+    return words, masks, woff[:-1] * 16, has_n
+
+
+def _as_i32(a: np.ndarray) -> np.ndarray:
+    return a.view(np.int32) if a.dtype == np.uint32 else a.astype(np.int32)
+
+
+@dataclass
+class SwInputs:
+    """A batch of pairs resident on the device (2-bit streams + per-pair geometry)."""
+
+    qseq: torch.Tensor
+    tseq: torch.Tensor
+    qmask: Optional[torch.Tensor]
+    tmask: Optional[torch.Tensor]
+    q_off: torch.Tensor
+    t_off: torch.Tensor
+    qlen: torch.Tensor
+    tlen: torch.Tensor
+    band_offset: torch.Tensor
+    qlen_host: np.ndarray
+    tlen_host: np.ndarray
+    boff_host: np.ndarray
+    has_n: bool
+
+    @property
+    def n_pairs(self) -> int:
+        return int(len(self.qlen_host))
+
+    @property
+    def device(self) -> torch.device:
+        return self.qseq.device
+
+    def batch_struct(self, order: Optional[torch.Tensor] = None) -> SwBatch:
+        b = SwBatch()
+        b.n_pairs = self.n_pairs
+        b.max_qlen = int(self.qlen_host.max()) if self.n_pairs else 0
+        b.qseq, b.tseq = ptr(self.qseq), ptr(self.tseq)
+        b.qmask = ptr(self.qmask) if self.has_n else None
+        b.tmask = ptr(self.tmask) if self.has_n else None
+        b.q_words, b.t_words = self.qseq.numel(), self.tseq.numel()
+        b.q_off, b.t_off = ptr(self.q_off), ptr(self.t_off)
+        b.qlen, b.tlen, b.band_offset = ptr(self.qlen), ptr(self.tlen), ptr(self.band_offset)
+        b.order = ptr(order)
+        return b
+
+
+def upload_pairs(queries: Sequence[np.ndarray], targets: Sequence[np.ndarray], band_offset,
+                 device, alphabet: str = "ref") -> SwInputs:
+    """Pack on the host and copy once to the device."""
+    qw, qm, qo, qn = pack_host(queries, alphabet)
+    tw, tm, to, tn = pack_host(targets, alphabet)
+    has_n = qn or tn
+    dev = torch.device(device)
+    ql = np.array([len(q) for q in queries], dtype=np.int32)
+    tl = np.array([len(t) for t in targets], dtype=np.int32)
+    bo = np.asarray(band_offset, dtype=np.int32).reshape(len(queries))
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    return SwInputs(qseq=up(_as_i32(qw)), tseq=up(_as_i32(tw)),
+                    qmask=up(_as_i32(qm)) if has_n else None,
+                    tmask=up(_as_i32(tm)) if has_n else None,
+                    q_off=up(qo), t_off=up(to), qlen=up(ql), tlen=up(tl), band_offset=up(bo),
+                    qlen_host=ql, tlen_host=tl, boff_host=bo, has_n=has_n)
+
+
+def pack_device(codes: torch.Tensor, lens: torch.Tensor, lens_host: np.ndarray):
+    """Pack a (B, L) int16 code matrix already on the device (drop-in path)."""
+    B, L = codes.shape
+    nwords = (lens_host.astype(np.int64) + 15) // 16
+    woff = np.zeros(B + 1, dtype=np.int64)
+    np.cumsum(nwords, out=woff[1:])
+    total = max(int(woff[-1]), 1)
+    dev = codes.device
+    seq = torch.zeros(total, dtype=torch.int32, device=dev)
+    mask = torch.full((total,), -1, dtype=torch.int32, device=dev)
+    bad = torch.zeros(1, dtype=torch.int32, device=dev)
+    woff_d = torch.from_numpy(woff[:-1]).to(dev)
+    lens32 = lens.to(device=dev, dtype=torch.int32).contiguous()
+    codes = codes.to(torch.int16).contiguous()
+    check(lib().gmfa_pack_codes(ptr(codes), B, L, ptr(lens32), ptr(woff_d), total, ptr(seq),
+                                ptr(mask), ptr(bad), stream_of(dev)), "gmfa_pack_codes")
+    return seq, mask, woff_d * 16, bad
+
+
+# ----------------------------------------------------------------------------
+# SW forward + traceback
+# ----------------------------------------------------------------------------
+
+
+def resolve_tile(half_band: int, cells_per_lane: int = 0, lanes_per_pair: int = 0) -> SwTile:
+    t = SwTile()
+    check(lib().gmfa_sw_resolve_tile(int(half_band), int(cells_per_lane), int(lanes_per_pair),
+                                     ctypes.byref(t)), "gmfa_sw_resolve_tile")
+    return t
+
+
+def list_tiles() -> list[tuple[int, int]]:
+    """Compiled (cells_per_lane, lanes_per_pair) tiles of the forward kernel."""
+    n = lib().gmfa_sw_list_tiles(None, None, 0)
+    c = (ctypes.c_int32 * n)()
+    g = (ctypes.c_int32 * n)()
+    lib().gmfa_sw_list_tiles(c, g, n)
+    return [(int(c[k]), int(g[k])) for k in range(n)]
+
+
+def tb_bytes(qlen: np.ndarray, tile: SwTile) -> np.ndarray:
+    """Traceback plane bytes per pair (same formula as gmfa_sw_tb_bytes)."""
+    nblk = (np.asarray(qlen, dtype=np.int64) + 7) // 8
+    return nblk * tile.lanes_stored * tile.cells_per_lane * 4
+
+
+@dataclass
+class SwForward:
+    score: torch.Tensor
+    query_end: torch.Tensor
+    target_end: torch.Tensor
+    tb: Optional[torch.Tensor]
+    tb_off: Optional[torch.Tensor]
+    rowkey: Optional[torch.Tensor]
+    rowkey_off_host: Optional[np.ndarray]
+    tile: SwTile
+    params: SwParams
+
+
+def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile) -> SwParams:
+    p = SwParams()
+    p.match, p.mismatch, p.gap_open, p.gap_extend = int(ma), int(mi), int(go), int(ge)
+    p.half_band = int(half_band)
+    p.flags = int(flags)
+    p.cells_per_lane = tile.cells_per_lane
+    p.lanes_per_pair = tile.lanes_per_pair
+    return p
+
+
+def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int,
+               traceback: bool = True, rowkeys: bool = False,
+               order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
+               workspace: Optional[dict] = None) -> SwForward:
+    dev = inp.device
+    n = inp.n_pairs
+    tile = tile or resolve_tile(half_band)
+    flags = 0
+    if traceback:
+        flags |= _lib.SW_TRACEBACK
+    if inp.has_n:
+        flags |= _lib.SW_NMASK
+    if rowkeys:
+        flags |= _lib.SW_ROWKEYS
+    ws = workspace if workspace is not None else {}
+    score = torch.empty(n, dtype=torch.int32, device=dev)
+    qend = torch.empty(n, dtype=torch.int32, device=dev)
+    tend = torch.empty(n, dtype=torch.int32, device=dev)
+    tb = tb_off = rk = rk_off = None
+    rk_off_host = None
+    if traceback:
+        if "tb" in ws:
+            tb, tb_off = ws["tb"], ws["tb_off"]
+        else:
+            nb = tb_bytes(inp.qlen_host, tile)
+            offs = np.zeros(n + 1, dtype=np.int64)
+            np.cumsum(nb // 4, out=offs[1:])
+            tb = torch.empty(max(int(offs[-1]), 1), dtype=torch.int32, device=dev)
+            tb_off = torch.from_numpy(offs[:-1]).to(dev)
+    if rowkeys:
+        per = inp.qlen_host.astype(np.int64) * tile.lanes_stored
+        offs = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(per, out=offs[1:])
+        rk = torch.empty(max(int(offs[-1]), 1), dtype=torch.int32, device=dev)
+        rk_off = torch.from_numpy(offs[:-1]).to(dev)
+        rk_off_host = offs
+    counter = ws.get("counter")
+    if counter is None:
+        counter = torch.zeros(1, dtype=torch.int32, device=dev)
+    out = SwOut()
+    out.score, out.query_end, out.target_end = ptr(score), ptr(qend), ptr(tend)
+    out.tb, out.tb_off = ptr(tb), ptr(tb_off)
+    out.rowkey, out.rowkey_off = ptr(rk), ptr(rk_off)
+    out.work_counter = ptr(counter)
+    params = make_params(ma, mi, go, ge, half_band, flags, tile)
+    batch = inp.batch_struct(order)
+    check(lib().gmfa_sw_forward(ctypes.byref(batch), ctypes.byref(params), ctypes.byref(out),
+                                stream_of(dev)), "gmfa_sw_forward")
+    return SwForward(score=score, query_end=qend, target_end=tend, tb=tb, tb_off=tb_off,
+                     rowkey=rk, rowkey_off_host=rk_off_host, tile=tile, params=params)
+
+
+@dataclass
+class SwTraceback:
+    cigar: torch.Tensor      # flat BAM runs (len << 4 | op), int32 view
+    cigar_off: torch.Tensor  # int64 [n]
+    n_cigar: torch.Tensor
+    read_start: torch.Tensor
+    ref_start: torch.Tensor
+    counts: torch.Tensor     # (n, 4): =, X, I, D
+
+
+def sw_traceback(inp: SwInputs, fwd: SwForward, score=None, query_end=None, target_end=None,
+                 workspace: Optional[dict] = None) -> SwTraceback:
+    if fwd.tb is None:
+        raise ValueError("traceback needs the forward pass with traceback=True")
+    dev = inp.device
+    n = inp.n_pairs
+    ws = workspace if workspace is not None else {}
+    if "cigar" in ws:
+        cig, cig_off = ws["cigar"], ws["cigar_off"]
+    else:
+        cap = inp.qlen_host.astype(np.int64) + inp.tlen_host.astype(np.int64) + 1
+        offs = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(cap, out=offs[1:])
+        cig = torch.empty(max(int(offs[-1]), 1), dtype=torch.int32, device=dev)
+        cig_off = torch.from_numpy(offs[:-1]).to(dev)
+    ncig = torch.empty(n, dtype=torch.int32, device=dev)
+    rs = torch.empty(n, dtype=torch.int32, device=dev)
+    ts = torch.empty(n, dtype=torch.int32, device=dev)
+    counts = torch.empty((n, 4), dtype=torch.int32, device=dev)
+    score = fwd.score if score is None else score
+    query_end = fwd.query_end if query_end is None else query_end
+    target_end = fwd.target_end if target_end is None else target_end
+    batch = inp.batch_struct()
+    check(lib().gmfa_sw_traceback(ctypes.byref(batch), ctypes.byref(fwd.params), ptr(score),
+                                  ptr(query_end), ptr(target_end), ptr(fwd.tb), ptr(fwd.tb_off),
+                                  ptr(cig_off), ptr(cig), ptr(ncig), ptr(rs), ptr(ts),
+                                  ptr(counts), stream_of(dev)), "gmfa_sw_traceback")
+    return SwTraceback(cigar=cig, cigar_off=cig_off, n_cigar=ncig, read_start=rs, ref_start=ts,
+                       counts=counts)
+
+
+def decode_cigars(tbk: SwTraceback) -> list[list[tuple[str, int]]]:
+    """Device CIGAR runs -> list of [(op, n)] per pair (one D2H copy)."""
+    cig = tbk.cigar.cpu().numpy().view(np.uint32)
+    off = tbk.cigar_off.cpu().numpy()
+    nc = tbk.n_cigar.cpu().numpy()
+    out = []
+    for p in range(len(nc)):
+        runs = cig[off[p]:off[p] + nc[p]]
+        out.append([(_lib.OP_CHAR[int(r & 15)], int(r >> 4)) for r in runs])
+    return out
+
+
+def xdrop_best(fwd: SwForward, qlen_host: np.ndarray, x_drop: float, hb: int,
+               boff_host: np.ndarray):
+    """Batch-wide x-drop (extension.py:190-193) from per-row keys.
+
+    The forward pass computed every row of every pair; the reference stops the
+    whole batch after the first row i where every pair has
+    row_max_seen - row_best > x_drop, so the best cell is the first strict
+    maximum over rows <= i.  Returns host arrays (score, qend, tend)."""
+    n = len(qlen_host)
+    keys = fwd.rowkey.cpu().numpy()
+    offs = fwd.rowkey_off_host
+    ls = fwd.tile.lanes_stored
+    maxq = int(qlen_host.max()) if n else 0
+    rb = np.zeros((n, maxq), dtype=np.int64)
+    rd = np.zeros((n, maxq), dtype=np.int64)
+    for p in range(n):
+        q = int(qlen_host[p])
+        if q == 0:
+            continue
+        k = keys[offs[p]:offs[p + 1]].reshape(q, ls).astype(np.int64).max(axis=1)
+        rb[p, :q] = k >> 11
+        rd[p, :q] = 2047 - (k & 2047)
+    stop = maxq
+    if x_drop > 0 and maxq:
+        thr = int(np.floor(x_drop))
+        seen = np.maximum.accumulate(rb, axis=1)
+        drop = (seen - rb) > thr
+        allrows = drop.all(axis=0)
+        hit = np.flatnonzero(allrows)
+        if len(hit):
+            stop = int(hit[0]) + 1
+    score = np.zeros(n, dtype=np.int32)
+    qe = np.zeros(n, dtype=np.int32)
+    te = np.zeros(n, dtype=np.int32)
+    for p in range(n):
+        r = rb[p, :stop]
+        if r.size == 0 or r.max() <= 0:
+            continue
+        i = int(np.argmax(r))  # first row reaching the max = first strict improvement
+        score[p] = int(r[i])
+        qe[p] = i + 1
+        te[p] = i + 1 + int(boff_host[p]) - hb + int(rd[p, i])
+    return score, qe, te
+
+
+# ----------------------------------------------------------------------------
+# chaining
+# ----------------------------------------------------------------------------
+
+
+def chain_dp(read_end: torch.Tensor, ref_end: torch.Tensor, weight: torch.Tensor,
+             seg_off: torch.Tensor, max_lookback: int, max_gap: int,
+             pen_lut: Optional[torch.Tensor] = None, gap_open: int = 0, gap_extend: int = 0,
+             bonus: Optional[torch.Tensor] = None):
+    """Segmented chaining DP on the device.
+
+    float64 weights -> float64 profile with ``pen_lut``; int32 weights -> the
+    integer profile (penalty = open + extend*gap)."""
+    dev = read_end.device
+    n_seg = seg_off.numel() - 1
+    n = read_end.numel()
+    parent = torch.empty(n, dtype=torch.int64, device=dev)
+    if weight.dtype == torch.float64:
+        if pen_lut is None:
+            raise ValueError("float64 chaining needs pen_lut")
+        f = torch.empty(n, dtype=torch.float64, device=dev)
+        check(lib().gmfa_chain_dp_f64(ptr(read_end), ptr(ref_end), ptr(weight), ptr(seg_off),
+                                      n_seg, int(max_lookback), int(max_gap), ptr(pen_lut),
+                                      ptr(bonus), ptr(f), ptr(parent), stream_of(dev)),
+              "gmfa_chain_dp_f64")
+    elif weight.dtype == torch.int32:
+        f = torch.empty(n, dtype=torch.int32, device=dev)
+        check(lib().gmfa_chain_dp_i32(ptr(read_end), ptr(ref_end), ptr(weight), ptr(seg_off),
+                                      n_seg, int(max_lookback), int(max_gap), int(gap_open),
+                                      int(gap_extend), ptr(f), ptr(parent), stream_of(dev)),
+              "gmfa_chain_dp_i32")
+    else:
+        raise ValueError("weight must be float64 or int32")
+    return f, parent
+
+
+def chain_backtrack(f: np.ndarray, parent: np.ndarray, min_anchors: int, min_score: float):
+    """Host AffineChainer._backtrack through the library (members in read order)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    parent = np.ascontiguousarray(parent, dtype=np.int64)
+    n = len(f)
+    members = np.zeros(max(n, 1), dtype=np.int64)
+    starts = np.zeros(n + 1, dtype=np.int64)
+    scores = np.zeros(max(n, 1), dtype=np.float64)
+    nc = check(lib().gmfa_chain_backtrack(n, f.ctypes.data, parent.ctypes.data, int(min_anchors),
+                                          float(min_score), members.ctypes.data,
+                                          starts.ctypes.data, scores.ctypes.data),
+               "gmfa_chain_backtrack")
+    return [(members[starts[c]:starts[c + 1]], float(scores[c])) for c in range(nc)]

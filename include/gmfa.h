@@ -1,0 +1,178 @@
+/*
+ * gmfa.h -- C ABI of libgmfa.so, the MI355X (gfx950) banded affine-gap
+ * Smith-Waterman + traceback and anchor-chaining library.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes (device
+ * pointers unless stated), returns 0 on success or a negative GMFA_E_* code,
+ * and leaves a message for gmfa_last_error() (thread-local) on failure.
+ * Streams are hipStream_t passed as void*; NULL means the null stream.
+ * The library keeps no mutable global state besides the thread-local error
+ * string and per-device read-only properties, so calls are reentrant.
+ *
+ * The reference path these entry points replace (files of
+ * sarakh1999/GraphMambaFormer):
+ *   gmfa_sw_forward    graphmambaformer/alignment/extension.py:78-204
+ *                      banded_affine_sw_batch (and the NVIDIA tier
+ *                      accel/cuda_kernels.py:504-566 banded_sw)
+ *   gmfa_sw_traceback  graphmambaformer/alignment/extension.py:207-302
+ *                      traceback_banded
+ *   gmfa_pack_codes    graphmambaformer/alignment/seeding.py:46-80 code
+ *                      alphabet (A,C,G,T=1..4, N=5, padding<0) -> 2-bit + N mask
+ *   gmfa_chain_dp_f64  graphmambaformer/alignment/chaining.py:129-180
+ *                      chain_dp_numpy (and chaining.py:183-258
+ *                      chain_dp_batched, cuda_kernels.py:440-501 chain_dp)
+ *   gmfa_chain_dp_i32  the same DP on the integer profile
+ *   gmfa_chain_backtrack graphmambaformer/alignment/chaining.py:530-576
+ *                      AffineChainer._backtrack (host code)
+ */
+#ifndef GMFA_H
+#define GMFA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMFA_ABI_VERSION 1
+
+#define GMFA_OK 0
+#define GMFA_E_INVALID -1   /* bad argument: maps to ValueError */
+#define GMFA_E_RUNTIME -2   /* HIP runtime / launch failure: maps to RuntimeError */
+#define GMFA_E_UNSUPPORTED -3 /* shape outside the compiled variants: RuntimeError */
+
+/* flags for gmfa_sw_params.flags */
+#define GMFA_SW_TRACEBACK 1u  /* write the 4-bit traceback planes */
+#define GMFA_SW_NMASK 2u      /* sequences carry N masks (qmask/tmask valid) */
+#define GMFA_SW_ROWKEYS 4u    /* write per-row (max H, first column) keys */
+
+/* BAM CIGAR op codes used in traceback output (len << 4 | op) */
+#define GMFA_OP_I 1
+#define GMFA_OP_D 2
+#define GMFA_OP_S 4
+#define GMFA_OP_EQ 7
+#define GMFA_OP_X 8
+
+typedef struct gmfa_sw_params {
+    int32_t match;       /* cfg.match_score (integer) */
+    int32_t mismatch;    /* cfg.mismatch_penalty (integer, positive) */
+    int32_t gap_open;    /* cfg.gap_open; a gap of length L costs open+(L-1)*extend */
+    int32_t gap_extend;  /* cfg.gap_extend, <= gap_open */
+    int32_t half_band;   /* band is 2*half_band+1 wide, <= 1023 */
+    uint32_t flags;      /* GMFA_SW_* */
+    int32_t cells_per_lane; /* 0 = library default for the band; else a compiled tile */
+    int32_t lanes_per_pair; /* 0 = library default; else 16, 32 or 64 */
+} gmfa_sw_params;
+
+/* A batch of pairs whose sequences live in 2-bit packed streams
+ * (16 bases per uint32, base k of a word at bits 2k..2k+1, A=0 C=1 G=2 T=3).
+ * N masks use the same layout with value 1 marking a base that never matches. */
+typedef struct gmfa_sw_batch {
+    int32_t n_pairs;
+    int32_t max_qlen;      /* host-side bound: every qlen <= max_qlen */
+    const uint32_t *qseq;  /* query stream */
+    const uint32_t *tseq;  /* target stream */
+    const uint32_t *qmask; /* N masks or NULL (requires GMFA_SW_NMASK) */
+    const uint32_t *tmask;
+    int64_t q_words;       /* words in qseq / qmask */
+    int64_t t_words;       /* words in tseq / tmask */
+    const int64_t *q_off;  /* [n_pairs] first base of each query in qseq */
+    const int64_t *t_off;  /* [n_pairs] first base of each target in tseq */
+    const int32_t *qlen;   /* [n_pairs] */
+    const int32_t *tlen;   /* [n_pairs] */
+    const int32_t *band_offset; /* [n_pairs] diagonal of the band centre */
+    const int32_t *order;  /* [n_pairs] processing order (longest first) or NULL */
+} gmfa_sw_batch;
+
+typedef struct gmfa_sw_out {
+    int32_t *score;        /* [n_pairs] best local score */
+    int32_t *query_end;    /* [n_pairs] 1-based row of the best cell (0 if none) */
+    int32_t *target_end;   /* [n_pairs] 1-based column of the best cell */
+    uint32_t *tb;          /* traceback planes (GMFA_SW_TRACEBACK) */
+    const int64_t *tb_off; /* [n_pairs] dword offset of each pair's planes */
+    int32_t *rowkey;       /* per-row keys (GMFA_SW_ROWKEYS): qlen x lanes_stored
+                              int32 per pair, key = H << 11 | (2047 - d) of a
+                              lane's first row maximum; max over a row = the row's
+                              (max H, first column) */
+    const int64_t *rowkey_off; /* [n_pairs] element offset of each pair's keys */
+    uint32_t *work_counter;    /* one device uint32 of scratch for the work queue */
+} gmfa_sw_out;
+
+typedef struct gmfa_sw_tile {
+    int32_t cells_per_lane; /* band diagonals per lane */
+    int32_t lanes_per_pair; /* lanes of a wavefront that share one pair */
+    int32_t lanes_stored;   /* ceil(band / cells_per_lane): lanes with planes */
+    int32_t half_band;
+} gmfa_sw_tile;
+
+/* library / device */
+const char *gmfa_last_error(void);
+int gmfa_abi_version(void);
+int gmfa_device_info(int32_t *cu_count, int32_t *warp_size, char *arch_name, int32_t arch_len);
+
+/* Tile (cells per lane, lanes per pair) the forward kernel uses for a band of
+ * half-width half_band given a requested tile (0 = default); fails with
+ * GMFA_E_UNSUPPORTED if no compiled variant covers it. */
+int gmfa_sw_resolve_tile(int32_t half_band, int32_t cells_per_lane, int32_t lanes_per_pair,
+                         gmfa_sw_tile *tile);
+
+/* Compiled (cells_per_lane, lanes_per_pair) tiles; returns how many exist and
+ * fills up to cap entries. */
+int gmfa_sw_list_tiles(int32_t *cells_per_lane, int32_t *lanes_per_pair, int32_t cap);
+
+/* Bytes of traceback planes one pair of query length qlen needs with tile. */
+int64_t gmfa_sw_tb_bytes(int32_t qlen, const gmfa_sw_tile *tile);
+
+/* Pack int16 base codes (rows of a (B, L) row-major matrix, row b of length
+ * lens[b]) into the 2-bit stream at word offset word_off[b] (a multiple of
+ * 16 bases).  Codes 1..4 -> A..T; code 5 and negative codes -> N mask.
+ * *bad_codes (device int32, zeroed by the caller) receives nonzero if a code
+ * outside {<0, 1..5} occurs within a row's length. */
+int gmfa_pack_codes(const int16_t *codes, int64_t B, int64_t L, const int32_t *lens,
+                    const int64_t *word_off, int64_t total_words, uint32_t *seq,
+                    uint32_t *mask, int32_t *bad_codes, void *stream);
+
+/* Forward pass: scores, ends and (optionally) traceback planes / row keys. */
+int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
+                    const gmfa_sw_out *out, void *stream);
+
+/* Walk the traceback planes from (query_end, target_end) of every pair and
+ * write BAM-style CIGAR runs (len << 4 | op, forward order) at
+ * cigar[cigar_off[p]], their count n_cigar[p], the start (0-based
+ * read_start / ref_start) and op tallies counts[4p..4p+3] = (=, X, I, D).
+ * score/query_end/target_end are the (possibly x-drop adjusted) best cells. */
+int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
+                      const int32_t *score, const int32_t *query_end,
+                      const int32_t *target_end, const uint32_t *tb, const int64_t *tb_off,
+                      const int64_t *cigar_off, uint32_t *cigar, int32_t *n_cigar,
+                      int32_t *read_start, int32_t *ref_start, int32_t *counts,
+                      void *stream);
+
+/* Chaining DP over reads stored back to back (segment s = anchors
+ * seg_off[s] .. seg_off[s+1]-1, already in DP order).  Float64 profile:
+ * pen_lut[g] = penalty of gap g for g < max_gap (device array); bonus is an
+ * optional (n_anchors, max_lookback) row-major graph bonus.
+ * Requires 1 <= max_lookback <= 64. */
+int gmfa_chain_dp_f64(const int64_t *read_end, const int64_t *ref_end, const double *weight,
+                      const int64_t *seg_off, int32_t n_segments, int32_t max_lookback,
+                      int64_t max_gap, const double *pen_lut, const double *bonus,
+                      double *f, int64_t *parent, void *stream);
+
+/* Integer profile: penalty(g) = g > 0 ? gap_open + gap_extend*g : 0. */
+int gmfa_chain_dp_i32(const int64_t *read_end, const int64_t *ref_end, const int32_t *weight,
+                      const int64_t *seg_off, int32_t n_segments, int32_t max_lookback,
+                      int64_t max_gap, int32_t gap_open, int32_t gap_extend,
+                      int32_t *f, int64_t *parent, void *stream);
+
+/* Host: AffineChainer._backtrack over one segment (host pointers).
+ * members[] receives DP-local indices chain by chain in read order,
+ * chain_start[0..n] the boundaries, chain_score[c] the scores.
+ * Returns the number of chains (>= 0) or a negative error. */
+int64_t gmfa_chain_backtrack(int64_t n, const double *f, const int64_t *parent,
+                             int64_t min_anchors, double min_score, int64_t *members,
+                             int64_t *chain_start, double *chain_score);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GMFA_H */

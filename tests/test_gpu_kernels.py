@@ -1,0 +1,214 @@
+"""GPU parity of the HIP kernels against the CPU oracle (bit-exact).
+
+Called through the C ABI (graphmambaformer_amd.kernels -> libgmfa.so).  Every
+case prints its name first so a failing run names it.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.golden.load import fhex, load
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def K():
+    from graphmambaformer_amd import kernels
+
+    return kernels
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _random_pair(rng, n, with_n=False):
+    q = rng.integers(1, 5, size=n).astype(np.int16)
+    t = q.tolist()
+    for _ in range(int(rng.integers(0, max(2, n // 15)))):
+        p = int(rng.integers(0, len(t)))
+        k = rng.integers(0, 3)
+        if k == 0:
+            t[p] = int(rng.integers(1, 5))
+        elif k == 1 and len(t) > 3:
+            del t[p:p + int(rng.integers(1, 4))]
+        else:
+            t[p:p] = [int(rng.integers(1, 5))] * int(rng.integers(1, 4))
+    t = rng.integers(1, 5, size=int(rng.integers(0, 20))).tolist() + t
+    t = t + rng.integers(1, 5, size=int(rng.integers(0, 20))).tolist()
+    t = np.array(t, np.int16)
+    if with_n:
+        for _ in range(2):
+            q[int(rng.integers(0, len(q)))] = 5
+            t[int(rng.integers(0, len(t)))] = 5
+    return q, t
+
+
+def _gpu_align(qs, ts, offs, hb, scoring, tile=None, traceback=True):
+    k = K()
+    inp = k.upload_pairs(qs, ts, offs, DEV, alphabet="ref")
+    t = k.resolve_tile(hb, *tile) if tile else None
+    fwd = k.sw_forward(inp, *scoring, hb, traceback=traceback, tile=t)
+    out = dict(score=fwd.score.cpu().numpy(), qend=fwd.query_end.cpu().numpy(),
+               tend=fwd.target_end.cpu().numpy())
+    if traceback:
+        tbk = k.sw_traceback(inp, fwd)
+        out["cigar"] = k.decode_cigars(tbk)
+        out["rs"] = tbk.read_start.cpu().numpy()
+        out["ts"] = tbk.ref_start.cpu().numpy()
+        out["counts"] = tbk.counts.cpu().numpy()
+    return out
+
+
+def _check_against_oracle(qs, ts, offs, hb, scoring, got, label):
+    ma, mi, go, ge = scoring
+    exp = O.sw_align_pairs(qs, ts, offs, hb, ma, mi, go, ge, threads=8)
+    for p, e in enumerate(exp):
+        ctx = (label, p, len(qs[p]), len(ts[p]), int(offs[p]))
+        assert int(got["score"][p]) == e["score"], ctx
+        assert int(got["qend"][p]) == e["read_end"], ctx
+        assert int(got["tend"][p]) == e["ref_end"], ctx
+        if "cigar" in got:
+            assert got["cigar"][p] == e["cigar"], ctx
+            assert int(got["rs"][p]) == e["read_start"], ctx
+            assert int(got["ts"][p]) == e["ref_start"], ctx
+            assert list(got["counts"][p]) == [e["n_match"], e["n_mismatch"], e["n_insertion"],
+                                               e["n_deletion"]], ctx
+
+
+def test_library_loads_on_device():
+    _say("case: library loads on device")
+    k = K()
+    assert len(k.list_tiles()) > 0
+    assert torch.cuda.is_available()
+
+
+@pytest.mark.parametrize("scoring", [(2, 4, 4, 2), (2, 4, 6, 2), (2, 4, 2, 2), (1, 3, 5, 1)])
+def test_sw_random_pairs_default_tile(scoring):
+    rng = np.random.default_rng(sum(scoring))
+    for hb in (2, 7, 32, 64, 100):
+        _say("case: default tile", scoring, "hb", hb)
+        qs, ts, offs = [], [], []
+        for p in range(24):
+            q, t = _random_pair(rng, int(rng.integers(1, 260)), with_n=(p % 5 == 0))
+            qs.append(q)
+            ts.append(t)
+            offs.append(int(rng.integers(-6, 25)))
+        got = _gpu_align(qs, ts, offs, hb, scoring)
+        _check_against_oracle(qs, ts, offs, hb, scoring, got, f"hb{hb}")
+
+
+def test_sw_every_compiled_tile():
+    rng = np.random.default_rng(77)
+    scoring = (2, 4, 4, 2)
+    for C, G in K().list_tiles():
+        cap = C * G
+        # exact fit (if odd), partial last lane, and a band far narrower than the tile
+        hbs = sorted({(cap - 1) // 2, max(1, (cap - 1) // 2 - 3), max(1, cap // 4)})
+        for hb in hbs:
+            _say("case: tile", C, G, "hb", hb)
+            qs, ts, offs = [], [], []
+            for p in range(12):
+                q, t = _random_pair(rng, int(rng.integers(1, 200)), with_n=(p % 4 == 1))
+                qs.append(q)
+                ts.append(t)
+                offs.append(int(rng.integers(-3, 30)))
+            got = _gpu_align(qs, ts, offs, hb, scoring, tile=(C, G))
+            _check_against_oracle(qs, ts, offs, hb, scoring, got, f"tile{C}x{G}hb{hb}")
+
+
+def test_sw_short_config_full():
+    """1,024 x 150 bp pairs, hb 32 at offset 32, scoring 2/-4/-4/-2 (whole config)."""
+    from graphmambaformer_amd import workloads
+
+    _say("case: sw_short full config")
+    w = workloads.sw_short()
+    qs = [w.as_ref_codes(k)[0] for k in range(w.n_pairs)]
+    ts = [w.as_ref_codes(k)[1] for k in range(w.n_pairs)]
+    sc = (w.match, w.mismatch, w.gap_open, w.gap_extend)
+    got = _gpu_align(qs, ts, w.band_offset, w.half_band, sc)
+    _check_against_oracle(qs, ts, w.band_offset, w.half_band, sc, got, "sw_short")
+
+
+def test_sw_long_pairs_sample():
+    """A few long pairs at band half-width 256 (long-read configuration shape)."""
+    from graphmambaformer_amd import workloads
+
+    _say("case: sw_long sample")
+    rng = np.random.default_rng(2)
+    qs, ts = [], []
+    for L in (5000, 6100, 7777):
+        tmpl = rng.integers(0, 4, size=L).astype(np.uint8)
+        qs.append(workloads.mutate(rng, tmpl, 0.005, 0.7, 2).astype(np.int16) + 1)
+        ts.append(tmpl.astype(np.int16) + 1)
+    got = _gpu_align(qs, ts, [0, 0, 0], 256, (2, 4, 4, 2))
+    _check_against_oracle(qs, ts, [0, 0, 0], 256, (2, 4, 4, 2), got, "sw_long")
+
+
+def test_golden_sw_ties_with_batch_xdrop():
+    """Reference golden vectors, incl. batch-wide x-drop via per-row keys."""
+    k = K()
+    for n, case in enumerate(load("sw_ties.json")):
+        _say("case: golden sw_ties", n)
+        ma, mi, go, ge = case["scoring"]
+        qs = [np.array(r[:L], np.int16) for r, L in zip(case["query"], case["qlen"])]
+        ts = [np.array(r[:L], np.int16) for r, L in zip(case["target"], case["tlen"])]
+        inp = k.upload_pairs(qs, ts, case["off"], DEV, alphabet="ref")
+        hb = case["half_band"]
+        fwd = k.sw_forward(inp, ma, mi, go, ge, hb, traceback=True, rowkeys=True)
+        sc, qe, te = k.xdrop_best(fwd, inp.qlen_host, case["x_drop"], hb, inp.boff_host)
+        dev = inp.device
+        tbk = k.sw_traceback(inp, fwd, torch.from_numpy(sc).to(dev), torch.from_numpy(qe).to(dev),
+                             torch.from_numpy(te).to(dev))
+        cig = k.decode_cigars(tbk)
+        rs = tbk.read_start.cpu().numpy()
+        tsr = tbk.ref_start.cpu().numpy()
+        for b, exp in enumerate(case["out"]):
+            assert int(sc[b]) == int(exp["score"]), (n, b)
+            assert int(qe[b]) == exp["query_end"], (n, b)
+            assert int(te[b]) == exp["target_end"], (n, b)
+            assert [list(x) for x in cig[b]] == [list(x) for x in exp["cigar"]], (n, b)
+            assert int(rs[b]) == exp["read_start"] and int(tsr[b]) == exp["ref_start"], (n, b)
+
+
+def test_chain_dp_f64_golden():
+    k = K()
+    data = load("chain_dp.json")
+    cases = data["cases"]
+    _say("case: chain f64 golden", len(cases))
+    for n, case in enumerate(cases):
+        lut = O.penalty_lut(case["gap_open"], case["gap_extend"], case["log_coeff"],
+                            case["max_gap"])
+        d = torch.device(DEV)
+        re_ = torch.tensor(case["read_end"], dtype=torch.int64, device=d)
+        fe = torch.tensor(case["ref_end"], dtype=torch.int64, device=d)
+        w = torch.tensor(fhex(case["weight"]), dtype=torch.float64, device=d)
+        seg = torch.tensor([0, len(case["read_end"])], dtype=torch.int64, device=d)
+        f, par = k.chain_dp(re_, fe, w, seg, case["max_lookback"], case["max_gap"],
+                            pen_lut=torch.from_numpy(lut).to(d))
+        assert np.array_equal(f.cpu().numpy(), fhex(case["f"])), n
+        assert par.cpu().numpy().tolist() == case["parent"], n
+
+
+def test_chain_dp_i32_segments_vs_oracle():
+    from graphmambaformer_amd import workloads
+
+    _say("case: chain i32 segments")
+    k = K()
+    w = workloads.chains(n_reads=96, seed=1)
+    d = torch.device(DEV)
+    f, par = k.chain_dp(torch.from_numpy(w.read_end).to(d), torch.from_numpy(w.ref_end).to(d),
+                        torch.from_numpy(w.weight).to(d), torch.from_numpy(w.seg_off).to(d),
+                        w.max_lookback, w.max_gap, gap_open=w.gap_open, gap_extend=w.gap_extend)
+    f = f.cpu().numpy()
+    par = par.cpu().numpy()
+    for s in range(w.n_reads):
+        a, b = w.seg_off[s], w.seg_off[s + 1]
+        ef, ep = O.chain_dp_i32(w.read_end[a:b], w.ref_end[a:b], w.weight[a:b], w.max_lookback,
+                                w.max_gap, w.gap_open, w.gap_extend)
+        assert np.array_equal(f[a:b], ef), s
+        assert np.array_equal(par[a:b], ep), s

@@ -1,0 +1,84 @@
+"""CPU checks of the C ABI: the library loads and exports every declared symbol,
+host-side entry points work without a GPU."""
+
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "gmfa.h")) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gmfa_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    from graphmambaformer_amd import _lib
+
+    assert set(declared_symbols()) == set(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    from graphmambaformer_amd import _lib
+
+    L = _lib.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    assert L.gmfa_abi_version() == 1
+
+
+def test_tile_resolution_and_tb_bytes():
+    from graphmambaformer_amd import kernels as K
+
+    t = K.resolve_tile(256)
+    assert (t.cells_per_lane, t.lanes_per_pair, t.lanes_stored) == (9, 64, 57)
+    t = K.resolve_tile(32)
+    assert t.cells_per_lane * t.lanes_per_pair >= 65
+    for hb in (0, 1, 5, 64, 200, 511, 751, 1023):
+        t = K.resolve_tile(hb)
+        assert t.cells_per_lane * t.lanes_per_pair >= 2 * hb + 1
+        assert t.lanes_stored == -(-(2 * hb + 1) // t.cells_per_lane)
+    from graphmambaformer_amd._lib import lib
+
+    assert lib().gmfa_sw_tb_bytes(13, t) == 2 * t.lanes_stored * t.cells_per_lane * 4
+    assert K.tb_bytes(np.array([13]), t)[0] == 2 * t.lanes_stored * t.cells_per_lane * 4
+
+
+def test_invalid_arguments_raise_value_error():
+    import pytest
+
+    from graphmambaformer_amd import kernels as K
+
+    with pytest.raises(ValueError):
+        K.resolve_tile(5000)
+    with pytest.raises(RuntimeError):
+        K.resolve_tile(300, 5, 16)  # tile too small for the band
+
+
+def test_host_backtrack_matches_oracle():
+    from graphmambaformer_amd import kernels as K
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(4)
+    for _ in range(30):
+        n = int(rng.integers(0, 80))
+        f = rng.integers(0, 60, size=n).astype(np.float64)
+        par = np.array([int(rng.integers(-1, i)) if i else -1 for i in range(n)], np.int64)
+        got = K.chain_backtrack(f, par, 1, 10.0)
+        exp = O.chain_backtrack(f, par, 1, 10.0)
+        assert [(m.tolist(), s) for m, s in got] == [(m.tolist(), s) for m, s in exp]
+
+
+def test_pack_host_layout():
+    from graphmambaformer_amd import kernels as K
+
+    seqs = [np.array([1, 2, 3, 4, 5, -1, 1], np.int16), np.array([4] * 17, np.int16)]
+    w, m, off, has_n = K.pack_host(seqs)
+    assert has_n and off.tolist() == [0, 16]
+    assert (w[0] & 0xFF) == 0b11100100  # A C G T -> 0 1 2 3
+    assert (m[0] >> 8) & 0xF == 0b0101  # N and padding masked
+    assert w[1] == 0xFFFFFFFF and (w[2] & 3) == 3
