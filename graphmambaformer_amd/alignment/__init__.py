@@ -1,0 +1,11 @@
+"""Drop-in stage modules (chaining + extension) of the alignment path."""
+
+from .chaining import AffineChainer, chain_dp_batched, chain_dp_numpy  # noqa: F401
+from .extension import (  # noqa: F401
+    BandedDPResult,
+    ExtensionEngine,
+    banded_affine_sw_batch,
+    traceback_banded,
+)
+from .seeding import N_CODE, decode_bases, encode_bases, reverse_complement_codes  # noqa: F401
+from .types import AnchorSet, Chain, ExtensionResult, merge_cigar, run_length_encode  # noqa: F401

@@ -1,0 +1,181 @@
+"""GPU tests of the drop-in API (same signatures as the reference's
+extension.py / chaining.py / accel/cuda_kernels.py), against the reference's
+own golden vectors and known answers."""
+
+import numpy as np
+import pytest
+import torch
+
+from tests.golden.load import fhex, load
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def test_kat_tier_functions():
+    """tests/test_accel.py:288-371 of the reference, on the HIP tier."""
+    from graphmambaformer_amd.accel import banded_sw, chain_dp
+    from graphmambaformer_amd.alignment import encode_bases
+
+    _say("case: tier KATs")
+    q = torch.as_tensor(encode_bases("ACGT"), dtype=torch.int8, device=DEV)[None]
+    t = torch.as_tensor(encode_bases("TTACGTAA"), dtype=torch.int8, device=DEV)[None]
+    s, qe, te = banded_sw(q, t, torch.tensor([4], device=DEV), torch.tensor([8], device=DEV),
+                          half_band=2, match_score=2.0, mismatch_penalty=4.0, gap_open=6.0,
+                          gap_extend=2.0, band_offset=torch.tensor([2], device=DEV))
+    assert float(s[0]) == 8.0 and int(qe[0]) == 4 and int(te[0]) == 6
+    anchors = torch.tensor([[[10, 10, 0], [20, 20, 0], [30, 31, 0]]], dtype=torch.int32,
+                           device=DEV)
+    f, p = chain_dp(anchors, torch.tensor([3], dtype=torch.int32, device=DEV), lookback=64,
+                    max_gap=5000, gap_open=6.0, gap_extend=0.05, log_coeff=0.5,
+                    weights=torch.tensor([[10.5, 9.25, 8.75]], dtype=torch.float32, device=DEV))
+    assert np.allclose(f[0].cpu().numpy(), [10.5, 19.75, 21.95], atol=1e-5)
+    assert p[0].cpu().numpy().tolist() == [-1, 0, 1]
+
+
+def test_banded_sw_drop_in_identical_and_full_band():
+    """test_alignment_stages.py:999-1048 of the reference on the drop-in."""
+    from graphmambaformer_amd.alignment import banded_affine_sw_batch, traceback_banded
+    from graphmambaformer_amd.config import ExtensionConfig
+    from oracle import brute
+
+    _say("case: identical / full band")
+    cfg = ExtensionConfig()
+    rng = np.random.default_rng(3)
+    seq = rng.integers(1, 5, size=40).astype(np.int16)
+    r = banded_affine_sw_batch(torch.as_tensor(seq, device=DEV)[None],
+                               torch.as_tensor(seq, device=DEV)[None], torch.tensor([40]),
+                               torch.tensor([40]), cfg, half_band=8, return_matrices=True)
+    assert float(r.score[0]) == 80.0
+    ext = traceback_banded(r, 0, seq, seq, cfg)
+    assert ext.cigar == [("=", 40)] and ext.edit_distance == 0
+    for trial in range(6):
+        n = int(rng.integers(20, 45))
+        a = rng.integers(1, 5, size=n).astype(np.int16)
+        b = a.copy()
+        for _ in range(2):
+            b[int(rng.integers(0, n))] = int(rng.integers(1, 5))
+        r = banded_affine_sw_batch(torch.as_tensor(a, device=DEV)[None],
+                                   torch.as_tensor(b, device=DEV)[None], torch.tensor([n]),
+                                   torch.tensor([n]), cfg, half_band=n)
+        assert float(r.score[0]) == brute.full_gotoh_score(a, b, 2, 4, 6, 2), trial
+
+
+def test_banded_sw_drop_in_golden_ties_xdrop():
+    from graphmambaformer_amd.alignment import banded_affine_sw_batch, traceback_banded
+    from graphmambaformer_amd.config import ExtensionConfig
+
+    for n, case in enumerate(load("sw_ties.json")):
+        _say("case: drop-in golden", n)
+        ma, mi, go, ge = case["scoring"]
+        cfg = ExtensionConfig(match_score=float(ma), mismatch_penalty=float(mi),
+                              gap_open=float(go), gap_extend=float(ge), x_drop=case["x_drop"])
+        q = torch.tensor(case["query"], dtype=torch.int16, device=DEV)
+        t = torch.tensor(case["target"], dtype=torch.int16, device=DEV)
+        r = banded_affine_sw_batch(q, t, torch.tensor(case["qlen"]), torch.tensor(case["tlen"]),
+                                   cfg, half_band=case["half_band"],
+                                   band_offset=torch.tensor(case["off"]), return_matrices=True)
+        for b, exp in enumerate(case["out"]):
+            assert float(r.score[b]) == exp["score"], (n, b)
+            assert int(r.query_end[b]) == exp["query_end"], (n, b)
+            assert int(r.target_end[b]) == exp["target_end"], (n, b)
+            ext = traceback_banded(r, b, None, None, cfg)
+            assert [list(x) for x in ext.cigar] == [list(x) for x in exp["cigar"]], (n, b)
+            assert ext.read_start == exp["read_start"] and ext.ref_start == exp["ref_start"]
+
+
+def test_extension_engine_golden():
+    from graphmambaformer_amd.alignment import AnchorSet, Chain, ExtensionEngine
+    from graphmambaformer_amd.config import ExtensionConfig
+
+    for n, case in enumerate(load("extend.json")):
+        _say("case: ExtensionEngine golden", n)
+        a = case["anchors"]
+        anchors = AnchorSet.from_lists(a["read_pos"], a["ref_pos"], a["length"], a["strand"],
+                                       read_len=len(case["read"]), ref_len=len(case["ref"]))
+        chains = [Chain(anchor_idx=np.array(c["anchor_idx"]), score=50.0, strand=c["strand"],
+                        read_start=c["read_start"], read_end=c["read_end"],
+                        ref_start=c["ref_start"], ref_end=c["ref_end"]) for c in case["chains"]]
+        cfg = ExtensionConfig(half_band=case["cfg"]["half_band"], flank=case["cfg"]["flank"],
+                              x_drop=case["cfg"]["x_drop"])
+        out = ExtensionEngine(cfg, device=DEV).extend_chains(case["read"], chains, anchors,
+                                                             case["ref"])
+        for got, exp in zip(out, case["out"]):
+            assert got.score == exp["score"], n
+            assert [list(x) for x in got.cigar] == [list(x) for x in exp["cigar"]], n
+            assert (got.read_start, got.read_end, got.ref_start, got.ref_end) == (
+                exp["read_start"], exp["read_end"], exp["ref_start"], exp["ref_end"]), n
+
+
+def test_chain_dp_numpy_and_batched_golden():
+    from graphmambaformer_amd.alignment import chain_dp_batched, chain_dp_numpy
+    from graphmambaformer_amd.config import ChainingConfig
+
+    data = load("chain_dp.json")
+    _say("case: chain_dp_numpy golden")
+    for n, case in enumerate(data["cases"]):
+        cfg = ChainingConfig(max_lookback=case["max_lookback"], max_gap=case["max_gap"])
+        f, p = chain_dp_numpy(np.array(case["read_end"]), np.array(case["ref_end"]),
+                              fhex(case["weight"]), cfg)
+        assert np.array_equal(f, fhex(case["f"])), n
+        assert p.tolist() == case["parent"], n
+    # ragged batch (test_alignment_stages.py:502-517): rows equal the per-read DP
+    cfg = ChainingConfig(max_lookback=4)
+    re_ = torch.tensor([[10, 20, 30], [8, 18, 0]], dtype=torch.float32)
+    fe = torch.tensor([[10, 21, 32], [8, 20, 0]], dtype=torch.float32)
+    w = torch.tensor([[10.5, 9.25, 8.75], [7.5, 8.25, 0.0]])
+    f, p = chain_dp_batched(re_, fe, w, torch.tensor([3, 2]), cfg)
+    assert np.allclose(f[0].numpy(), [10.5, 13.2, 15.4]) and p[0].tolist() == [-1, 0, 1]
+    assert np.isclose(float(f[1, 1]), 7.5 + 8.25 - (6.1 + 0.5 * np.log2(3.0)))
+    assert p[1].tolist() == [-1, 0, -1] and float(f[1, 2]) == 0.0
+
+
+def test_affine_chainer_golden():
+    from graphmambaformer_amd.alignment import AffineChainer, AnchorSet
+    from graphmambaformer_amd.config import ChainingConfig
+
+    cases = load("chainer.json")
+    _say("case: AffineChainer golden", len(cases))
+    sets, cfgs = [], []
+    for case in cases:
+        sets.append(AnchorSet.from_lists(case["read_pos"], case["ref_pos"], case["length"],
+                                         case["strand"], read_len=case["read_len"]))
+        cfgs.append(ChainingConfig(max_lookback=case["max_lookback"],
+                                   min_chain_score=case["min_chain_score"]))
+    for n, (case, anchors, cfg) in enumerate(zip(cases, sets, cfgs)):
+        got = AffineChainer(cfg).chain(anchors)
+        exp = case["chains"]
+        assert len(got) == len(exp), n
+        for g, e in zip(got, exp):
+            assert g.anchor_idx.tolist() == e["anchor_idx"], n
+            assert g.score == float.fromhex(e["score"]), n
+            assert (g.strand, g.read_start, g.read_end, g.ref_start, g.ref_end, g.is_primary) == (
+                e["strand"], e["read_start"], e["read_end"], e["ref_start"], e["ref_end"],
+                e["is_primary"]), n
+    # a batch of reads sharing one launch gives the same chains as read by read
+    same = [c for c in cfgs if c.max_lookback == 64 and c.min_chain_score == 20.0]
+    idx = [k for k, c in enumerate(cfgs) if c in same]
+    batch = AffineChainer(cfgs[idx[0]]).chain_batch([sets[k] for k in idx])
+    for k, chains in zip(idx, batch):
+        single = AffineChainer(cfgs[k]).chain(sets[k])
+        assert [(c.anchor_idx.tolist(), c.score) for c in chains] == \
+            [(c.anchor_idx.tolist(), c.score) for c in single]
+
+
+def test_invalid_inputs_raise():
+    from graphmambaformer_amd.alignment import banded_affine_sw_batch
+    from graphmambaformer_amd.config import ExtensionConfig
+
+    _say("case: invalid inputs")
+    q = torch.tensor([[1, 2, 0, 4]], dtype=torch.int16, device=DEV)  # code 0 is not a base
+    with pytest.raises(ValueError):
+        banded_affine_sw_batch(q, q, torch.tensor([4]), torch.tensor([4]), ExtensionConfig(),
+                               half_band=2)
+    q = torch.tensor([[1, 2, 3, 4]], dtype=torch.int16, device=DEV)
+    with pytest.raises(ValueError):
+        banded_affine_sw_batch(q, q, torch.tensor([4]), torch.tensor([4]),
+                               ExtensionConfig(match_score=1.5), half_band=2)
