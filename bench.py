@@ -58,7 +58,12 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-extras", action="store_true")
-    ap.add_argument("--cpu-sample-stride", type=int, default=64)
+    ap.add_argument("--walk", choices=["separate", "fused"], default="fused",
+                    help="CIGAR walk as its own launch (one wavefront per pair) or inside "
+                         "the forward launch")
+    ap.add_argument("--cpu-sample-stride", type=int, default=32)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -72,6 +77,14 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        x = torch.ones(1 << 28, dtype=torch.int32, device=dev)  # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 3}), flush=True)
+        return
 
     # ------------------------------------------------------------------ batch
     t0 = time.time()
@@ -100,17 +113,23 @@ def main():
         f"{tile.cells_per_lane}x{tile.lanes_per_pair}")
 
     stream = torch.cuda.current_stream()
-    fwd_ev, tb_ev = [], []
+    fwd_ev = []
+
+    fused = args.walk == "fused"
+    tb_ev = []
 
     def step(record):
+        # forward rows + 4-bit planes, then every pair's CIGAR walk: either a
+        # second launch with one wavefront per pair ("separate") or inside the
+        # forward launch right after each pair's last row ("fused")
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         c = torch.cuda.Event(enable_timing=True)
         a.record(stream)
         fwd = K.sw_forward(inp, *sc, w.half_band, traceback=True, order=order, tile=tile,
-                           workspace=ws)
+                           workspace=ws, cigar=fused)
         b.record(stream)
-        tbk = K.sw_traceback(inp, fwd, workspace=ws)
+        tbk = fwd.walk if fused else K.sw_traceback(inp, fwd, workspace=ws)
         c.record(stream)
         if record:
             fwd_ev.append((a, b))
@@ -132,7 +151,7 @@ def main():
     tb_ms = float(np.mean([a.elapsed_time(b) for a, b in tb_ev]))
     gcups = total_cells / (ms_step / 1e3) / 1e9
     log(f"[bench] timed {args.steps} steps: {ms_step:.1f} ms/step (forward {fwd_ms:.1f} ms, "
-        f"traceback {tb_ms:.1f} ms) -> {gcups:.1f} GCUPS")
+        f"walk {tb_ms:.1f} ms, {args.walk}) -> {gcups:.1f} GCUPS")
 
     # algorithmic traffic of the forward kernel: 0.5 B/cell of traceback planes
     # + the 2-bit sequences read once
@@ -160,9 +179,11 @@ def main():
             "cells": total_cells,
             "half_band": w.half_band,
             "tile": [tile.cells_per_lane, tile.lanes_per_pair],
-            "step": "sw_forward (scores, ends, 4-bit planes) + traceback walk to CIGAR",
+            "step": "sw_forward (scores, ends, 4-bit planes) + CIGAR walk of every pair "
+                    f"({args.walk})",
         },
-        "kernels_ms": {"sw_forward": round(fwd_ms, 3), "sw_traceback": round(tb_ms, 3)},
+        "kernels_ms": {"sw_forward": round(fwd_ms, 3), "sw_walk": round(tb_ms, 3),
+                       "walk_mode": args.walk},
         "roofline": {
             "bound": "hbm",
             "achieved": round(hbm_gbs, 2),
@@ -194,7 +215,8 @@ def main():
 
         def per_pair():
             for one in inps:
-                f1 = K.sw_forward(one, *sc, w.half_band, traceback=True, tile=tile, workspace=ws1)
+                f1 = K.sw_forward(one, *sc, w.half_band, traceback=True, tile=tile,
+                                  workspace=ws1)
                 K.sw_traceback(one, f1)
 
         per_pair()
@@ -205,7 +227,7 @@ def main():
         pp_s = time.perf_counter() - t1
         result["per_pair_gpu"] = {"value": round(sub_cells / pp_s / 1e9, 3), "unit": "GCUPS",
                                   "sample": f"{len(sub)} pairs of sw_long (every 256th), one "
-                                            "forward + traceback launch pair per pair"}
+                                            "forward launch + one walk launch per pair"}
         result["batched_over_per_pair"] = round(gcups / (sub_cells / pp_s / 1e9), 1)
         log(f"[bench] per-pair GPU form: {result['per_pair_gpu']['value']} GCUPS")
 

@@ -24,6 +24,7 @@ GMFA_E_UNSUPPORTED = -3
 SW_TRACEBACK = 1
 SW_NMASK = 2
 SW_ROWKEYS = 4
+SW_CIGAR = 8
 
 OP_I, OP_D, OP_S, OP_EQ, OP_X = 1, 2, 4, 7, 8
 OP_CHAR = {OP_I: "I", OP_D: "D", OP_S: "S", OP_EQ: "=", OP_X: "X"}
@@ -48,7 +49,9 @@ class SwBatch(ctypes.Structure):
 
 class SwOut(ctypes.Structure):
     _fields_ = [("score", _P), ("query_end", _P), ("target_end", _P), ("tb", _P),
-                ("tb_off", _P), ("rowkey", _P), ("rowkey_off", _P), ("work_counter", _P)]
+                ("tb_off", _P), ("rowkey", _P), ("rowkey_off", _P), ("work_counter", _P),
+                ("cigar_off", _P), ("cigar", _P), ("n_cigar", _P), ("read_start", _P),
+                ("ref_start", _P), ("counts", _P)]
 
 
 class SwTile(ctypes.Structure):

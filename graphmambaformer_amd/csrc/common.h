@@ -52,17 +52,19 @@ __device__ __forceinline__ int dpp(int old, int src)
 
 __device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
-// Inclusive max-scan over segments of G lanes (G in {16, 32, 64}); lanes
-// outside a segment's prefix see kNeg.
+// Inclusive max-scan over segments of G lanes (G in {16, 32, 64}).  The DPP
+// "old" operand is INT_MIN, the identity of signed max, which lets the
+// compiler fuse each step into a single v_max_i32_dpp.
 template <int G>
 __device__ __forceinline__ int seg_scan_max(int x)
 {
-    x = max(x, dpp<kRowShr1>(kNeg, x));
-    x = max(x, dpp<kRowShr2>(kNeg, x));
-    x = max(x, dpp<kRowShr4>(kNeg, x));
-    x = max(x, dpp<kRowShr8>(kNeg, x));
-    if constexpr (G >= 32) x = max(x, dpp<kRowBcast15, 0xa>(kNeg, x));
-    if constexpr (G >= 64) x = max(x, dpp<kRowBcast31, 0xc>(kNeg, x));
+    constexpr int id = (int)0x80000000;
+    x = max(x, dpp<kRowShr1>(id, x));
+    x = max(x, dpp<kRowShr2>(id, x));
+    x = max(x, dpp<kRowShr4>(id, x));
+    x = max(x, dpp<kRowShr8>(id, x));
+    if constexpr (G >= 32) x = max(x, dpp<kRowBcast15, 0xa>(id, x));
+    if constexpr (G >= 64) x = max(x, dpp<kRowBcast31, 0xc>(id, x));
     return x;
 }
 

@@ -19,16 +19,17 @@
 //   * the target bases of a lane shift by one per row; they come from a
 //     2-bit packed window loaded once per 8 rows, the query base is
 //     wave-uniform (SGPR) when G = 64.
-// Values are kept in the row-shifted domain X~(i) = X(i) - i*match, which turns
-// the diagonal step into H~(i-1,d) - mismatch_bit*(match+mismatch) (one mad)
-// and the zero clamp into a per-row scalar.  All arithmetic is exact int32;
+// Values are kept in the row-shifted domain X'(i) = X(i) + i*mismatch, which
+// turns the diagonal step into H'(i-1,d) + match_bit*(match+mismatch) (one
+// v_mad_u32_u24) and the zero clamp into a per-row scalar.  All arithmetic is exact int32;
 // -inf is kNeg with enough headroom that it never wraps.
 //
 // Traceback planes
 // ----------------
-// 4 bits per cell, 8 rows per block; lane l's C dwords of a block hold, row by
-// row, C "F continues" bits then (not-diagonal, not-F, E continues) for each
-// cell -- exactly the decisions traceback_banded makes (extension.py:249-283):
+// 4 bits per cell, 8 rows per block.  Dword k of lane l's C dwords of a block
+// holds the 8 rows of diagonal d = l*C + k, row r in bits 31-4r .. 28-4r as
+// (f_cont, not_diag, not_f, e_cont) -- exactly the decisions traceback_banded
+// makes (extension.py:249-283):
 //   not_diag = H != max(H(i-1,j-1),0) + sub        (diag has priority)
 //   not_f    = H != F
 //   f_cont   = F(i-1,j) - e > H(i-1,j) - o          (stay in the vertical gap)
@@ -37,7 +38,8 @@
 // d-1 beats d-1), which equals the reference's test whenever open > extend;
 // for open == extend the reference never continues a horizontal gap and the
 // walker ignores the bit.  Bits are shifted into the dwords with v_alignbit
-// from sign bits, so a block's dword holds 32 decisions, first one in bit 31.
+// from sign bits.  A diagonal run of the walk stays inside one dword for up to
+// 8 rows.
 //
 // Best cell: the reference keeps the first row with a strict improvement and
 // the first column of that row (extension.py:178-182), i.e. the
@@ -49,6 +51,18 @@
 #include <climits>
 
 namespace gmfa {
+
+struct WalkCtx {
+    const uint32_t *qseq, *tseq, *qmask, *tmask;
+    int64_t q_words, t_words;
+    int32_t nm, hb, W, Gs, ma, mi, go, ge;
+};
+
+struct WalkOut {
+    const int64_t *cig_off;
+    uint32_t *cigar;
+    int32_t *ncig, *rstart, *tstart, *counts;
+};
 
 struct SwKArgs {
     const uint32_t *qseq, *tseq, *qmask, *tmask;
@@ -63,6 +77,9 @@ struct SwKArgs {
     int32_t *rowkey;
     const int64_t *rk_off;
     uint32_t *counter;
+    int32_t walk;  // walk every pair right after its forward pass (CIGAR out)
+    WalkCtx wc;
+    WalkOut wo;
 };
 
 __device__ __forceinline__ int64_t clampi64(int64_t x, int64_t hi)
@@ -97,7 +114,228 @@ __device__ __forceinline__ int wave_max(int x)
     return x;
 }
 
-template <int C, int G, bool TB, bool NM, bool RK>
+// a * b + c on 24-bit unsigned operands in one VALU op.  Written as asm
+// because the compiler proves a is 0/1 and would lower the product to a
+// compare + select + add.
+__device__ __forceinline__ int mad_u24(uint32_t a, uint32_t b, int c)
+{
+    int r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+__device__ __forceinline__ int64_t readlane64(int64_t v, int l)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// -------------------------------------------------------------------------
+// traceback walk (extension.py:249-285 decisions replayed from the planes)
+// -------------------------------------------------------------------------
+// Decision nibble of cell (i, d) of a pair's planes: bit 3 f_cont, bit 2
+// not_diag, bit 1 not_f, bit 0 e_cont.
+template <int C>
+__device__ __forceinline__ int64_t plane_word(int i, int d, int Gs)
+{
+    const int l = d / C;
+    return ((int64_t)((i - 1) >> 3) * Gs + l) * C + (d - l * C);
+}
+
+// 32 bases of a 2-bit stream starting at base e (two cached words).
+struct SeqWindow {
+    int64_t w = -2;
+    uint32_t lo = 0u, hi = 0u;
+};
+
+template <typename Ld>
+__device__ __forceinline__ uint32_t seq_bits(const uint32_t *src, int64_t words, int64_t e,
+                                             SeqWindow &c, Ld ld)
+{
+    const int64_t w = e >> 4;
+    if (w != c.w) {
+        if (w == c.w - 1) {  // walking backwards: slide the window down one word
+            c.hi = c.lo;
+            c.lo = ld(src, w);
+        } else {
+            c.lo = ld(src, w);
+            c.hi = (w + 1 < words) ? ld(src, w + 1) : 0u;
+        }
+        c.w = w;
+    }
+    return __builtin_amdgcn_alignbit(c.hi, c.lo, (uint32_t)(e & 15) * 2u);
+}
+
+// The walk (extension.py:243-285) shared by both walkers.  `ld(ptr, idx)`
+// loads a dword (plane or sequence); `put(x, run)` stores CIGAR run x.
+// Diagonal moves are taken in bulk: the not_diag bits of the rows below the
+// current one in the same plane dword give the run length (up to 8 rows),
+// one XOR of 2-bit base windows gives its matches and mismatches, and the
+// tracked H only has to stay positive, which h > (m-1)*match guarantees.
+// The plane dword of the same diagonal one block up is prefetched.
+template <int C, typename Ld, typename Put>
+__device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int h, int off,
+                                           int64_t qo, int64_t to, const uint32_t *tb, Ld ld,
+                                           Put put, int &nr, int (&cnt)[4])
+{
+    const bool econt_ok = c.go != c.ge;
+    const int blkstride = c.Gs * C;  // dwords between vertically adjacent blocks
+    int state = 0;  // 0 H, 1 F (I), 2 E (D)
+    int run_op = -1, run_len = 0;
+    int64_t cur = -1, pf_addr = -1;
+    uint32_t v = 0u, pf = 0u;
+    SeqWindow qwin, twin, qmwin, tmwin;
+    auto emit = [&](int op, int n) {
+        if (op == run_op) {
+            run_len += n;
+        } else {
+            if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
+            run_op = op;
+            run_len = n;
+        }
+    };
+    while (i > 0 && j > 0) {
+        const int d = j - i - off + c.hb;
+        if (d < 0 || d >= c.W) break;  // unreachable for planes this library wrote
+        const int64_t addr = plane_word<C>(i, d, c.Gs);
+        if (addr != cur) {
+            v = (addr == pf_addr) ? pf : ld(tb, addr);
+            cur = addr;
+            pf_addr = (i > 8) ? addr - blkstride : -1;
+            if (pf_addr >= 0) pf = ld(tb, pf_addr);
+        }
+        const int r0 = (i - 1) & 7;
+        const uint32_t nib = (v >> (28 - 4 * r0)) & 15u;
+        if (state == 0) {
+            if (h <= 0) break;
+            if (!(nib & 4u)) {
+                // rows r0, r0-1, ... of this dword while not_diag stays 0
+                const uint32_t y = (v & 0x44444444u) & (~0u << (30 - 4 * r0));
+                int m = y ? r0 - ((30 - (int)__builtin_ctz(y)) >> 2) : r0 + 1;
+                m = min(m, min(i, j));
+                if (h <= (m - 1) * c.ma) m = 1;
+                const uint32_t lenmask = (m >= 16) ? ~0u : ((1u << (2 * m)) - 1u);
+                const int64_t eq = qo + i - m, et = to + j - m;
+                uint32_t x = seq_bits(c.qseq, c.q_words, eq, qwin, ld) ^
+                             seq_bits(c.tseq, c.t_words, et, twin, ld);
+                x = (x | (x >> 1)) & 0x55555555u;
+                if (c.nm)
+                    x |= (seq_bits(c.qmask, c.q_words, eq, qmwin, ld) |
+                          seq_bits(c.tmask, c.t_words, et, tmwin, ld)) & 0x55555555u;
+                x &= lenmask;  // bit 2t: base i-m+t mismatches
+                const int nmis = __builtin_popcount(x);
+                h -= c.ma * (m - nmis) - c.mi * nmis;
+                cnt[0] += m - nmis;
+                cnt[1] += nmis;
+                if (x == 0u) {
+                    emit(GMFA_OP_EQ, m);
+                } else {
+                    for (int t = m - 1; t >= 0; --t)
+                        emit(((x >> (2 * t)) & 1u) ? GMFA_OP_X : GMFA_OP_EQ, 1);
+                }
+                i -= m;
+                j -= m;
+            } else {
+                state = (nib & 2u) ? 2 : 1;
+            }
+        } else if (state == 1) {
+            emit(GMFA_OP_I, 1);
+            cnt[2]++;
+            --i;
+            if (nib & 8u) h += c.ge;
+            else { h += c.go; state = 0; }
+        } else {
+            emit(GMFA_OP_D, 1);
+            cnt[3]++;
+            --j;
+            if (econt_ok && (nib & 1u)) h += c.ge;
+            else { h += c.go; state = 0; }
+        }
+    }
+    if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
+}
+
+// Fused-path walk of one pair by a whole wavefront (state wave-uniform).
+template <int C>
+__device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, int p, int i, int j,
+                                          int h, int off, int64_t qo, int64_t to,
+                                          const uint32_t *tb)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    uint32_t *out = o.cigar + o.cig_off[p];
+    int nr = 0;
+    int cnt[4] = {0, 0, 0, 0};
+    auto ld = [&](const uint32_t *src, int64_t idx) {
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)src[idx]);
+    };
+    auto put = [&](int x, uint32_t vrun) {
+        if (lane == 0) out[x] = vrun;
+    };
+    walk_steps<C>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int x = lane; x < nr / 2; x += 64) {
+        const int y = nr - 1 - x;
+        const uint32_t va = out[x], vb = out[y];
+        out[x] = vb;
+        out[y] = va;
+    }
+    if (lane == 0) {
+        o.ncig[p] = nr;
+        o.rstart[p] = i;
+        o.tstart[p] = j;
+        o.counts[4 * p + 0] = cnt[0];
+        o.counts[4 * p + 1] = cnt[1];
+        o.counts[4 * p + 2] = cnt[2];
+        o.counts[4 * p + 3] = cnt[3];
+    }
+}
+
+struct TbKArgs {
+    WalkCtx c;
+    WalkOut o;
+    const int64_t *q_off, *t_off;
+    const int32_t *boff;
+    int32_t n_pairs;
+    const int32_t *score, *qend, *tend;
+    const uint32_t *tb;
+    const int64_t *tb_off;
+};
+
+// Standalone walk: one lane per pair (64 pairs per wavefront), the current
+// plane dword and 16-base query / target words cached in registers.
+template <int C>
+__global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
+{
+    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= a.n_pairs) return;
+    const WalkCtx &c = a.c;
+    int i = a.qend[p], j = a.tend[p];
+    const int h = a.score[p], off = a.boff[p];
+    const int64_t qo = a.q_off[p], to = a.t_off[p];
+    const uint32_t *tb = a.tb + a.tb_off[p];
+    uint32_t *out = a.o.cigar + a.o.cig_off[p];
+    int nr = 0;
+    int cnt[4] = {0, 0, 0, 0};
+    auto ld = [&](const uint32_t *src, int64_t idx) { return src[idx]; };
+    auto put = [&](int x, uint32_t vrun) { out[x] = vrun; };
+    walk_steps<C>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    for (int x = 0, y = nr - 1; x < y; ++x, --y) {
+        const uint32_t t = out[x];
+        out[x] = out[y];
+        out[y] = t;
+    }
+    a.o.ncig[p] = nr;
+    a.o.rstart[p] = i;
+    a.o.tstart[p] = j;
+    a.o.counts[4 * p + 0] = cnt[0];
+    a.o.counts[4 * p + 1] = cnt[1];
+    a.o.counts[4 * p + 2] = cnt[2];
+    a.o.counts[4 * p + 3] = cnt[3];
+}
+
+template <int C, int G, int KS, bool TB, bool NM, bool RK>
 __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 {
     constexpr int NCW = (C + 15) / 16;  // target words feeding the cells
@@ -109,12 +347,15 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 
     const int W = a.W, hb = a.hb, ma = a.ma, mi = a.mi, go = a.go, ge = a.ge;
     const int lstar = (W - 1) / C;  // group lane holding diagonal W-1
-    const int kstar = (W - 1) % C;
+    // register of diagonal W-1 in that lane: compile time when KS >= 0 (the
+    // host picks such a tile only if (W-1) % C == KS)
+    const int kstar = KS >= 0 ? KS : (W - 1) % C;
     const bool dead_hi = gl >= lstar;                     // registers k > kstar dead
     const bool guard = (kstar == C - 1) && (gl == lstar); // its d+1 is the band guard
     const bool lane_live = gl <= lstar;
     const int lCe = gl * C * ge;
-    const int cH = go + ma, cF = ge + ma, mms = ma + mi;
+    const int cH = go - mi, cF = ge - mi;
+    const uint32_t mms = (uint32_t)(ma + mi);
     const int64_t t_hi = a.t_words - 1, q_hi = a.q_words - 1;
 
     for (;;) {
@@ -149,7 +390,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         const int64_t e_q = qo;  // query element of row 1
 
         uint32_t wt[NWR], wn[NWR];
-        auto load_t = [&](int64_t e, uint32_t (&w)[NWR], const uint32_t *src) {
+        auto load_t = [&](int64_t e, uint32_t (&w)[NWR], const uint32_t *src) __attribute__((always_inline)) {
             const int64_t wb = e >> 4;
 #pragma unroll
             for (int x = 0; x < NWR; ++x) w[x] = src[clampi64(wb + x, t_hi)];
@@ -159,8 +400,23 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             if (NM) load_t(e_t, wn, a.tmask);
         }
 
+        // Blocks whose 8 rows have every band cell inside the matrix form one
+        // contiguous range [fb0, fb1); the edge blocks around it are masked.
+        int fb0 = ((hb + 1 - off) - 1 + 7) >> 3;  // first b with 8b+1 >= hb+1-off
+        fb0 = max(fb0, 0);
+        int fb1 = min(qlen, tlen - hb - off);     // rows <= fb1 are fast-capable
+        fb1 = fb1 >= 8 ? (fb1 >> 3) : 0;          // blocks with 8b+8 <= that bound
+        if (!has) { fb0 = 0; fb1 = nblk; }
+        if constexpr (G < 64) {
+            fb0 = wave_max(fb0);
+            fb1 = -wave_max(-fb1);
+        }
+        fb1 = min(max(fb1, fb0), nblk);
+        fb0 = min(fb0, fb1);
         for (int b = 0; b < nblk; ++b) {
             const int i0 = 8 * b + 1;
+            int rkeys[8];
+            (void)rkeys;
             // align this block's windows, then prefetch the next block
             uint32_t ta[NA], na[NA];
             {
@@ -191,14 +447,11 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 }
             }
 
-            const bool fast_l = (i0 >= hb + 1 - off) && (i0 + 7 <= min(qlen, tlen - hb - off));
-            const bool fast = (G == 64) ? fast_l : wave_all(fast_l || !has);
-
-            auto row = [&](auto slow_tag, const int r) {
+            auto row = [&](auto slow_tag, const int r) __attribute__((always_inline)) {
                 constexpr bool SLOW = decltype(slow_tag)::value;
                 const int i = i0 + r;
-                const int zrow = -i * ma;
-                const int zprev = zrow + ma;
+                const int zrow = i * mi;  // H = 0 in this row's domain
+                const int zprev = zrow - mi;
                 const uint32_t qc = (qbits >> (2 * r)) & 3u;
                 const uint32_t qpat = qc * 0x55555555u;
                 const bool qn = NM && ((qnbits >> (2 * r)) & 1u);
@@ -211,7 +464,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                         y |= __builtin_amdgcn_alignbit(na[x + 1], na[x], 2 * r);
                         y = qn ? ~0u : y;
                     }
-                    z[x] = y;
+                    z[x] = ~y & 0x55555555u;  // 2-bit field k = 1 <=> cell k matches
                 }
                 int lo = 0;
                 uint32_t span = 0;
@@ -228,7 +481,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 hN = guard ? zprev : hN;
                 fN = guard ? kNeg : fN;
 
-                int Mv[C], Fv[C], Dg[C], Av[C], Pv[C];
+                int Mv[C], Fv[C], Dg[C], Av[C], Gv[C];
 #pragma unroll
                 for (int k = 0; k < C; ++k) {
                     const int uh = (k + 1 < C) ? Hp[(k + 1) % C] : hN;
@@ -236,14 +489,9 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     const int av = uh - cH;
                     const int bv = uf - cF;
                     Fv[k] = max(av, bv);
-                    if (TB) {
-                        constexpr int dummy = 0;
-                        (void)dummy;
-                        const int n = r * 4 * C + k;
-                        acc[n >> 5] = insert_sign(acc[n >> 5], av - bv);
-                    }
-                    const uint32_t mm = __builtin_amdgcn_ubfe(z[k >> 4], 2 * (k & 15), 1);
-                    Dg[k] = Hp[k] - __mul24((int)mm, mms);
+                    if (TB) acc[k] = insert_sign(acc[k], av - bv);  // f_cont
+                    const uint32_t mt = __builtin_amdgcn_ubfe(z[k >> 4], 2 * (k & 15), 2);
+                    Dg[k] = mad_u24(mt, mms, Hp[k]);
                     const int m = max(max(Dg[k], Fv[k]), zrow);
                     Mv[k] = m;
                     int A = m + k * ge;
@@ -252,19 +500,26 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                         A = live ? A : kNeg;
                     }
                     Av[k] = A;
-                    Pv[k] = k ? max(Pv[k ? k - 1 : 0], A) : A;
                 }
-                int X = from_prev_lane(seg_scan_max<G>(Pv[C - 1] + lCe));
+                // lane total max_k A_k as a max3 tree, then the cross-lane scan
+                int T = Av[0];
+#pragma unroll
+                for (int k = 1; k < C; k += 2) T = (k + 1 < C) ? max(max(T, Av[k]), Av[(k + 1) % C]) : max(T, Av[k]);
+                int X = from_prev_lane(seg_scan_max<G>(T + lCe));
                 if constexpr (G < 64) X = (gl == 0) ? kNeg : X;
                 const int Xl = X - lCe;
-                const int Glast = (C > 1) ? max(Xl, Pv[C > 1 ? C - 2 : 0]) : Xl;
+                // exclusive prefix inside the lane: G_0 = X, G_{k+1} = max(G_k, A_k)
+                Gv[0] = Xl;
+#pragma unroll
+                for (int k = 1; k < C; ++k) Gv[k] = max(Gv[k - 1], Av[k - 1]);
+                const int Glast = Gv[C - 1];
                 const int dprev = from_prev_lane(Av[C - 1] - Glast);
 
                 int rk = INT_MIN;
                 int Gkm1 = Xl;
 #pragma unroll
                 for (int k = 0; k < C; ++k) {
-                    const int Gk = k ? max(Xl, Pv[k ? k - 1 : 0]) : Xl;
+                    const int Gk = Gv[k];
                     const int E = Gk - (go + (k - 1) * ge);
                     int H = max(Mv[k], E);
                     int Fk = Fv[k];
@@ -273,17 +528,17 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                         H = live ? H : zrow;
                         Fk = live ? Fk : kNeg;
                     } else {
+                        // registers past diagonal W-1 of lane lstar (and of the
+                        // dead lanes) must read as out of band for the guard
                         if (k > kstar) {
                             H = dead_hi ? zrow : H;
                             Fk = dead_hi ? kNeg : Fk;
                         }
                     }
                     if (TB) {
-                        const int n = r * 4 * C + C + 3 * k;
-                        acc[n >> 5] = insert_sign(acc[n >> 5], Dg[k] - H);
-                        acc[(n + 1) >> 5] = insert_sign(acc[(n + 1) >> 5], Fk - H);
-                        acc[(n + 2) >> 5] =
-                            insert_sign(acc[(n + 2) >> 5], k ? (Av[k ? k - 1 : 0] - Gkm1) : dprev);
+                        acc[k] = insert_sign(acc[k], Dg[k] - H);  // not_diag
+                        acc[k] = insert_sign(acc[k], Fk - H);     // not_f
+                        acc[k] = insert_sign(acc[k], k ? (Av[k ? k - 1 : 0] - Gkm1) : dprev);  // e_cont
                     }
                     rk = max(rk, (int)(((uint32_t)H << 5) | (uint32_t)(31 - k)));
                     Hp[k] = H;  // Hp[k] is not read again in this row
@@ -292,27 +547,33 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 }
 #pragma unroll
                 for (int k = 0; k < C; ++k) Fp[k] = Fv[k];
-                const int tk = rk + (int)((uint32_t)(i * ma) << 5);
+                const int tk = rk - (int)((uint32_t)(i * mi) << 5);
                 if (tk > (bestK | 31)) { bestK = tk; besti = i; }
                 if constexpr (RK) {
-                    // lane-local (max H, first d) of this row; the host takes the
-                    // max over a row's lanes (keys order by H, then by smaller d)
-                    if (has && lane_live && i <= qlen) {
-                        const int htrue = (rk >> 5) + i * ma;
-                        const int d = gl * C + 31 - (rk & 31);
-                        a.rowkey[rk_base + (int64_t)(i - 1) * (lstar + 1) + gl] =
-                            (int)(((uint32_t)htrue << 11) | (uint32_t)(2047 - d));
-                    }
+                    // lane-local (max H, first d) of this row, stored per block;
+                    // the host takes the max over a row's lanes (keys order by H,
+                    // then by smaller d)
+                    const int htrue = (rk >> 5) - i * mi;
+                    const int d = gl * C + 31 - (rk & 31);
+                    rkeys[r] = (int)(((uint32_t)htrue << 11) | (uint32_t)(2047 - d));
                 }
             };
             // Hp[k] is overwritten in pass 2 only after pass 1 consumed it, and
             // Hp[k+1] (read by cell k) is consumed in pass 1 as well.
-            if (fast) {
+            // interior blocks (every band cell of the 8 rows inside the matrix)
+            // skip the per-cell masking
+            if (b >= fb0 && b < fb1) {
 #pragma unroll
                 for (int r = 0; r < 8; ++r) row(std::false_type{}, r);
             } else {
 #pragma unroll
                 for (int r = 0; r < 8; ++r) row(std::true_type{}, r);
+            }
+            if (RK && has && lane_live) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    if (i0 + r <= qlen)
+                        a.rowkey[rk_base + (int64_t)(i0 + r - 1) * (lstar + 1) + gl] = rkeys[r];
             }
             if (TB && has && lane_live && b < nblk_mine) {
                 uint32_t *dst = a.tb + tb_base + ((int64_t)b * (lstar + 1) + gl) * C;
@@ -335,123 +596,42 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             const uint64_t other = ((uint64_t)hi32 << 32) | lo32;
             key = other > key ? other : key;
         }
+        int res_s = 0, res_i = 0, res_j = 0;
+        if (key != 0) {
+            res_s = (int)(key >> 40);
+            res_i = (int)(0xFFFFFu - (uint32_t)((key >> 12) & 0xFFFFFu));
+            res_j = res_i + off - hb + (int)(0xFFFu - (uint32_t)(key & 0xFFFu));
+        }
         if (gl == 0 && has) {
-            if (key == 0) {
-                a.score[p] = 0; a.qend[p] = 0; a.tend[p] = 0;
-            } else {
-                const int H = (int)(key >> 40);
-                const int bi = (int)(0xFFFFFu - (uint32_t)((key >> 12) & 0xFFFFFu));
-                const int dd = (int)(0xFFFu - (uint32_t)(key & 0xFFFu));
-                a.score[p] = H;
-                a.qend[p] = bi;
-                a.tend[p] = bi + off - hb + dd;
+            a.score[p] = res_s;
+            a.qend[p] = res_i;
+            a.tend[p] = res_j;
+        }
+        if constexpr (TB && !RK) {
+            if (a.walk) {
+                // this wave's plane stores must be visible to its own loads
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const WalkCtx &wc = a.wc;
+                const WalkOut &wo = a.wo;
+#pragma unroll 1
+                for (int g = 0; g < 64 / G; ++g) {
+                    const int L0 = g * G;
+                    if (!__builtin_amdgcn_readlane((int)has, L0)) continue;
+                    const int pg = __builtin_amdgcn_readlane(p, L0);
+                    const int offg = __builtin_amdgcn_readlane(off, L0);
+                    const int64_t qog = readlane64(qo, L0), tog = readlane64(to, L0);
+                    const int64_t tbg = readlane64(tb_base, L0);
+                    walk_pair<C>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
+                                 __builtin_amdgcn_readlane(res_j, L0),
+                                 __builtin_amdgcn_readlane(res_s, L0), offg, qog, tog, a.tb + tbg);
+                }
             }
         }
     }
 }
 
 __global__ void reset_counter_kernel(uint32_t *c) { *c = 0u; }
-
-// -------------------------------------------------------------------------
-// traceback walk: one thread per pair
-// -------------------------------------------------------------------------
-struct TbKArgs {
-    const uint32_t *qseq, *tseq, *qmask, *tmask;
-    const int64_t *q_off, *t_off;
-    const int32_t *boff;
-    int32_t n_pairs, hb, W, C, Gs, ma, mi, go, ge;
-    int32_t nm;
-    const int32_t *score, *qend, *tend;
-    const uint32_t *tb;
-    const int64_t *tb_off;
-    const int64_t *cig_off;
-    uint32_t *cigar;
-    int32_t *ncig, *rstart, *tstart, *counts;
-};
-
-__device__ __forceinline__ uint32_t base2(const uint32_t *s, int64_t e)
-{
-    return (s[e >> 4] >> ((uint32_t)(e & 15) * 2u)) & 3u;
-}
-
-__global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
-{
-    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p >= a.n_pairs) return;
-    const int C = a.C, Gs = a.Gs, hb = a.hb, W = a.W;
-    const int off = a.boff[p];
-    const int64_t qo = a.q_off[p], to = a.t_off[p];
-    const uint32_t *tb = a.tb + a.tb_off[p];
-    uint32_t *out = a.cigar + a.cig_off[p];
-    const bool econt_ok = a.go != a.ge;
-    int i = a.qend[p], j = a.tend[p];
-    int h = a.score[p];
-    int state = 0;  // 0 H, 1 F (I), 2 E (D)
-    int run_op = -1, run_len = 0, nr = 0;
-    int cnt[4] = {0, 0, 0, 0};
-    auto emit = [&](int op) {
-        if (op == run_op) {
-            ++run_len;
-        } else {
-            if (run_len) out[nr++] = ((uint32_t)run_len << 4) | (uint32_t)run_op;
-            run_op = op;
-            run_len = 1;
-        }
-    };
-    while (i > 0 && j > 0) {
-        const int d = j - i - off + hb;
-        if (d < 0 || d >= W) break;  // unreachable for planes this library wrote
-        const int blk = (i - 1) >> 3, r = (i - 1) & 7, l = d / C, k = d - l * C;
-        const uint32_t *cell = tb + ((int64_t)blk * Gs + l) * C;
-        const int nf = r * 4 * C + k;
-        const int nd = r * 4 * C + C + 3 * k;
-        auto bit = [&](int n) { return (cell[n >> 5] >> (31 - (n & 31))) & 1u; };
-        if (state == 0) {
-            if (h <= 0) break;
-            if (!bit(nd)) {
-                const int64_t eq = qo + i - 1, et = to + j - 1;
-                bool match = base2(a.qseq, eq) == base2(a.tseq, et);
-                if (a.nm) match = match && !(base2(a.qmask, eq) & 1u) && !(base2(a.tmask, et) & 1u);
-                emit(match ? GMFA_OP_EQ : GMFA_OP_X);
-                cnt[match ? 0 : 1]++;
-                h -= match ? a.ma : -a.mi;
-                --i;
-                --j;
-            } else if (!bit(nd + 1)) {
-                state = 1;
-            } else {
-                state = 2;
-            }
-        } else if (state == 1) {
-            const bool cont = bit(nf);
-            emit(GMFA_OP_I);
-            cnt[2]++;
-            --i;
-            if (cont) h += a.ge;
-            else { h += a.go; state = 0; }
-        } else {
-            const bool cont = econt_ok && bit(nd + 2);
-            emit(GMFA_OP_D);
-            cnt[3]++;
-            --j;
-            if (cont) h += a.ge;
-            else { h += a.go; state = 0; }
-        }
-    }
-    if (run_len) out[nr++] = ((uint32_t)run_len << 4) | (uint32_t)run_op;
-    for (int x = 0, y = nr - 1; x < y; ++x, --y) {
-        const uint32_t t = out[x];
-        out[x] = out[y];
-        out[y] = t;
-    }
-    a.ncig[p] = nr;
-    a.rstart[p] = i;
-    a.tstart[p] = j;
-    a.counts[4 * p + 0] = cnt[0];
-    a.counts[4 * p + 1] = cnt[1];
-    a.counts[4 * p + 2] = cnt[2];
-    a.counts[4 * p + 3] = cnt[3];
-}
 
 // -------------------------------------------------------------------------
 // int16 codes -> 2-bit stream + N mask; one thread per output word
@@ -490,51 +670,69 @@ __global__ void pack_codes_kernel(const int16_t *codes, int64_t L, const int32_t
 // -------------------------------------------------------------------------
 using SwKernel = void (*)(SwKArgs);
 
-template <int C, int G>
+template <int C, int G, int KS>
 SwKernel pick_kernel(uint32_t flags)
 {
     const bool tb = flags & GMFA_SW_TRACEBACK, nm = flags & GMFA_SW_NMASK, rk = flags & GMFA_SW_ROWKEYS;
-    if (rk && tb && nm) return sw_forward_kernel<C, G, true, true, true>;
-    if (rk && tb) return sw_forward_kernel<C, G, true, false, true>;
-    if (rk && nm) return sw_forward_kernel<C, G, false, true, true>;
-    if (rk) return sw_forward_kernel<C, G, false, false, true>;
-    if (tb && nm) return sw_forward_kernel<C, G, true, true, false>;
-    if (tb) return sw_forward_kernel<C, G, true, false, false>;
-    if (nm) return sw_forward_kernel<C, G, false, true, false>;
-    return sw_forward_kernel<C, G, false, false, false>;
+    if (rk && tb && nm) return sw_forward_kernel<C, G, KS, true, true, true>;
+    if (rk && tb) return sw_forward_kernel<C, G, KS, true, false, true>;
+    if (rk && nm) return sw_forward_kernel<C, G, KS, false, true, true>;
+    if (rk) return sw_forward_kernel<C, G, KS, false, false, true>;
+    if (tb && nm) return sw_forward_kernel<C, G, KS, true, true, false>;
+    if (tb) return sw_forward_kernel<C, G, KS, true, false, false>;
+    if (nm) return sw_forward_kernel<C, G, KS, false, true, false>;
+    return sw_forward_kernel<C, G, KS, false, false, false>;
 }
 
+// KS >= 0: the tile serves only bands with (W-1) % C == KS (compile-time dead
+// registers; KS == C-1 is an exact fit with none).  KS == -1: any band.
 struct TileEntry {
-    int C, G;
+    int C, G, KS;
     SwKernel (*get)(uint32_t);
+    void (*walk)(TbKArgs);
 };
 
-#define GMFA_TILE(C, G) {C, G, &pick_kernel<C, G>}
+#define GMFA_TILE(C, G, KS) {C, G, KS, &pick_kernel<C, G, KS>, &sw_traceback_kernel<C>}
 static const TileEntry kTiles[] = {
-    GMFA_TILE(2, 16),  GMFA_TILE(5, 16),  GMFA_TILE(8, 16),
-    GMFA_TILE(4, 32),  GMFA_TILE(8, 32),
-    GMFA_TILE(2, 64),  GMFA_TILE(4, 64),  GMFA_TILE(8, 64),  GMFA_TILE(9, 64),
-    GMFA_TILE(12, 64), GMFA_TILE(16, 64), GMFA_TILE(24, 64), GMFA_TILE(32, 64),
+    // exact fits of the benchmark bands: W=65 (5x13), W=513 (9x57); W=1503 (24x63, 15 live
+    // registers in the last lane)
+    GMFA_TILE(5, 16, 4),   GMFA_TILE(9, 64, 8),  GMFA_TILE(24, 64, 14),
+    // general tiles, any band up to C*G
+    GMFA_TILE(2, 16, -1),  GMFA_TILE(5, 16, -1), GMFA_TILE(8, 16, -1),
+    GMFA_TILE(4, 32, -1),  GMFA_TILE(8, 32, -1),
+    GMFA_TILE(2, 64, -1),  GMFA_TILE(4, 64, -1), GMFA_TILE(8, 64, -1),  GMFA_TILE(9, 64, -1),
+    GMFA_TILE(12, 64, -1), GMFA_TILE(16, 64, -1), GMFA_TILE(24, 64, -1), GMFA_TILE(32, 64, -1),
 };
 #undef GMFA_TILE
 
-static const TileEntry *find_tile(int C, int G)
+static bool tile_serves(const TileEntry &t, int W)
 {
-    for (const auto &t : kTiles)
-        if (t.C == C && t.G == G) return &t;
-    return nullptr;
+    if (t.C * t.G < W) return false;
+    return t.KS < 0 || (W - 1) % t.C == t.KS;
 }
 
-// default: the compiled tile with the least padding, fewer lanes per pair
-// first (more pairs per wavefront), then more cells per lane
+// explicit (C, G): the specialised variant for this band if one exists, else
+// the general one
+static const TileEntry *find_tile(int C, int G, int W)
+{
+    const TileEntry *gen = nullptr;
+    for (const auto &t : kTiles) {
+        if (t.C != C || t.G != G) continue;
+        if (t.KS >= 0 && tile_serves(t, W)) return &t;
+        if (t.KS < 0) gen = &t;
+    }
+    return gen;
+}
+
+// default: the tile with the least padding (live cells / lane slots), then
+// specialised before general, then more cells per lane
 static const TileEntry *default_tile(int W)
 {
     const TileEntry *best = nullptr;
     double best_eff = -1.0;
     for (const auto &t : kTiles) {
-        if (t.C * t.G < W) continue;
-        const int lanes = (W + t.C - 1) / t.C;
-        const double eff = (double)W / (double)(t.G * t.C) * (lanes <= t.G ? 1.0 : 0.0);
+        if (!tile_serves(t, W)) continue;
+        const double eff = (double)W / (double)(t.G * t.C) + (t.KS >= 0 ? 1e-3 : 0.0);
         if (eff > best_eff + 1e-9) { best = &t; best_eff = eff; }
     }
     return best;
@@ -552,7 +750,7 @@ extern "C" int gmfa_sw_resolve_tile(int32_t half_band, int32_t cells_per_lane,
     const int W = 2 * half_band + 1;
     const TileEntry *t = nullptr;
     if (cells_per_lane > 0 || lanes_per_pair > 0) {
-        t = find_tile(cells_per_lane, lanes_per_pair);
+        t = find_tile(cells_per_lane, lanes_per_pair, W);
         if (!t) {
             set_error("no compiled SW tile with %d cells per lane and %d lanes per pair",
                       cells_per_lane, lanes_per_pair);
@@ -578,10 +776,16 @@ extern "C" int gmfa_sw_resolve_tile(int32_t half_band, int32_t cells_per_lane,
 
 extern "C" int gmfa_sw_list_tiles(int32_t *cells_per_lane, int32_t *lanes_per_pair, int32_t cap)
 {
-    const int n = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
-    for (int k = 0; k < n && k < cap; ++k) {
-        if (cells_per_lane) cells_per_lane[k] = kTiles[k].C;
-        if (lanes_per_pair) lanes_per_pair[k] = kTiles[k].G;
+    // general tiles only: each (C, G) once; specialised variants are picked
+    // automatically for the bands they serve
+    int n = 0;
+    for (const auto &t : kTiles) {
+        if (t.KS >= 0) continue;
+        if (n < cap) {
+            if (cells_per_lane) cells_per_lane[n] = t.C;
+            if (lanes_per_pair) lanes_per_pair[n] = t.G;
+        }
+        ++n;
     }
     return n;
 }
@@ -623,6 +827,20 @@ static int resident_waves(SwKernel k)
     return cus * nb;
 }
 
+static WalkCtx make_walk_ctx(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
+                             const gmfa_sw_tile &tile, bool nm)
+{
+    WalkCtx c;
+    c.qseq = batch->qseq; c.tseq = batch->tseq;
+    c.q_words = batch->q_words; c.t_words = batch->t_words;
+    c.qmask = nm ? batch->qmask : nullptr; c.tmask = nm ? batch->tmask : nullptr;
+    c.nm = nm ? 1 : 0;
+    c.hb = params->half_band; c.W = 2 * params->half_band + 1;
+    c.Gs = tile.lanes_stored;
+    c.ma = params->match; c.mi = params->mismatch; c.go = params->gap_open; c.ge = params->gap_extend;
+    return c;
+}
+
 extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
                                const gmfa_sw_out *out, void *stream)
 {
@@ -646,8 +864,8 @@ extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params 
     GMFA_REQUIRE(batch->q_words > 0 && batch->t_words > 0, "empty sequence streams");
     GMFA_REQUIRE(batch->max_qlen >= 0 && batch->max_qlen <= (1 << 20),
                  "query length bound %d outside [0, 2^20]", batch->max_qlen);
-    GMFA_REQUIRE((int64_t)batch->max_qlen * P.match < (1 << 24),
-                 "query length x match score must stay below 2^24 for the int32 DP");
+    GMFA_REQUIRE((int64_t)batch->max_qlen * (P.match + P.mismatch) < (1 << 24),
+                 "query length x (match + mismatch) must stay below 2^24 for the int32 DP");
     GMFA_REQUIRE(!(P.flags & GMFA_SW_ROWKEYS) || (int64_t)batch->max_qlen * P.match < (1 << 19),
                  "per-row keys need query length x match below 2^19");
     const bool nm = P.flags & GMFA_SW_NMASK;
@@ -658,7 +876,7 @@ extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params 
                  "GMFA_SW_TRACEBACK needs tb and tb_off");
     GMFA_REQUIRE(!(P.flags & GMFA_SW_ROWKEYS) || (out->rowkey && out->rowkey_off),
                  "GMFA_SW_ROWKEYS needs rowkey and rowkey_off");
-    const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair);
+    const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, 2 * P.half_band + 1);
     SwKernel k = t->get(P.flags);
 
     SwKArgs a;
@@ -675,6 +893,20 @@ extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params 
     a.tb = out->tb; a.tb_off = out->tb_off;
     a.rowkey = out->rowkey; a.rk_off = out->rowkey_off;
     a.counter = out->work_counter;
+    a.walk = 0;
+    a.wc = make_walk_ctx(batch, params, tile, nm);
+    a.wo = WalkOut{};
+    if (P.flags & GMFA_SW_CIGAR) {
+        GMFA_REQUIRE(P.flags & GMFA_SW_TRACEBACK, "GMFA_SW_CIGAR needs GMFA_SW_TRACEBACK");
+        GMFA_REQUIRE(!(P.flags & GMFA_SW_ROWKEYS),
+                     "GMFA_SW_CIGAR walks the unadjusted best cell; x-drop batches use gmfa_sw_traceback");
+        GMFA_REQUIRE(out->cigar_off && out->cigar && out->n_cigar && out->read_start &&
+                         out->ref_start && out->counts,
+                     "GMFA_SW_CIGAR needs the CIGAR outputs");
+        a.walk = 1;
+        a.wo.cig_off = out->cigar_off; a.wo.cigar = out->cigar; a.wo.ncig = out->n_cigar;
+        a.wo.rstart = out->read_start; a.wo.tstart = out->ref_start; a.wo.counts = out->counts;
+    }
 
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
@@ -705,16 +937,15 @@ extern "C" int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_param
     const bool nm = params->flags & GMFA_SW_NMASK;
     GMFA_REQUIRE(!nm || (batch->qmask && batch->tmask), "GMFA_SW_NMASK needs qmask and tmask");
     TbKArgs a;
-    a.qseq = batch->qseq; a.tseq = batch->tseq; a.qmask = batch->qmask; a.tmask = batch->tmask;
+    a.c = make_walk_ctx(batch, params, tile, nm);
+    a.o.cig_off = cigar_off; a.o.cigar = cigar; a.o.ncig = n_cigar;
+    a.o.rstart = read_start; a.o.tstart = ref_start; a.o.counts = counts;
     a.q_off = batch->q_off; a.t_off = batch->t_off; a.boff = batch->band_offset;
-    a.n_pairs = n; a.hb = params->half_band; a.W = 2 * params->half_band + 1;
-    a.C = tile.cells_per_lane; a.Gs = tile.lanes_stored;
-    a.ma = params->match; a.mi = params->mismatch; a.go = params->gap_open; a.ge = params->gap_extend;
-    a.nm = nm ? 1 : 0;
+    a.n_pairs = n;
     a.score = score; a.qend = query_end; a.tend = target_end;
-    a.tb = tb; a.tb_off = tb_off; a.cig_off = cigar_off; a.cigar = cigar;
-    a.ncig = n_cigar; a.rstart = read_start; a.tstart = ref_start; a.counts = counts;
-    hipLaunchKernelGGL(sw_traceback_kernel, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, a);
+    a.tb = tb; a.tb_off = tb_off;
+    const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, a.c.W);
+    hipLaunchKernelGGL(t->walk, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }

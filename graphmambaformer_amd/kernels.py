@@ -189,6 +189,7 @@ class SwForward:
     rowkey_off_host: Optional[np.ndarray]
     tile: SwTile
     params: SwParams
+    walk: Optional["SwTraceback"] = None  # CIGARs when walked inside the forward launch
 
 
 def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile) -> SwParams:
@@ -201,10 +202,30 @@ def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile) -> SwParams:
     return p
 
 
+def _cigar_buffers(inp: SwInputs, ws: dict):
+    n = inp.n_pairs
+    dev = inp.device
+    if "cigar" in ws:
+        cig, cig_off = ws["cigar"], ws["cigar_off"]
+    else:
+        cap = inp.qlen_host.astype(np.int64) + inp.tlen_host.astype(np.int64) + 1
+        offs = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(cap, out=offs[1:])
+        cig = torch.empty(max(int(offs[-1]), 1), dtype=torch.int32, device=dev)
+        cig_off = torch.from_numpy(offs[:-1]).to(dev)
+    return SwTraceback(cigar=cig, cigar_off=cig_off,
+                       n_cigar=torch.empty(n, dtype=torch.int32, device=dev),
+                       read_start=torch.empty(n, dtype=torch.int32, device=dev),
+                       ref_start=torch.empty(n, dtype=torch.int32, device=dev),
+                       counts=torch.empty((n, 4), dtype=torch.int32, device=dev))
+
+
 def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int,
                traceback: bool = True, rowkeys: bool = False,
                order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
-               workspace: Optional[dict] = None) -> SwForward:
+               workspace: Optional[dict] = None, cigar: bool = False) -> SwForward:
+    """Forward pass.  ``cigar=True`` also walks every pair inside the same
+    launch (GMFA_SW_CIGAR) and returns the CIGARs in ``.walk``."""
     dev = inp.device
     n = inp.n_pairs
     tile = tile or resolve_tile(half_band)
@@ -215,6 +236,10 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
         flags |= _lib.SW_NMASK
     if rowkeys:
         flags |= _lib.SW_ROWKEYS
+    if cigar:
+        if not traceback or rowkeys:
+            raise ValueError("cigar=True needs traceback=True and no row keys")
+        flags |= _lib.SW_CIGAR
     ws = workspace if workspace is not None else {}
     score = torch.empty(n, dtype=torch.int32, device=dev)
     qend = torch.empty(n, dtype=torch.int32, device=dev)
@@ -245,12 +270,19 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
     out.tb, out.tb_off = ptr(tb), ptr(tb_off)
     out.rowkey, out.rowkey_off = ptr(rk), ptr(rk_off)
     out.work_counter = ptr(counter)
+    walk = None
+    if cigar:
+        walk = _cigar_buffers(inp, ws)
+        out.cigar_off, out.cigar = ptr(walk.cigar_off), ptr(walk.cigar)
+        out.n_cigar, out.read_start = ptr(walk.n_cigar), ptr(walk.read_start)
+        out.ref_start, out.counts = ptr(walk.ref_start), ptr(walk.counts)
     params = make_params(ma, mi, go, ge, half_band, flags, tile)
     batch = inp.batch_struct(order)
     check(lib().gmfa_sw_forward(ctypes.byref(batch), ctypes.byref(params), ctypes.byref(out),
                                 stream_of(dev)), "gmfa_sw_forward")
+    params.flags = int(flags) & ~_lib.SW_CIGAR  # a later standalone walk uses the same params
     return SwForward(score=score, query_end=qend, target_end=tend, tb=tb, tb_off=tb_off,
-                     rowkey=rk, rowkey_off_host=rk_off_host, tile=tile, params=params)
+                     rowkey=rk, rowkey_off_host=rk_off_host, tile=tile, params=params, walk=walk)
 
 
 @dataclass
@@ -268,20 +300,10 @@ def sw_traceback(inp: SwInputs, fwd: SwForward, score=None, query_end=None, targ
     if fwd.tb is None:
         raise ValueError("traceback needs the forward pass with traceback=True")
     dev = inp.device
-    n = inp.n_pairs
     ws = workspace if workspace is not None else {}
-    if "cigar" in ws:
-        cig, cig_off = ws["cigar"], ws["cigar_off"]
-    else:
-        cap = inp.qlen_host.astype(np.int64) + inp.tlen_host.astype(np.int64) + 1
-        offs = np.zeros(n + 1, dtype=np.int64)
-        np.cumsum(cap, out=offs[1:])
-        cig = torch.empty(max(int(offs[-1]), 1), dtype=torch.int32, device=dev)
-        cig_off = torch.from_numpy(offs[:-1]).to(dev)
-    ncig = torch.empty(n, dtype=torch.int32, device=dev)
-    rs = torch.empty(n, dtype=torch.int32, device=dev)
-    ts = torch.empty(n, dtype=torch.int32, device=dev)
-    counts = torch.empty((n, 4), dtype=torch.int32, device=dev)
+    bufs = _cigar_buffers(inp, ws)
+    cig, cig_off = bufs.cigar, bufs.cigar_off
+    ncig, rs, ts, counts = bufs.n_cigar, bufs.read_start, bufs.ref_start, bufs.counts
     score = fwd.score if score is None else score
     query_end = fwd.query_end if query_end is None else query_end
     target_end = fwd.target_end if target_end is None else target_end

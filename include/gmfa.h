@@ -45,6 +45,8 @@ extern "C" {
 #define GMFA_SW_TRACEBACK 1u  /* write the 4-bit traceback planes */
 #define GMFA_SW_NMASK 2u      /* sequences carry N masks (qmask/tmask valid) */
 #define GMFA_SW_ROWKEYS 4u    /* write per-row (max H, first column) keys */
+#define GMFA_SW_CIGAR 8u      /* walk each pair right after its forward pass and
+                                 write its CIGAR (needs GMFA_SW_TRACEBACK, no ROWKEYS) */
 
 /* BAM CIGAR op codes used in traceback output (len << 4 | op) */
 #define GMFA_OP_I 1
@@ -96,6 +98,13 @@ typedef struct gmfa_sw_out {
                               (max H, first column) */
     const int64_t *rowkey_off; /* [n_pairs] element offset of each pair's keys */
     uint32_t *work_counter;    /* one device uint32 of scratch for the work queue */
+    /* GMFA_SW_CIGAR outputs, same layout as gmfa_sw_traceback's */
+    const int64_t *cigar_off;
+    uint32_t *cigar;
+    int32_t *n_cigar;
+    int32_t *read_start;
+    int32_t *ref_start;
+    int32_t *counts;
 } gmfa_sw_out;
 
 typedef struct gmfa_sw_tile {

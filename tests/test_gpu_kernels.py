@@ -48,15 +48,15 @@ def _random_pair(rng, n, with_n=False):
     return q, t
 
 
-def _gpu_align(qs, ts, offs, hb, scoring, tile=None, traceback=True):
+def _gpu_align(qs, ts, offs, hb, scoring, tile=None, traceback=True, fused=False):
     k = K()
     inp = k.upload_pairs(qs, ts, offs, DEV, alphabet="ref")
     t = k.resolve_tile(hb, *tile) if tile else None
-    fwd = k.sw_forward(inp, *scoring, hb, traceback=traceback, tile=t)
+    fwd = k.sw_forward(inp, *scoring, hb, traceback=traceback, tile=t, cigar=fused)
     out = dict(score=fwd.score.cpu().numpy(), qend=fwd.query_end.cpu().numpy(),
                tend=fwd.target_end.cpu().numpy())
     if traceback:
-        tbk = k.sw_traceback(inp, fwd)
+        tbk = fwd.walk if fused else k.sw_traceback(inp, fwd)
         out["cigar"] = k.decode_cigars(tbk)
         out["rs"] = tbk.read_start.cpu().numpy()
         out["ts"] = tbk.ref_start.cpu().numpy()
@@ -98,7 +98,7 @@ def test_sw_random_pairs_default_tile(scoring):
             qs.append(q)
             ts.append(t)
             offs.append(int(rng.integers(-6, 25)))
-        got = _gpu_align(qs, ts, offs, hb, scoring)
+        got = _gpu_align(qs, ts, offs, hb, scoring, fused=(hb % 2 == 0))
         _check_against_oracle(qs, ts, offs, hb, scoring, got, f"hb{hb}")
 
 
@@ -117,8 +117,27 @@ def test_sw_every_compiled_tile():
                 qs.append(q)
                 ts.append(t)
                 offs.append(int(rng.integers(-3, 30)))
-            got = _gpu_align(qs, ts, offs, hb, scoring, tile=(C, G))
+            got = _gpu_align(qs, ts, offs, hb, scoring, tile=(C, G), fused=(hb % 2 == 1))
             _check_against_oracle(qs, ts, offs, hb, scoring, got, f"tile{C}x{G}hb{hb}")
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("hb", [32, 256, 751])
+def test_sw_specialised_band_tiles(hb, fused):
+    """Bands with a compiled exact / fixed-remainder tile (65 = 5x13, 513 = 9x57,
+    1503 = 24x63 with 15 live registers in the last lane), incl. N codes and
+    pairs shorter than the band."""
+    rng = np.random.default_rng(hb)
+    qs, ts, offs = [], [], []
+    for p in range(10):
+        n = int(rng.integers(1, 3 * hb + 40)) if p % 3 else int(rng.integers(1, 30))
+        q, t = _random_pair(rng, n, with_n=(p % 4 == 2))
+        qs.append(q)
+        ts.append(t)
+        offs.append(int(rng.integers(-5, 25)))
+    _say("case: specialised tile hb", hb, K().resolve_tile(hb).cells_per_lane)
+    got = _gpu_align(qs, ts, offs, hb, (2, 4, 4, 2), fused=fused)
+    _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"spec{hb}")
 
 
 def test_sw_short_config_full():
@@ -130,7 +149,7 @@ def test_sw_short_config_full():
     qs = [w.as_ref_codes(k)[0] for k in range(w.n_pairs)]
     ts = [w.as_ref_codes(k)[1] for k in range(w.n_pairs)]
     sc = (w.match, w.mismatch, w.gap_open, w.gap_extend)
-    got = _gpu_align(qs, ts, w.band_offset, w.half_band, sc)
+    got = _gpu_align(qs, ts, w.band_offset, w.half_band, sc, fused=True)
     _check_against_oracle(qs, ts, w.band_offset, w.half_band, sc, got, "sw_short")
 
 
@@ -145,7 +164,7 @@ def test_sw_long_pairs_sample():
         tmpl = rng.integers(0, 4, size=L).astype(np.uint8)
         qs.append(workloads.mutate(rng, tmpl, 0.005, 0.7, 2).astype(np.int16) + 1)
         ts.append(tmpl.astype(np.int16) + 1)
-    got = _gpu_align(qs, ts, [0, 0, 0], 256, (2, 4, 4, 2))
+    got = _gpu_align(qs, ts, [0, 0, 0], 256, (2, 4, 4, 2), fused=True)
     _check_against_oracle(qs, ts, [0, 0, 0], 256, (2, 4, 4, 2), got, "sw_long")
 
 
