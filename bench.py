@@ -34,6 +34,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(cells: int, fused: bool):
+    """HBM bytes per forward launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (calibrated against a plain 1 GiB copy), if they were
+    taken on this exact batch and walk mode; else None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "sw_forward_sw_long_traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if rec.get("cells") != cells or not fused:
+        return None
+    return rec.get("traffic_bytes_per_launch")
+
+
 def lpt_order(cells: np.ndarray):
     return np.argsort(-cells, kind="stable").astype(np.int32)
 
@@ -190,7 +206,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(hbm_gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(total_cells, fused),
             "kernel": "sw_forward_kernel",
             "note": "the kernel is int32-VALU bound; see roofline_valu",
         },
