@@ -6,11 +6,12 @@
 A step is one pass of the hot path over the whole long-read batch (16,384
 pairs, lengths lognormal median 12 kb clipped 5-25 kb, 0.5% errors, band
 half-width 256, scoring 2/-4/-4/-2): the forward kernel writes scores, ends
-and 4-bit traceback planes, then the walk turns every pair into a CIGAR.
-Inputs are resident in HBM before the timed region.  ``value`` is GCUPS over
-in-band cells.  Beside it the line carries the per-pair-launch GPU form, the
-CPU oracle on the host cores (bounded stratified sample), the short-read
-batch, the chaining DP and the roofline of the dominant kernel.
+and 4-bit traceback planes and walks every pair into a CIGAR.  Inputs are
+resident in HBM before the timed region.  ``value`` is GCUPS over in-band
+cells.  Beside it the line carries the per-pair-launch GPU form, the CPU
+oracle on the host cores (bounded stratified sample), the short-read batch,
+the noisy wide-band batch, the chaining DP, and the roofline of the dominant
+kernel.
 """
 
 from __future__ import annotations
@@ -68,6 +69,90 @@ def time_events(torch, fn, reps):
     return float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
 
+def scoring(w):
+    return (w.match, w.mismatch, w.gap_open, w.gap_extend)
+
+
+class BatchRun:
+    """One workload resident on the device: packed pairs, longest-first
+    order, tile and every buffer preallocated; ``step()`` is one forward
+    launch (+ the walk, fused or as a second launch)."""
+
+    def __init__(self, torch, K, w, dev, fused=True):
+        self.w, self.K, self.fused = w, K, fused
+        self.cells = w.cells()
+        self.total_cells = int(self.cells.sum())
+        self.inp = K.upload_pairs(w.queries, w.targets, w.band_offset, dev, alphabet="2bit")
+        self.order = torch.from_numpy(lpt_order(self.cells)).to(dev)
+        self.tile = K.resolve_tile(w.half_band)
+        self.ws = K.make_workspace(self.inp, self.tile, cigar=True)
+        torch.cuda.synchronize()
+
+    def forward(self):
+        K, w = self.K, self.w
+        return K.sw_forward(self.inp, *scoring(w), w.half_band, traceback=True, order=self.order,
+                            tile=self.tile, workspace=self.ws, cigar=self.fused)
+
+    def walk(self, fwd):
+        return fwd.walk if self.fused else self.K.sw_traceback(self.inp, fwd, workspace=self.ws)
+
+    def step(self):
+        fwd = self.forward()
+        return fwd, self.walk(fwd)
+
+
+def per_pair_gcups(torch, K, w, idx, dev):
+    """The plain per-pair form: one forward launch + one walk launch per pair."""
+    from graphmambaformer_amd import workloads as WL
+
+    sub = WL.subsample(w, idx)
+    tile = K.resolve_tile(w.half_band)
+    ws1 = {"counter": torch.zeros(1, dtype=torch.int32, device=dev)}
+    inps = [K.upload_pairs([sub.queries[k]], [sub.targets[k]], sub.band_offset[k:k + 1], dev,
+                           alphabet="2bit") for k in range(sub.n_pairs)]
+
+    def run():
+        for one in inps:
+            f1 = K.sw_forward(one, *scoring(w), w.half_band, traceback=True, tile=tile,
+                              workspace=ws1)
+            K.sw_traceback(one, f1)
+
+    run()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    s = time.perf_counter() - t1
+    return int(sub.cells().sum()) / s / 1e9
+
+
+def cpu_gcups(w, idx, threads):
+    """The C oracle (forward + traceback, one pair per thread task)."""
+    from oracle import oracle as O
+
+    if not os.path.exists(O._LIB_PATH):
+        O.build()
+    qs = [w.as_ref_codes(int(k))[0] for k in idx]
+    ts = [w.as_ref_codes(int(k))[1] for k in idx]
+    t1 = time.perf_counter()
+    O.sw_align_pairs(qs, ts, w.band_offset[idx], w.half_band, *scoring(w), threads=threads)
+    s = time.perf_counter() - t1
+    return float(w.cells()[idx].sum()) / s / 1e9, s
+
+
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,9 +160,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--walk", choices=["separate", "fused"], default="fused",
-                    help="CIGAR walk as its own launch (one wavefront per pair) or inside "
-                         "the forward launch")
-    ap.add_argument("--cpu-sample-stride", type=int, default=32)
+                    help="CIGAR walk inside the forward launch (fused) or as its own launch "
+                         "(one lane per pair)")
+    ap.add_argument("--cpu-sample-stride", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -105,47 +190,25 @@ def main():
     # ------------------------------------------------------------------ batch
     t0 = time.time()
     w = WL.sw_long()
-    cells = w.cells()
-    total_cells = int(cells.sum())
-    log(f"[bench] sw_long generated: {w.n_pairs} pairs, {total_cells:.3e} cells, "
+    log(f"[bench] sw_long generated: {w.n_pairs} pairs, {w.cells().sum():.3e} cells, "
         f"{time.time() - t0:.1f}s")
-    inp = K.upload_pairs(w.queries, w.targets, w.band_offset, dev, alphabet="2bit")
-    order = torch.from_numpy(lpt_order(cells)).to(dev)
-    tile = K.resolve_tile(w.half_band)
-    sc = (w.match, w.mismatch, w.gap_open, w.gap_extend)
-    ws = {"counter": torch.zeros(1, dtype=torch.int32, device=dev)}
-    nb = K.tb_bytes(inp.qlen_host, tile)
-    offs = np.zeros(w.n_pairs + 1, np.int64)
-    np.cumsum(nb // 4, out=offs[1:])
-    ws["tb"] = torch.empty(int(offs[-1]), dtype=torch.int32, device=dev)
-    ws["tb_off"] = torch.from_numpy(offs[:-1]).to(dev)
-    cap = inp.qlen_host.astype(np.int64) + inp.tlen_host.astype(np.int64) + 1
-    coffs = np.zeros(w.n_pairs + 1, np.int64)
-    np.cumsum(cap, out=coffs[1:])
-    ws["cigar"] = torch.empty(int(coffs[-1]), dtype=torch.int32, device=dev)
-    ws["cigar_off"] = torch.from_numpy(coffs[:-1]).to(dev)
-    torch.cuda.synchronize()
-    log(f"[bench] resident: tb planes {offs[-1] * 4 / 1e9:.1f} GB, tile "
+    fused = args.walk == "fused"
+    run = BatchRun(torch, K, w, dev, fused)
+    total_cells, cells, tile = run.total_cells, run.cells, run.tile
+    log(f"[bench] resident: tb planes {run.ws['tb_bytes'] / 1e9:.1f} GB, tile "
         f"{tile.cells_per_lane}x{tile.lanes_per_pair}")
 
     stream = torch.cuda.current_stream()
-    fwd_ev = []
-
-    fused = args.walk == "fused"
-    tb_ev = []
+    fwd_ev, tb_ev = [], []
 
     def step(record):
-        # forward rows + 4-bit planes, then every pair's CIGAR walk: either a
-        # second launch with one wavefront per pair ("separate") or inside the
-        # forward launch right after each pair's last row ("fused")
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         c = torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        fwd = K.sw_forward(inp, *sc, w.half_band, traceback=True, order=order, tile=tile,
-                           workspace=ws, cigar=fused)
+        fwd = run.forward()
         b.record(stream)
-        tbk = fwd.walk if fused else K.sw_traceback(inp, fwd, workspace=ws)
+        tbk = run.walk(fwd)
         c.record(stream)
         if record:
             fwd_ev.append((a, b))
@@ -159,7 +222,7 @@ def main():
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
-        fwd, tbk = step(True)
+        step(True)
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
     ms_step = t_total * 1e3 / max(args.steps, 1)
@@ -171,6 +234,7 @@ def main():
 
     # algorithmic traffic of the forward kernel: 0.5 B/cell of traceback planes
     # + the 2-bit sequences read once
+    inp = run.inp
     seq_bytes = float((inp.qlen_host.sum() + inp.tlen_host.sum()) / 4)
     alg_bytes = total_cells * TB_BYTES_PER_CELL + seq_bytes
     hbm_gbs = alg_bytes / (fwd_ms / 1e3) / 1e9
@@ -219,121 +283,151 @@ def main():
             "ops_per_cell": OPS_PER_CELL_TB,
         },
     }
+    if args.skip_extras:
+        print(json.dumps(result), flush=True)
+        return
+
+    threads = min(os.cpu_count() or 1, 16)
+    result["host"] = host_info()
 
     # ------------------------------------------------------------- per-pair form
-    if not args.skip_extras:
-        sub = np.arange(0, w.n_pairs, 256)[:64]
-        ws1 = {"counter": torch.zeros(1, dtype=torch.int32, device=dev)}
-        sw1 = WL.subsample(w, sub)
-        inps = [K.upload_pairs([sw1.queries[k]], [sw1.targets[k]], sw1.band_offset[k:k + 1], dev,
-                               alphabet="2bit") for k in range(sw1.n_pairs)]
-        sub_cells = int(cells[sub].sum())
-
-        def per_pair():
-            for one in inps:
-                f1 = K.sw_forward(one, *sc, w.half_band, traceback=True, tile=tile,
-                                  workspace=ws1)
-                K.sw_traceback(one, f1)
-
-        per_pair()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        per_pair()
-        torch.cuda.synchronize()
-        pp_s = time.perf_counter() - t1
-        result["per_pair_gpu"] = {"value": round(sub_cells / pp_s / 1e9, 3), "unit": "GCUPS",
-                                  "sample": f"{len(sub)} pairs of sw_long (every 256th), one "
-                                            "forward launch + one walk launch per pair"}
-        result["batched_over_per_pair"] = round(gcups / (sub_cells / pp_s / 1e9), 1)
-        log(f"[bench] per-pair GPU form: {result['per_pair_gpu']['value']} GCUPS")
+    sub = np.arange(0, w.n_pairs, 256)[:64]
+    pp = per_pair_gcups(torch, K, w, sub, dev)
+    result["per_pair_gpu"] = {"value": round(pp, 3), "unit": "GCUPS",
+                              "sample": f"{len(sub)} pairs of sw_long (every 256th), one "
+                                        "forward launch + one walk launch per pair"}
+    result["batched_over_per_pair"] = round(gcups / pp, 1)
+    log(f"[bench] per-pair GPU form: {pp:.3f} GCUPS")
 
     # --------------------------------------------------------------- CPU oracle
-    if not args.skip_extras:
-        from oracle import oracle as O
+    srt = np.argsort(cells, kind="stable")
+    sub = srt[::args.cpu_sample_stride]
+    cg, cs = cpu_gcups(w, sub, threads)
+    result["cpu_baseline"] = {
+        "value": round(cg, 4),
+        "unit": "GCUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(sub)} sw_long pairs (every {args.cpu_sample_stride}th by cell count, "
+                  f"{int(cells[sub].sum()):.3e} cells), forward + traceback, one pair per "
+                  f"thread task, {cs:.1f} s",
+    }
+    result["gpu_over_cpu"] = round(gcups / cg, 1)
+    log(f"[bench] CPU oracle: {cg:.3f} GCUPS on {threads} threads ({cs:.1f} s)")
+    del run
+    torch.cuda.empty_cache()
 
-        O.build() if not os.path.exists(O._LIB_PATH) else None
-        srt = np.argsort(cells, kind="stable")
-        sub = srt[::args.cpu_sample_stride]
-        threads = min(os.cpu_count() or 1, 16)
-        qs = [w.as_ref_codes(int(k))[0] for k in sub]
-        ts = [w.as_ref_codes(int(k))[1] for k in sub]
-        t1 = time.perf_counter()
-        O.sw_align_pairs(qs, ts, w.band_offset[sub], w.half_band, *sc, threads=threads)
-        cpu_s = time.perf_counter() - t1
-        cpu_gcups = float(cells[sub].sum()) / cpu_s / 1e9
-        result["cpu_baseline"] = {
-            "value": round(cpu_gcups, 4),
-            "unit": "GCUPS",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{len(sub)} sw_long pairs (every {args.cpu_sample_stride}th by cell count, "
-                      f"{int(cells[sub].sum()):.3e} cells), forward + traceback, one pair per "
-                      f"thread task, {cpu_s:.1f} s",
-        }
-        result["gpu_over_cpu"] = round(gcups / cpu_gcups, 1)
-        log(f"[bench] CPU oracle: {cpu_gcups:.3f} GCUPS on {threads} threads")
+    # ------------------------------------------------------- short-read batch
+    ws_ = WL.sw_short()
+    rs = BatchRun(torch, K, ws_, dev, True)
+    rs.step()
+    ms = time_events(torch, rs.step, 10)
+    pps = per_pair_gcups(torch, K, ws_, np.arange(ws_.n_pairs), dev)
+    cgs, css = cpu_gcups(ws_, np.arange(ws_.n_pairs), threads)
+    result["sw_short"] = {
+        "batched_gcups": round(rs.total_cells / (ms / 1e3) / 1e9, 3),
+        "batched_ms": round(ms, 4),
+        "per_pair_gcups": round(pps, 4),
+        "cpu_gcups": round(cgs, 4),
+        "cpu_threads": threads,
+        "tile": [rs.tile.cells_per_lane, rs.tile.lanes_per_pair],
+        "workload": "1024 x 150 bp, half_band 32 at offset 32, seed 0 (CPU: whole batch)",
+    }
+    log(f"[bench] sw_short: {result['sw_short']}")
+    del rs
 
-    # ------------------------------------------------------- short reads, chains
-    if not args.skip_extras:
-        ws_ = WL.sw_short()
-        inp_s = K.upload_pairs(ws_.queries, ws_.targets, ws_.band_offset, dev, alphabet="2bit")
-        cs = ws_.cells()
-        tile_s = K.resolve_tile(ws_.half_band)
-        ords = torch.from_numpy(lpt_order(cs)).to(dev)
-        wss = {"counter": torch.zeros(1, dtype=torch.int32, device=dev)}
+    # ------------------------------------------------- noisy wide-band batch
+    t0 = time.time()
+    wn = WL.sw_noisy()
+    log(f"[bench] sw_noisy generated: {wn.n_pairs} pairs, {wn.cells().sum():.3e} cells, "
+        f"{time.time() - t0:.1f}s")
+    rn = BatchRun(torch, K, wn, dev, True)
+    rn.step()
+    torch.cuda.synchronize()
+    msn = time_events(torch, rn.step, 2)
+    ncells = rn.cells
+    # single longest pair alone: the lower bound a batch cannot beat
+    longest = int(np.argmax(ncells))
+    one = BatchRun(torch, K, WL.subsample(wn, [longest]), dev, True)
+    one.step()
+    ms1 = time_events(torch, one.step, 2)
+    del one
+    lens = np.array([len(q) for q in wn.queries])
+    edges = np.unique(np.round(np.geomspace(lens.min(), lens.max() + 1, 9)).astype(np.int64))
+    bins = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        idx = np.flatnonzero((lens >= lo) & (lens < hi))
+        if len(idx):
+            bins.append({"len": [int(lo), int(hi)], "pairs": int(len(idx)),
+                         "cells": int(ncells[idx].sum())})
+    tile_n = [rn.tile.cells_per_lane, rn.tile.lanes_per_pair]
+    del rn
+    torch.cuda.empty_cache()
+    srt = np.argsort(ncells, kind="stable")
+    ppn = per_pair_gcups(torch, K, wn, srt[::256], dev)
+    subn = srt[::16]
+    cgn, csn = cpu_gcups(wn, subn, threads)
+    result["sw_noisy"] = {
+        "batched_gcups": round(int(ncells.sum()) / (msn / 1e3) / 1e9, 3),
+        "batched_ms": round(msn, 3),
+        "longest_pair_ms": round(ms1, 3),
+        "longest_pair_cells": int(ncells[longest]),
+        "per_pair_gcups": round(ppn, 4),
+        "per_pair_sample": "every 256th pair by cell count (32 pairs)",
+        "cpu_gcups": round(cgn, 4),
+        "cpu_threads": threads,
+        "cpu_sample": f"every 16th pair by cell count ({len(subn)} pairs, "
+                      f"{int(ncells[subn].sum()):.3e} cells), {csn:.1f} s",
+        "tile": tile_n,
+        "length_bins": bins,
+        "workload": "8192 pairs, lognormal(8 kb, 1.0) clipped 0.5-100 kb, 6% errors "
+                    "(homopolymer-weighted indels), half_band 751, seed 3, fused walk",
+    }
+    log(f"[bench] sw_noisy: {msn:.1f} ms -> {result['sw_noisy']['batched_gcups']} GCUPS, "
+        f"longest pair alone {ms1:.1f} ms, per-pair {ppn:.3f}, CPU {cgn:.3f}")
 
-        def short_step():
-            f = K.sw_forward(inp_s, *sc, ws_.half_band, traceback=True, order=ords, tile=tile_s,
-                             workspace=wss)
-            K.sw_traceback(inp_s, f)
+    # ------------------------------------------------------------------ chains
+    from oracle import oracle as O
 
-        short_step()
-        ms = time_events(torch, short_step, 10)
-        inps1 = [K.upload_pairs([ws_.queries[k]], [ws_.targets[k]], ws_.band_offset[k:k + 1], dev,
-                                alphabet="2bit") for k in range(ws_.n_pairs)]
+    wc = WL.chains()
+    re_ = torch.from_numpy(wc.read_end).to(dev)
+    fe_ = torch.from_numpy(wc.ref_end).to(dev)
+    wt = torch.from_numpy(wc.weight).to(dev)
+    so = torch.from_numpy(wc.seg_off).to(dev)
 
-        def short_pp():
-            for one in inps1:
-                f1 = K.sw_forward(one, *sc, ws_.half_band, traceback=True, tile=tile_s,
-                                  workspace=wss)
-                K.sw_traceback(one, f1)
+    def chain_step():
+        K.chain_dp(re_, fe_, wt, so, wc.max_lookback, wc.max_gap, gap_open=wc.gap_open,
+                   gap_extend=wc.gap_extend)
 
-        short_pp()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        short_pp()
-        torch.cuda.synchronize()
-        pps = time.perf_counter() - t1
-        result["sw_short"] = {
-            "batched_gcups": round(int(cs.sum()) / (ms / 1e3) / 1e9, 3),
-            "per_pair_gcups": round(int(cs.sum()) / pps / 1e9, 4),
-            "batched_ms": round(ms, 4),
-            "tile": [tile_s.cells_per_lane, tile_s.lanes_per_pair],
-            "workload": "1024 x 150 bp, half_band 32 at offset 32, seed 0",
-        }
-        log(f"[bench] sw_short: {result['sw_short']}")
+    chain_step()
+    cms = time_events(torch, chain_step, 5)
+    lut = torch.from_numpy(O.penalty_lut(6.0, 0.05, 0.5, wc.max_gap)).to(dev)
+    wt64 = wt.to(torch.float64)
 
-        wc = WL.chains()
-        d = dev
-        re_ = torch.from_numpy(wc.read_end).to(d)
-        fe_ = torch.from_numpy(wc.ref_end).to(d)
-        wt = torch.from_numpy(wc.weight).to(d)
-        so = torch.from_numpy(wc.seg_off).to(d)
+    def chain_step64():
+        K.chain_dp(re_, fe_, wt64, so, wc.max_lookback, wc.max_gap, pen_lut=lut)
 
-        def chain_step():
-            K.chain_dp(re_, fe_, wt, so, wc.max_lookback, wc.max_gap, gap_open=wc.gap_open,
-                       gap_extend=wc.gap_extend)
-
-        chain_step()
-        cms = time_events(torch, chain_step, 5)
-        result["chains"] = {
-            "reads_per_s": round(wc.n_reads / (cms / 1e3), 1),
-            "pair_evals_per_s": round(wc.pair_evaluations() / (cms / 1e3), 1),
-            "ms": round(cms, 4),
-            "workload": "4096 reads, Poisson(400) anchors, lookback 50, max_gap 5000, "
-                        "int profile 6/1, seed 1",
-        }
-        log(f"[bench] chains: {result['chains']}")
+    chain_step64()
+    cms64 = time_events(torch, chain_step64, 5)
+    t1 = time.perf_counter()
+    for s in range(wc.n_reads):
+        a, b = wc.seg_off[s], wc.seg_off[s + 1]
+        O.chain_dp_i32(wc.read_end[a:b], wc.ref_end[a:b], wc.weight[a:b], wc.max_lookback,
+                       wc.max_gap, wc.gap_open, wc.gap_extend)
+    ccpu = time.perf_counter() - t1
+    result["chains"] = {
+        "reads_per_s": round(wc.n_reads / (cms / 1e3), 1),
+        "pair_evals_per_s": round(wc.pair_evaluations() / (cms / 1e3), 1),
+        "ms": round(cms, 4),
+        "f64_ref_default_ms": round(cms64, 4),
+        "f64_reads_per_s": round(wc.n_reads / (cms64 / 1e3), 1),
+        "cpu_reads_per_s": round(wc.n_reads / ccpu, 1),
+        "cpu_threads": 1,
+        "workload": "4096 reads, Poisson(400) anchors, lookback 50, max_gap 5000, "
+                    "int profile 6/1 (and float64 6/0.05/0.5), seed 1; CPU: whole batch, "
+                    "int profile, one thread",
+    }
+    log(f"[bench] chains: {result['chains']}")
 
     print(json.dumps(result), flush=True)
 

@@ -178,6 +178,30 @@ def tb_bytes(qlen: np.ndarray, tile: SwTile) -> np.ndarray:
     return nblk * tile.lanes_stored * tile.cells_per_lane * 4
 
 
+def make_workspace(inp: SwInputs, tile: SwTile, traceback: bool = True,
+                   cigar: bool = True) -> dict:
+    """Preallocate every buffer ``sw_forward`` / ``sw_traceback`` would
+    allocate for this batch (work counter, traceback planes, CIGAR runs), so
+    repeated passes over a resident batch allocate nothing."""
+    dev = inp.device
+    n = inp.n_pairs
+    ws = {"counter": torch.zeros(1, dtype=torch.int32, device=dev)}
+    if traceback:
+        nb = tb_bytes(inp.qlen_host, tile)
+        offs = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(nb // 4, out=offs[1:])
+        ws["tb"] = torch.empty(max(int(offs[-1]), 1), dtype=torch.int32, device=dev)
+        ws["tb_off"] = torch.from_numpy(offs[:-1]).to(dev)
+        ws["tb_bytes"] = int(offs[-1]) * 4
+    if cigar:
+        cap = inp.qlen_host.astype(np.int64) + inp.tlen_host.astype(np.int64) + 1
+        coffs = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(cap, out=coffs[1:])
+        ws["cigar"] = torch.empty(max(int(coffs[-1]), 1), dtype=torch.int32, device=dev)
+        ws["cigar_off"] = torch.from_numpy(coffs[:-1]).to(dev)
+    return ws
+
+
 @dataclass
 class SwForward:
     score: torch.Tensor
