@@ -168,7 +168,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(pairs and reads shard with no collective; see DESIGN.md).", file=sys.stderr)
+              "(pairs and reads are independent and shard across GPUs with no collective).",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -59,6 +59,17 @@ class SwTile(ctypes.Structure):
                 ("half_band", _i32)]
 
 
+class ServeClass(ctypes.Structure):
+    _fields_ = [("batch", SwBatch), ("params", SwParams), ("out", SwOut),
+                ("slab_words", _i64), ("n_slots", _i32), ("reserved", _i32),
+                ("arrival_s", _P), ("complete_s", _P)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("completed", _i64), ("timed_out", _i32),
+                ("waves_gave_up", _i32)]
+
+
 # Every exported symbol with its ctypes signature (must match include/gmfa.h).
 SIGNATURES = {
     "gmfa_last_error": (ctypes.c_char_p, []),
@@ -77,6 +88,8 @@ SIGNATURES = {
     "gmfa_chain_dp_i32": (ctypes.c_int, [_P, _P, _P, _P, _i32, _i32, _i64, _i32, _i32, _P,
                                          _P, _P]),
     "gmfa_chain_backtrack": (_i64, [_i64, _P, _P, _i64, ctypes.c_double, _P, _P, _P]),
+    "gmfa_serve_run": (ctypes.c_int, [ctypes.POINTER(ServeClass), _i32, ctypes.c_double,
+                                      ctypes.POINTER(ServeReport)]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libgmfa.so")
-SOURCES = ["api.hip", "sw.hip", "chain.hip"]
+SOURCES = ["api.hip", "sw.hip", "chain.hip", "serve.hip"]
 ARCH = "gfx950"
 
 
@@ -37,7 +37,7 @@ def compile_flags():
         "-fPIC",
         f"--offload-arch={ARCH}",
         "-mcode-object-version=5",
-        # no packed-FP32 instruction selection (see DESIGN.md hardware notes)
+        # no packed-FP32 instruction selection (the v_pk_*_f32 forms are kept out)
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-fno-gpu-rdc",
         "-I", os.path.join(ROOT, "include"),

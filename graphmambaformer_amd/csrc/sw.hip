@@ -46,6 +46,7 @@
 // lexicographically smallest (i, j) among cells with the maximal H.  Each lane
 // tracks its own first maximum; the group reduces (max H, min i, min d).
 #include "common.h"
+#include "sw_internal.h"
 
 #include <algorithm>
 #include <climits>
@@ -80,7 +81,47 @@ struct SwKArgs {
     int32_t walk;  // walk every pair right after its forward pass (CIGAR out)
     WalkCtx wc;
     WalkOut wo;
+    // continuous serving (serve_tail == nullptr: one offline batch).  Request
+    // k may be claimed once the host has published k < *serve_tail; the
+    // flags live in host-coherent memory.
+    const int32_t *serve_tail;    // requests published so far
+    const int32_t *serve_closed;  // nonzero: nothing more will be published
+    int32_t *serve_done;          // [n_pairs] set to 1 when request k is complete
+    int32_t *serve_gave_up;       // set to 1 if a wave's wait hit deadline_ticks
+    int64_t slab_words;           // traceback plane words per wave slot
+    uint64_t deadline_ticks;      // bound of one idle wait (wall-clock ticks)
 };
+
+// Wait until request `item` is published or the host closes the queue; returns
+// the number of published requests.  Every wait ends: at the latest after
+// deadline_ticks of the constant-rate wall clock (s_sleep 127 = 8128 cycles).  (Scalars by value: taking
+// the kernel argument block by reference would move it to scratch memory.)
+__device__ __forceinline__ int serve_wait(const int32_t *tail_p, const int32_t *closed_p,
+                                          int32_t *gave_up, uint64_t deadline_ticks, int item)
+{
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    // polls cross the host link: back off to one poll per ~50 us while idle
+    int nap = 1;
+    for (;;) {
+        const int tail = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(tail_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM));
+        if (item < tail) return tail;
+        const int closed = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(closed_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM));
+        if (closed) {
+            // the host publishes the final tail before closing
+            return __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(tail_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM));
+        }
+        if ((uint64_t)wall_clock64() - t0 > deadline_ticks) {
+            if (threadIdx.x == 0)
+                __hip_atomic_store(gave_up, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            return tail;
+        }
+        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(127);
+        nap = nap < 16 ? 2 * nap : 16;
+    }
+}
 
 __device__ __forceinline__ int64_t clampi64(int64_t x, int64_t hi)
 {
@@ -335,7 +376,9 @@ __global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
     a.o.counts[4 * p + 3] = cnt[3];
 }
 
-template <int C, int G, int KS, bool TB, bool NM, bool RK>
+// SV: serving variant (persistent queue fed by the host); a separate
+// instantiation so the offline kernels carry none of its state.
+template <int C, int G, int KS, bool TB, bool NM, bool RK, bool SV = false>
 __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 {
     constexpr int NCW = (C + 15) / 16;  // target words feeding the cells
@@ -362,10 +405,15 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         int item = 0;
         if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
         item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= a.n_items) break;
+        // serving: one pair per wavefront (G == 64), bounded by what the host
+        // has published
+        int n_items = a.n_items;
+        if constexpr (SV)
+            n_items = serve_wait(a.serve_tail, a.serve_closed, a.serve_gave_up, a.deadline_ticks, item);
+        if (item >= n_items) break;
 
         const int pidx = item * (64 / G) + grp;
-        const bool has = pidx < a.n_pairs;
+        const bool has = pidx < (SV ? n_items : a.n_pairs);
         const int p = has ? (a.order ? a.order[pidx] : pidx) : 0;
         const int qlen = uni<G>(has ? a.qlen[p] : 0);
         const int tlen = uni<G>(has ? a.tlen[p] : 0);
@@ -382,7 +430,9 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 
         const int nblk = (maxq + 7) >> 3;
         const int nblk_mine = (qlen + 7) >> 3;
-        const int64_t tb_base = (TB && has) ? a.tb_off[p] : 0;
+        // serving: each wave slot owns one plane slab, reused pair after pair
+        const int64_t tb_base = (TB && has) ? (SV ? (int64_t)blockIdx.x * a.slab_words : a.tb_off[p])
+                                            : 0;
         const int64_t rk_base = (RK && has) ? a.rk_off[p] : 0;
 
         // target element (0-based in the pair) of cell 0 at the block's first row
@@ -628,6 +678,12 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 }
             }
         }
+        if constexpr (SV) {
+            if (has && lane == 0) {
+                // every output of this pair (stored by this wave) before the flag
+                __hip_atomic_store(a.serve_done + p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
     }
 }
 
@@ -684,15 +740,28 @@ SwKernel pick_kernel(uint32_t flags)
     return sw_forward_kernel<C, G, KS, false, false, false>;
 }
 
+// serving variant: traceback + fused CIGAR walk, one pair per wavefront
+template <int C, int G, int KS>
+SwKernel pick_serve_kernel(uint32_t flags)
+{
+    if constexpr (G == 64) {
+        if (flags & GMFA_SW_NMASK) return sw_forward_kernel<C, G, KS, true, true, false, true>;
+        return sw_forward_kernel<C, G, KS, true, false, false, true>;
+    }
+    return nullptr;
+}
+
 // KS >= 0: the tile serves only bands with (W-1) % C == KS (compile-time dead
 // registers; KS == C-1 is an exact fit with none).  KS == -1: any band.
 struct TileEntry {
     int C, G, KS;
     SwKernel (*get)(uint32_t);
     void (*walk)(TbKArgs);
+    SwKernel (*serve)(uint32_t);
 };
 
-#define GMFA_TILE(C, G, KS) {C, G, KS, &pick_kernel<C, G, KS>, &sw_traceback_kernel<C>}
+#define GMFA_TILE(C, G, KS) \
+    {C, G, KS, &pick_kernel<C, G, KS>, &sw_traceback_kernel<C>, &pick_serve_kernel<C, G, KS>}
 static const TileEntry kTiles[] = {
     // exact fits of the benchmark bands: W=65 (5x13), W=513 (9x57); W=1503 (24x63, 15 live
     // registers in the last lane)
@@ -841,8 +910,10 @@ static WalkCtx make_walk_ctx(const gmfa_sw_batch *batch, const gmfa_sw_params *p
     return c;
 }
 
-extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
-                               const gmfa_sw_out *out, void *stream)
+namespace gmfa {
+
+int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gmfa_sw_out *out,
+              const ServeLaunch *serve, hipStream_t s)
 {
     GMFA_REQUIRE(batch && params && out, "NULL argument");
     const gmfa_sw_params &P = *params;
@@ -872,12 +943,12 @@ extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params 
     GMFA_REQUIRE(!nm || (batch->qmask && batch->tmask), "GMFA_SW_NMASK needs qmask and tmask");
     GMFA_REQUIRE(out->score && out->query_end && out->target_end && out->work_counter,
                  "NULL output pointer");
-    GMFA_REQUIRE(!(P.flags & GMFA_SW_TRACEBACK) || (out->tb && out->tb_off),
+    GMFA_REQUIRE(!(P.flags & GMFA_SW_TRACEBACK) || (out->tb && (out->tb_off || serve)),
                  "GMFA_SW_TRACEBACK needs tb and tb_off");
     GMFA_REQUIRE(!(P.flags & GMFA_SW_ROWKEYS) || (out->rowkey && out->rowkey_off),
                  "GMFA_SW_ROWKEYS needs rowkey and rowkey_off");
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, 2 * P.half_band + 1);
-    SwKernel k = t->get(P.flags);
+    SwKernel k = serve ? t->serve(P.flags) : t->get(P.flags);
 
     SwKArgs a;
     a.qseq = batch->qseq; a.tseq = batch->tseq; a.qmask = batch->qmask; a.tmask = batch->tmask;
@@ -908,13 +979,41 @@ extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params 
         a.wo.rstart = out->read_start; a.wo.tstart = out->ref_start; a.wo.counts = out->counts;
     }
 
-    hipStream_t s = (hipStream_t)stream;
+    a.serve_tail = nullptr; a.serve_closed = nullptr; a.serve_done = nullptr;
+    a.serve_gave_up = nullptr; a.slab_words = 0; a.deadline_ticks = 0;
+    int waves = 1;
+    if (!serve) {
+        waves = std::max(1, std::min(a.n_items, resident_waves(k)));
+    } else {
+        GMFA_REQUIRE(t->G == 64 && k != nullptr,
+                     "serving needs a tile with one pair per wavefront (64 lanes per pair), got %d",
+                     t->G);
+        GMFA_REQUIRE((P.flags & GMFA_SW_CIGAR) && !(P.flags & GMFA_SW_ROWKEYS),
+                     "serving needs GMFA_SW_TRACEBACK | GMFA_SW_CIGAR and no row keys");
+        GMFA_REQUIRE(batch->order == nullptr, "serving takes requests in arrival order (order must be NULL)");
+        GMFA_REQUIRE(serve->n_slots >= 1, "n_slots must be positive");
+        const int64_t need = gmfa_sw_tb_bytes(batch->max_qlen, &tile) / 4;
+        GMFA_REQUIRE(serve->slab_words >= need,
+                     "slab of %lld plane words per slot is below the %lld the longest query needs",
+                     (long long)serve->slab_words, (long long)need);
+        a.serve_tail = serve->tail; a.serve_closed = serve->closed; a.serve_done = serve->done;
+        a.serve_gave_up = serve->gave_up; a.slab_words = serve->slab_words;
+        a.deadline_ticks = serve->deadline_ticks;
+        waves = serve->n_slots;
+    }
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
     GMFA_CHECK_HIP(hipGetLastError());
-    const int waves = std::max(1, std::min(a.n_items, resident_waves(k)));
     hipLaunchKernelGGL(k, dim3(waves), dim3(64), 0, s, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
+}
+
+}  // namespace gmfa
+
+extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
+                               const gmfa_sw_out *out, void *stream)
+{
+    return sw_launch(batch, params, out, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_params *params,

@@ -1,0 +1,205 @@
+"""Continuous serving of banded SW + CIGAR requests on one device.
+
+The reference aligns reads one after another: `pipeline.py:355-363` fans
+reads over host threads, each calling `ExtensionEngine.extend_chains`
+(`extension.py:511-641`) with its own small batch.  Here a request stream is
+served by `gmfa_serve_run`: one persistent forward kernel per band class, a
+wavefront per request, the CIGAR walked in the same launch, requests published
+by the host as they arrive (continuous batching; no batch boundaries, no launch
+per request).  Targets are windows of a reference kept resident on the device
+as a 2-bit stream; a request names its window by base offset.
+
+Memory per class: one traceback-plane slab per wavefront slot (sized for the
+class's longest query), per-request outputs (score, ends, CIGAR runs) and the
+request table.  The library allocates only a small host-coherent control block
+for the duration of a run.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib, kernels
+from ._lib import ServeClass, ServeReport, SwTile, check, lib, ptr
+
+
+def serve_tile(half_band: int) -> SwTile:
+    """The compiled tile with one pair per wavefront (64 lanes) and the least
+    padding for this band."""
+    W = 2 * int(half_band) + 1
+    fits = [C for C, G in kernels.list_tiles() if G == 64 and C * 64 >= W]
+    if not fits:
+        raise RuntimeError(f"no 64-lane tile holds a band of width {W}")
+    return kernels.resolve_tile(half_band, min(fits), 64)
+
+
+@dataclass
+class ResidentReference:
+    """A reference held on the device as one 2-bit stream (16 bases per word)."""
+
+    words: torch.Tensor
+    n_bases: int
+
+    @classmethod
+    def upload(cls, codes: np.ndarray, device) -> "ResidentReference":
+        """codes: 0..3 (A, C, G, T)."""
+        w, _, _, _ = kernels.pack_host([np.asarray(codes, np.uint8)], alphabet="2bit")
+        t = torch.from_numpy(w.view(np.int32)).to(torch.device(device))
+        return cls(words=t, n_bases=int(len(codes)))
+
+
+@dataclass
+class RequestPool:
+    """Queries of one class packed on the device, with their reference windows."""
+
+    half_band: int
+    qseq: torch.Tensor
+    q_off: np.ndarray   # base offset of each query in qseq
+    qlen: np.ndarray
+    t_start: np.ndarray  # base offset of each window in the reference
+    t_len: np.ndarray
+    band_offset: np.ndarray
+
+    @classmethod
+    def upload(cls, queries: Sequence[np.ndarray], t_start, t_len, band_offset, half_band: int,
+               device) -> "RequestPool":
+        w, _, qo, _ = kernels.pack_host([np.asarray(q, np.uint8) for q in queries], "2bit")
+        return cls(half_band=int(half_band),
+                   qseq=torch.from_numpy(w.view(np.int32)).to(torch.device(device)),
+                   q_off=qo.astype(np.int64),
+                   qlen=np.array([len(q) for q in queries], np.int32),
+                   t_start=np.asarray(t_start, np.int64), t_len=np.asarray(t_len, np.int32),
+                   band_offset=np.asarray(band_offset, np.int32))
+
+
+@dataclass
+class ServeClassRun:
+    """One band class of a serving run: its request table (pool entries in
+    arrival order), outputs, slabs and arrival times."""
+
+    name: str
+    inp: kernels.SwInputs
+    scoring: tuple
+    half_band: int
+    tile: SwTile
+    n_slots: int
+    slab_words: int
+    arrival_s: np.ndarray
+    complete_s: np.ndarray
+    score: torch.Tensor
+    qend: torch.Tensor
+    tend: torch.Tensor
+    tb: torch.Tensor
+    counter: torch.Tensor
+    walk: kernels.SwTraceback
+    picks: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+
+    @property
+    def n(self) -> int:
+        return self.inp.n_pairs
+
+    def struct(self) -> ServeClass:
+        s = ServeClass()
+        s.batch = self.inp.batch_struct(None)
+        ma, mi, go, ge = self.scoring
+        flags = _lib.SW_TRACEBACK | _lib.SW_CIGAR
+        s.params = kernels.make_params(ma, mi, go, ge, self.half_band, flags, self.tile)
+        o = s.out
+        o.score, o.query_end, o.target_end = ptr(self.score), ptr(self.qend), ptr(self.tend)
+        o.tb, o.tb_off = ptr(self.tb), None
+        o.rowkey, o.rowkey_off = None, None
+        o.work_counter = ptr(self.counter)
+        w = self.walk
+        o.cigar_off, o.cigar, o.n_cigar = ptr(w.cigar_off), ptr(w.cigar), ptr(w.n_cigar)
+        o.read_start, o.ref_start, o.counts = ptr(w.read_start), ptr(w.ref_start), ptr(w.counts)
+        s.slab_words = int(self.slab_words)
+        s.n_slots = int(self.n_slots)
+        s.arrival_s = self.arrival_s.ctypes.data
+        s.complete_s = self.complete_s.ctypes.data
+        return s
+
+
+def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks: np.ndarray,
+               arrival_s: np.ndarray, scoring: tuple, n_slots: int,
+               tile: Optional[SwTile] = None) -> ServeClassRun:
+    """Request table for pool entries `picks` (in arrival order)."""
+    dev = reference.words.device
+    picks = np.asarray(picks, np.int64)
+    n = len(picks)
+    if len(arrival_s) != n:
+        raise ValueError("one arrival time per request")
+    tile = tile or serve_tile(pool.half_band)
+    if tile.lanes_per_pair != 64:
+        raise ValueError("serving needs a 64-lane tile (one pair per wavefront)")
+    if n and (pool.t_start[picks].min() < 0 or
+              (pool.t_start[picks] + pool.t_len[picks]).max() > reference.n_bases):
+        raise ValueError("a request window leaves the reference")
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    ql, tl = pool.qlen[picks], pool.t_len[picks]
+    bo = pool.band_offset[picks]
+    inp = kernels.SwInputs(qseq=pool.qseq, tseq=reference.words, qmask=None, tmask=None,
+                           q_off=up(pool.q_off[picks]), t_off=up(pool.t_start[picks]),
+                           qlen=up(ql), tlen=up(tl), band_offset=up(bo), qlen_host=ql,
+                           tlen_host=tl, boff_host=bo, has_n=False)
+    max_q = int(ql.max()) if n else 0
+    slab = max(int(kernels.tb_bytes(np.array([max_q]), tile)[0]) // 4, 1)
+    cap = ql.astype(np.int64) + tl.astype(np.int64) + 1
+    coff = np.zeros(n + 1, np.int64)
+    np.cumsum(cap, out=coff[1:])
+    walk = kernels.SwTraceback(
+        cigar=torch.empty(max(int(coff[-1]), 1), dtype=torch.int32, device=dev),
+        cigar_off=up(coff[:-1]), n_cigar=torch.zeros(n, dtype=torch.int32, device=dev),
+        read_start=torch.zeros(n, dtype=torch.int32, device=dev),
+        ref_start=torch.zeros(n, dtype=torch.int32, device=dev),
+        counts=torch.zeros((n, 4), dtype=torch.int32, device=dev))
+    return ServeClassRun(
+        name=name, inp=inp, scoring=tuple(int(x) for x in scoring), half_band=pool.half_band,
+        tile=tile, n_slots=int(n_slots), slab_words=slab,
+        arrival_s=np.ascontiguousarray(arrival_s, dtype=np.float64),
+        complete_s=np.full(n, -1.0), score=torch.zeros(n, dtype=torch.int32, device=dev),
+        qend=torch.zeros(n, dtype=torch.int32, device=dev),
+        tend=torch.zeros(n, dtype=torch.int32, device=dev),
+        tb=torch.empty(int(n_slots) * slab, dtype=torch.int32, device=dev),
+        counter=torch.zeros(1, dtype=torch.int32, device=dev), walk=walk, picks=picks)
+
+
+@dataclass
+class ServeResult:
+    wall_s: float
+    completed: int
+    timed_out: bool
+    waves_gave_up: bool
+
+
+def run(classes: Sequence[ServeClassRun], timeout_s: float = 60.0) -> ServeResult:
+    """Serve every class's requests at their arrival times; blocks until all
+    completed (or the timeout).  complete_s of each class is filled in."""
+    arr = (ServeClass * len(classes))(*[c.struct() for c in classes])
+    rep = ServeReport()
+    torch.cuda.synchronize()
+    rc = lib().gmfa_serve_run(arr, len(classes), float(timeout_s), ctypes.byref(rep))
+    check(rc, "gmfa_serve_run")
+    return ServeResult(wall_s=rep.wall_s, completed=int(rep.completed),
+                       timed_out=bool(rep.timed_out), waves_gave_up=bool(rep.waves_gave_up))
+
+
+def latency_summary(arrival_s: np.ndarray, complete_s: np.ndarray, skip: int = 0) -> dict:
+    """p50 / p99 / mean latency (ms) of completed requests after the first
+    `skip` arrivals."""
+    a = np.asarray(arrival_s)[skip:]
+    c = np.asarray(complete_s)[skip:]
+    ok = c >= 0
+    lat = (c[ok] - a[ok]) * 1e3
+    if lat.size == 0:
+        return {"n": 0}
+    return {"n": int(lat.size), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3), "max_ms": round(float(lat.max()), 3)}

@@ -218,3 +218,94 @@ def chains(n_reads: int = 4096, seed: int = 1, read_len: int = 10000,
     return ChainWorkload(read_end=read_end, ref_end=ref_end,
                          weight=np.full(len(read_end), span, np.int32),
                          seg_off=np.array(offs, np.int64))
+
+
+# ----------------------------------------------------------------------------
+# serving: requests against a resident reference
+# ----------------------------------------------------------------------------
+
+
+@dataclass
+class ServePool:
+    """Distinct requests of one band class.  Request k aligns query k against
+    the reference window [t_start[k], t_start[k] + t_len[k])."""
+
+    name: str
+    half_band: int
+    queries: list
+    t_start: np.ndarray  # int64, base offset in the reference
+    t_len: np.ndarray    # int32
+    band_offset: np.ndarray  # int32
+
+    @property
+    def n(self) -> int:
+        return len(self.queries)
+
+    def cells(self) -> np.ndarray:
+        return band_cells([len(q) for q in self.queries], self.t_len, self.band_offset,
+                          self.half_band)
+
+    def pair(self, k: int, reference: np.ndarray):
+        """(query, target) of request k as 0..3 codes."""
+        s = int(self.t_start[k])
+        return self.queries[k], reference[s:s + int(self.t_len[k])]
+
+
+@dataclass
+class ServingMix:
+    reference: np.ndarray  # uint8 codes 0..3
+    pools: list            # ServePool per class
+    mix: tuple             # request share of each class
+    match: int = MATCH
+    mismatch: int = MISMATCH
+    gap_open: int = GAP_OPEN
+    gap_extend: int = GAP_EXTEND
+
+
+def serving_mix(seed: int = 4, ref_len: int = 46_000_000,
+                pool_sizes: tuple = (2048, 1024, 512)) -> ServingMix:
+    """A random resident reference and request pools of three band classes:
+    short (150 bp reads against 214 bp windows, half_band 32 at offset 32, 1%
+    substitutions + 0.2% indels), long (lognormal(12 kb, 0.4) clipped 5-25 kb,
+    0.5% errors, half_band 256) and noisy (lognormal(8 kb, 1.0) clipped
+    0.5-100 kb, 6% homopolymer-weighted errors, half_band 751).  Windows start
+    at uniform positions; the query is the mutated window."""
+    rng = np.random.default_rng(seed)
+    ref = rng.integers(0, 4, size=ref_len, dtype=np.uint8)
+    pools = []
+    # short reads
+    n = pool_sizes[0]
+    pos = rng.integers(0, ref_len - 214, size=n)
+    qs = [mutate(rng, ref[p + 32:p + 182], 0.0, 0.0, 3, sub_rate=0.01, indel_rate=0.002)
+          for p in pos]
+    pools.append(ServePool("short", 32, qs, pos.astype(np.int64), np.full(n, 214, np.int32),
+                           np.full(n, 32, np.int32)))
+    # long reads
+    n = pool_sizes[1]
+    lens = _lognormal_lengths(rng, n, 12000, 0.4, 5000, 25000)
+    pos = np.array([rng.integers(0, ref_len - L) for L in lens], np.int64)
+    qs = [mutate(rng, ref[p:p + L], 0.005, 0.7, 2) for p, L in zip(pos, lens)]
+    pools.append(ServePool("long", 256, qs, pos, lens.astype(np.int32), np.zeros(n, np.int32)))
+    # noisy reads
+    n = pool_sizes[2]
+    lens = _lognormal_lengths(rng, n, 8000, 1.0, 500, 100000)
+    pos = np.array([rng.integers(0, ref_len - L) for L in lens], np.int64)
+    qs = [mutate(rng, ref[p:p + L], 0.06, 0.6, 3, homopolymer_weight=2.0)
+          for p, L in zip(pos, lens)]
+    pools.append(ServePool("noisy", 751, qs, pos, lens.astype(np.int32), np.zeros(n, np.int32)))
+    return ServingMix(ref, pools, (0.6, 0.3, 0.1))
+
+
+def serving_requests(mix: ServingMix, n_requests: int, seed: int = 5):
+    """Class and pool index of each request of a stream (class drawn with the
+    mix shares, pool entry uniformly)."""
+    rng = np.random.default_rng(seed)
+    cls = rng.choice(len(mix.pools), size=n_requests, p=np.asarray(mix.mix) / sum(mix.mix))
+    idx = np.array([rng.integers(0, mix.pools[c].n) for c in cls], np.int64)
+    return cls.astype(np.int32), idx
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 6) -> np.ndarray:
+    """Arrival times (s) of a Poisson stream of n requests at `rate` per second."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))

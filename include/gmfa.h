@@ -24,6 +24,11 @@
  *   gmfa_chain_dp_i32  the same DP on the integer profile
  *   gmfa_chain_backtrack graphmambaformer/alignment/chaining.py:530-576
  *                      AffineChainer._backtrack (host code)
+ *   gmfa_serve_run     the per-read extension calls of
+ *                      graphmambaformer/alignment/pipeline.py:355-363
+ *                      (ExtensionEngine.extend_chains, extension.py:511-641,
+ *                      one read after another) as a continuously batched
+ *                      server: requests are aligned as they arrive
  */
 #ifndef GMFA_H
 #define GMFA_H
@@ -180,6 +185,48 @@ int gmfa_chain_dp_i32(const int64_t *read_end, const int64_t *ref_end, const int
 int64_t gmfa_chain_backtrack(int64_t n, const double *f, const int64_t *parent,
                              int64_t min_anchors, double min_score, int64_t *members,
                              int64_t *chain_start, double *chain_score);
+
+/* ---------------------------------------------------------------------------
+ * Continuous serving.  One class = requests sharing one band width and
+ * scoring.  Each class runs one persistent forward kernel of n_slots
+ * wavefronts on a stream of its own; a wavefront takes the next published
+ * request, computes it, walks its CIGAR, and marks it complete.  The host
+ * thread publishes request k of a class at arrival_s[k] (seconds after the
+ * start of the run) and records complete_s[k] when it sees the completion.
+ *
+ * batch:  every request of the class, in arrival order (device memory, all
+ *         written before the call); batch.order must be NULL.
+ * params: flags GMFA_SW_TRACEBACK | GMFA_SW_CIGAR (+ GMFA_SW_NMASK), a tile
+ *         with 64 lanes per pair (one pair per wavefront).
+ * out:    per-request outputs as for gmfa_sw_forward with GMFA_SW_CIGAR;
+ *         out.tb holds n_slots plane slabs of slab_words uint32 each
+ *         (slab_words >= gmfa_sw_tb_bytes(batch.max_qlen) / 4); out.tb_off
+ *         is unused; out.work_counter is the class's queue head.
+ * A run ends when every request completed or after timeout_s; a wavefront
+ * never waits longer than timeout_s + 5 s for a request.  The control words
+ * (queue tails and completion flags) are allocated in host-coherent memory
+ * for the duration of the call.  One run at a time per device.
+ * ------------------------------------------------------------------------- */
+typedef struct gmfa_serve_class {
+    gmfa_sw_batch batch;
+    gmfa_sw_params params;
+    gmfa_sw_out out;
+    int64_t slab_words;
+    int32_t n_slots;
+    int32_t reserved;
+    const double *arrival_s; /* host [batch.n_pairs], nondecreasing */
+    double *complete_s;      /* host [batch.n_pairs], out (-1: not completed) */
+} gmfa_serve_class;
+
+typedef struct gmfa_serve_report {
+    double wall_s;          /* start of the run .. last completion seen */
+    int64_t completed;      /* requests completed over all classes */
+    int32_t timed_out;      /* 1: timeout_s ended the run before all completed */
+    int32_t waves_gave_up;  /* 1: a wavefront's wait reached its deadline */
+} gmfa_serve_report;
+
+int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes, double timeout_s,
+                   gmfa_serve_report *report);
 
 #ifdef __cplusplus
 }

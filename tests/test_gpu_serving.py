@@ -1,0 +1,150 @@
+"""GPU tests of continuous serving (gmfa_serve_run): every request of a mixed
+stream completes, and its score, ends and CIGAR are bit-identical to the
+oracle's on the same pair, whatever the arrival pattern or the class mix
+(alone vs mixed, paced vs all-at-once)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+SC = (2, 4, 4, 2)
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _small_mix(seed=11):
+    """A 2 Mbp reference and three small pools (short / long / noisy bands)."""
+    from graphmambaformer_amd import workloads as WL
+
+    rng = np.random.default_rng(seed)
+    ref = rng.integers(0, 4, size=2_000_000, dtype=np.uint8)
+    pools = []
+    pos = rng.integers(0, len(ref) - 214, size=48)
+    qs = [WL.mutate(rng, ref[p + 32:p + 182], 0.0, 0.0, 3, sub_rate=0.01, indel_rate=0.002)
+          for p in pos]
+    pools.append(("short", 32, qs, pos, np.full(48, 214), np.full(48, 32)))
+    lens = rng.integers(800, 4000, size=12)
+    pos = np.array([rng.integers(0, len(ref) - L) for L in lens])
+    qs = [WL.mutate(rng, ref[p:p + L], 0.005, 0.7, 2) for p, L in zip(pos, lens)]
+    pools.append(("long", 256, qs, pos, lens, np.zeros(12)))
+    lens = rng.integers(300, 3000, size=8)
+    pos = np.array([rng.integers(0, len(ref) - L) for L in lens])
+    qs = [WL.mutate(rng, ref[p:p + L], 0.06, 0.6, 3, homopolymer_weight=2.0)
+          for p, L in zip(pos, lens)]
+    pools.append(("noisy", 751, qs, pos, lens, np.zeros(8)))
+    return ref, pools
+
+
+def _expected(ref, pool):
+    name, hb, qs, pos, lens, offs = pool
+    q1 = [q.astype(np.int16) + 1 for q in qs]
+    t1 = [ref[p:p + L].astype(np.int16) + 1 for p, L in zip(pos, lens)]
+    return O.sw_align_pairs(q1, t1, np.asarray(offs, np.int64), hb, *SC, threads=8)
+
+
+def _build(ref_dev, pools_dev, picks, arrivals, slots=(16, 16, 8)):
+    from graphmambaformer_amd import serving as S
+
+    out = []
+    for c, (pool, pk, arr) in enumerate(zip(pools_dev, picks, arrivals)):
+        out.append(S.make_class(f"class{c}", ref_dev, pool, pk, arr, SC, slots[c]))
+    return out
+
+
+def _check(cls, exp, label):
+    from graphmambaformer_amd import kernels as K
+
+    assert np.all(cls.complete_s >= 0), label
+    score = cls.score.cpu().numpy()
+    qe = cls.qend.cpu().numpy()
+    te = cls.tend.cpu().numpy()
+    cig = K.decode_cigars(cls.walk)
+    rs = cls.walk.read_start.cpu().numpy()
+    ts = cls.walk.ref_start.cpu().numpy()
+    for k, pidx in enumerate(cls.picks):
+        e = exp[int(pidx)]
+        ctx = (label, k, int(pidx))
+        assert int(score[k]) == e["score"], ctx
+        assert int(qe[k]) == e["read_end"] and int(te[k]) == e["ref_end"], ctx
+        assert cig[k] == e["cigar"], ctx
+        assert int(rs[k]) == e["read_start"] and int(ts[k]) == e["ref_start"], ctx
+
+
+@pytest.fixture(scope="module")
+def mix():
+    from graphmambaformer_amd import serving as S
+
+    ref, pools = _small_mix()
+    ref_dev = S.ResidentReference.upload(ref, DEV)
+    pools_dev = [S.RequestPool.upload(qs, pos, lens, offs, hb, DEV)
+                 for (_, hb, qs, pos, lens, offs) in pools]
+    exp = [_expected(ref, p) for p in pools]
+    return ref_dev, pools_dev, exp
+
+
+def test_serving_paced_mixed_stream_matches_oracle(mix):
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    rng = np.random.default_rng(3)
+    picks = [rng.integers(0, len(e), size=n) for e, n in zip(exp, (120, 40, 16))]
+    arrivals = [np.sort(rng.uniform(0.0, 0.05, size=len(p))) for p in picks]
+    classes = _build(ref_dev, pools_dev, picks, arrivals)
+    _say("case: serving paced mixed stream", [c.n for c in classes])
+    res = S.run(classes, timeout_s=30.0)
+    assert res.completed == sum(c.n for c in classes)
+    assert not res.timed_out and not res.waves_gave_up
+    for c, e in zip(classes, exp):
+        _check(c, e, c.name)
+        assert np.all(c.complete_s >= c.arrival_s)
+
+
+def test_serving_all_at_once_and_alone_identical(mix):
+    """Every request arrives at t = 0 (saturation); then the noisy class alone.
+    Outputs equal the oracle's in both, i.e. independent of the mix."""
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    picks = [np.arange(len(e)) for e in exp]
+    arrivals = [np.zeros(len(p)) for p in picks]
+    classes = _build(ref_dev, pools_dev, picks, arrivals, slots=(4, 8, 3))
+    _say("case: serving saturation", [c.n for c in classes])
+    res = S.run(classes, timeout_s=30.0)
+    assert res.completed == sum(c.n for c in classes) and not res.timed_out
+    for c, e in zip(classes, exp):
+        _check(c, e, c.name + "/sat")
+    _say("case: serving noisy class alone")
+    alone = S.make_class("noisy-alone", ref_dev, pools_dev[2], picks[2][::-1].copy(),
+                         np.linspace(0.0, 0.01, len(picks[2])), SC, 2)
+    res = S.run([alone], timeout_s=30.0)
+    assert res.completed == alone.n and not res.timed_out
+    _check(alone, exp[2], "noisy-alone")
+
+
+def test_serving_rejects_bad_inputs(mix):
+    from graphmambaformer_amd import kernels as K
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _say("case: serving invalid inputs")
+    with pytest.raises(ValueError):
+        S.make_class("x", ref_dev, pools_dev[0], np.array([0, 1]), np.zeros(3), SC, 2)
+    with pytest.raises(ValueError):  # a 16-lane tile holds several pairs per wavefront
+        S.make_class("x", ref_dev, pools_dev[0], np.array([0]), np.zeros(1), SC, 2,
+                     tile=K.resolve_tile(32, 5, 16))
+    c = S.make_class("x", ref_dev, pools_dev[0], np.array([0, 1]), np.array([0.0, 1.0]), SC, 2)
+    c.arrival_s = np.array([1.0, 0.0])  # not nondecreasing: rejected before any launch
+    with pytest.raises(ValueError):
+        S.run([c], timeout_s=5.0)
+    c = S.make_class("x", ref_dev, pools_dev[1], np.array([0]), np.zeros(1), SC, 2)
+    c.slab_words = 1  # too small for the query
+    with pytest.raises(ValueError):
+        S.run([c], timeout_s=5.0)
+    torch.cuda.synchronize()

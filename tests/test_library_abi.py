@@ -82,3 +82,29 @@ def test_pack_host_layout():
     assert (w[0] & 0xFF) == 0b11100100  # A C G T -> 0 1 2 3
     assert (m[0] >> 8) & 0xF == 0b0101  # N and padding masked
     assert w[1] == 0xFFFFFFFF and (w[2] & 3) == 3
+
+
+def test_serve_run_validates_before_touching_the_device():
+    import ctypes
+
+    import pytest
+
+    from graphmambaformer_amd import _lib
+
+    L = _lib.lib()
+    rep = _lib.ServeReport()
+    assert L.gmfa_serve_run(None, 1, 1.0, ctypes.byref(rep)) == _lib.GMFA_E_INVALID
+    arr = (_lib.ServeClass * 1)()
+    assert L.gmfa_serve_run(arr, 0, 1.0, ctypes.byref(rep)) == _lib.GMFA_E_INVALID
+    assert L.gmfa_serve_run(arr, 1, -1.0, ctypes.byref(rep)) == _lib.GMFA_E_INVALID
+    # a class with requests but no arrival times
+    arr[0].batch.n_pairs = 3
+    assert L.gmfa_serve_run(arr, 1, 1.0, ctypes.byref(rep)) == _lib.GMFA_E_INVALID
+    t = (ctypes.c_double * 3)(0.0, 2.0, 1.0)  # not nondecreasing
+    c = (ctypes.c_double * 3)()
+    arr[0].arrival_s = ctypes.cast(t, ctypes.c_void_p)
+    arr[0].complete_s = ctypes.cast(c, ctypes.c_void_p)
+    assert L.gmfa_serve_run(arr, 1, 1.0, ctypes.byref(rep)) == _lib.GMFA_E_INVALID
+    assert b"nondecreasing" in L.gmfa_last_error()
+    with pytest.raises(ValueError):
+        _lib.check(L.gmfa_serve_run(arr, 1, 1.0, ctypes.byref(rep)))
