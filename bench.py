@@ -140,6 +140,92 @@ def cpu_gcups(w, idx, threads):
     return float(w.cells()[idx].sum()) / s / 1e9, s
 
 
+SERVE_SLOTS = (512, 2048, 1024)  # persistent wavefronts of the short / long / noisy classes
+
+
+def serving_extra(torch, WL, dev, n_req, load):
+    """p50 / p99 latency of a Poisson request stream at a fixed rate.
+
+    A resident 46 Mbp reference; requests 60% short (150 bp, half_band 32), 30%
+    long (lognormal 12 kb, half_band 256), 10% noisy (lognormal 8 kb clipped
+    0.5-100 kb, 6% errors, half_band 751), drawn from pools of distinct
+    windows.  First a saturation run (every request at t = 0) gives the
+    capacity; then the paced run offers `load` x capacity requests/s.
+    Latency = completion seen by the host - arrival; the first 10% of arrivals
+    are warm-up."""
+    from graphmambaformer_amd import serving as S
+
+    t0 = time.time()
+    mix = WL.serving_mix()
+    ref = S.ResidentReference.upload(mix.reference, dev)
+    pools = [S.RequestPool.upload(p.queries, p.t_start, p.t_len, p.band_offset, p.half_band, dev)
+             for p in mix.pools]
+    cls, idx = WL.serving_requests(mix, n_req)
+    pool_cells = [p.cells() for p in mix.pools]
+    req_cells = np.array([pool_cells[c][k] for c, k in zip(cls, idx)], np.int64)
+    total_cells = int(req_cells.sum())
+    sc = (mix.match, mix.mismatch, mix.gap_open, mix.gap_extend)
+    log(f"[bench] serving stream: {n_req} requests, {total_cells:.3e} cells, "
+        f"{time.time() - t0:.1f}s")
+
+    def build(arrivals):
+        out = []
+        for c, p in enumerate(mix.pools):
+            sel = np.flatnonzero(cls == c)
+            out.append(S.make_class(p.name, ref, pools[c], idx[sel], arrivals[sel], sc,
+                                    SERVE_SLOTS[c]))
+        return out
+
+    def gather(classes):
+        comp = np.empty(n_req)
+        for c, k in enumerate(classes):
+            comp[np.flatnonzero(cls == c)] = k.complete_s
+        return comp
+
+    sat = build(np.zeros(n_req))
+    r_sat = S.run(sat, timeout_s=300.0)
+    capacity = n_req / r_sat.wall_s
+    del sat
+    torch.cuda.empty_cache()
+    log(f"[bench] serving saturation: {r_sat.wall_s * 1e3:.1f} ms -> {capacity:.0f} req/s, "
+        f"{total_cells / r_sat.wall_s / 1e9:.1f} GCUPS")
+
+    rate = load * capacity
+    arr = WL.poisson_arrivals(n_req, rate)
+    paced = build(arr)
+    r = S.run(paced, timeout_s=300.0)
+    comp = gather(paced)
+    skip_t = arr[n_req // 10]
+    keep = arr >= skip_t
+    lat_all = S.latency_summary(arr[keep], comp[keep])
+    per_class = {}
+    for c, p in enumerate(mix.pools):
+        m = keep & (cls == c)
+        per_class[p.name] = S.latency_summary(arr[m], comp[m])
+    del paced
+    res = {
+        "qps": round(rate, 1),
+        "offered_load": load,
+        "capacity_qps": round(capacity, 1),
+        "saturation_gcups": round(total_cells / r_sat.wall_s / 1e9, 2),
+        "paced_wall_s": round(r.wall_s, 4),
+        "paced_gcups": round(total_cells / r.wall_s / 1e9, 2),
+        "latency": lat_all,
+        "latency_by_class": per_class,
+        "completed": r.completed,
+        "timed_out": r.timed_out,
+        "slots": dict(zip([p.name for p in mix.pools], SERVE_SLOTS)),
+        "workload": f"{n_req} Poisson requests, 60% short (150 bp, half_band 32) / 30% long "
+                    "(lognormal 12 kb, half_band 256) / 10% noisy (lognormal 8 kb, 6% errors, "
+                    "half_band 751) against a resident 46 Mbp 2-bit reference; pools of "
+                    "2048/1024/512 distinct windows, seed 4; first 10% of arrivals are warm-up; "
+                    "queries pre-staged in HBM",
+    }
+    log(f"[bench] serving paced at {rate:.0f} req/s: {res['latency']} by class "
+        f"{per_class}")
+    return res
+
+
 def host_info():
     model = "unknown"
     try:
@@ -163,6 +249,11 @@ def main():
                     help="CIGAR walk inside the forward launch (fused) or as its own launch "
                          "(one lane per pair)")
     ap.add_argument("--cpu-sample-stride", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=200_000,
+                    help="requests of the serving stream (0: skip the serving measurement)")
+    ap.add_argument("--serve-load", type=float, default=0.65,
+                    help="offered load of the paced serving run, as a fraction of the "
+                         "saturation capacity")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -386,6 +477,11 @@ def main():
     }
     log(f"[bench] sw_noisy: {msn:.1f} ms -> {result['sw_noisy']['batched_gcups']} GCUPS, "
         f"longest pair alone {ms1:.1f} ms, per-pair {ppn:.3f}, CPU {cgn:.3f}")
+
+    # ----------------------------------------------------------------- serving
+    if args.serve_requests > 0:
+        result["serving"] = serving_extra(torch, WL, dev, args.serve_requests, args.serve_load)
+        torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ chains
     from oracle import oracle as O

@@ -94,7 +94,11 @@ struct SwKArgs {
 
 // Wait until request `item` is published or the host closes the queue; returns
 // the number of published requests.  Every wait ends: at the latest after
-// deadline_ticks of the constant-rate wall clock (s_sleep 127 = 8128 cycles).  (Scalars by value: taking
+// deadline_ticks of the constant-rate wall clock (s_sleep 127 = 8128 cycles).
+// The loads are relaxed (system scope, so they reach host memory, but with no
+// acquire): the request table and sequences are written before the launch,
+// only the tail value itself travels while the kernel runs, and an acquire
+// would invalidate the L2 under every working wavefront at each poll.  (Scalars by value: taking
 // the kernel argument block by reference would move it to scratch memory.)
 __device__ __forceinline__ int serve_wait(const int32_t *tail_p, const int32_t *closed_p,
                                           int32_t *gave_up, uint64_t deadline_ticks, int item)
@@ -104,14 +108,14 @@ __device__ __forceinline__ int serve_wait(const int32_t *tail_p, const int32_t *
     int nap = 1;
     for (;;) {
         const int tail = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(tail_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM));
+            __hip_atomic_load(tail_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
         if (item < tail) return tail;
         const int closed = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(closed_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM));
+            __hip_atomic_load(closed_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
         if (closed) {
             // the host publishes the final tail before closing
             return __builtin_amdgcn_readfirstlane(
-                __hip_atomic_load(tail_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM));
+                __hip_atomic_load(tail_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
         }
         if ((uint64_t)wall_clock64() - t0 > deadline_ticks) {
             if (threadIdx.x == 0)
@@ -408,8 +412,10 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         // serving: one pair per wavefront (G == 64), bounded by what the host
         // has published
         int n_items = a.n_items;
-        if constexpr (SV)
+        if constexpr (SV) {
+            if (item >= a.n_pairs) break;  // past the class's last request: nothing to wait for
             n_items = serve_wait(a.serve_tail, a.serve_closed, a.serve_gave_up, a.deadline_ticks, item);
+        }
         if (item >= n_items) break;
 
         const int pidx = item * (64 / G) + grp;
