@@ -136,7 +136,9 @@ def banded_affine_sw_batch(query: torch.Tensor, target: torch.Tensor, query_len:
     inp = _inputs_from_codes(query, target, qlen_h, tlen_h, off_h, dev)
     xdrop = float(cfg.x_drop) > 0
     fwd = K.sw_forward(inp, ma, mi, go, ge, hb, traceback=return_matrices, rowkeys=xdrop)
-    if xdrop:
+    if xdrop and B <= 64:
+        sc_d, qe_d, te_d = K.sw_xdrop(fwd, inp, np.array([0, B]), float(cfg.x_drop), hb)
+    elif xdrop:
         sc, qe, te = K.xdrop_best(fwd, inp.qlen_host, float(cfg.x_drop), hb, inp.boff_host)
         sc_d = torch.from_numpy(sc).to(dev)
         qe_d = torch.from_numpy(qe).to(dev)
@@ -176,6 +178,7 @@ class ExtensionEngine:
         self.cfg = cfg or ExtensionConfig()
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.backend = backend
+        self._ref_cache = None
 
     def _window(self, chain: Chain, anchors: AnchorSet, read_len: int, ref_len: int):
         """(target_start, target_end, half_band) -- extension.py:489-509."""
@@ -191,52 +194,119 @@ class ExtensionEngine:
             spread = int(dg.max() - dg.min())
         return start, end, min(self.cfg.half_band + spread, self.cfg.max_half_band)
 
-    def _prepare(self, read_seq: str, chains: Sequence[Chain], anchors: AnchorSet,
-                 ref_codes: np.ndarray):
-        fwd = encode_bases(read_seq)
-        rev = reverse_complement_codes(fwd)
-        read_len, ref_len = len(fwd), len(ref_codes)
-        wins = [self._window(c, anchors, read_len, ref_len) for c in chains]
-        hb = max(w[2] for w in wins)
-        width = max(w[1] - w[0] for w in wins)
-        B = len(chains)
-        q = np.full((B, read_len), -1, np.int16)
-        t = np.full((B, max(width, 1)), -1, np.int16)
-        tlen = np.zeros(B, np.int64)
-        off = np.zeros(B, np.int64)
-        for b, (c, (start, end, _)) in enumerate(zip(chains, wins)):
-            codes = fwd if c.strand > 0 else rev
-            q[b, :len(codes)] = codes
-            w = ref_codes[start:end]
-            t[b, :len(w)] = w
-            tlen[b] = len(w)
-            off[b] = (c.ref_start - c.read_start) - start
-        return q, t, np.full(B, read_len, np.int64), tlen, off, hb, wins, read_len
+    def _reference(self, ref_seq: str):
+        """The reference as a resident 2-bit stream + N mask (cached for the
+        last reference seen); windows are then offsets, never copies."""
+        key = (len(ref_seq), hash(ref_seq))
+        if self._ref_cache is None or self._ref_cache[0] != key:
+            codes = encode_bases(ref_seq)
+            w, m, _, has_n = K.pack_host([codes], alphabet="ref")
+            dev = self.device
+            self._ref_cache = (key, torch.from_numpy(w.view(np.int32)).to(dev),
+                               torch.from_numpy(m.view(np.int32)).to(dev), has_n, len(codes))
+        return self._ref_cache[1:]
 
     def extend_chains(self, read_seq: str, chains: Sequence[Chain], anchors: AnchorSet,
                       ref_seq: str) -> list:
         """One ExtensionResult per chain (extension.py:511-641)."""
         if not chains:
             return []
-        if self.cfg.algorithm == "wfa":
-            raise NotImplementedError("the wavefront aligner is not part of this library")
-        ref_codes = encode_bases(ref_seq)
-        q, t, ql, tl, off, hb, wins, read_len = self._prepare(read_seq, chains, anchors,
-                                                               ref_codes)
-        dev = self.device
-        res = banded_affine_sw_batch(torch.from_numpy(q).to(dev), torch.from_numpy(t).to(dev),
-                                     torch.from_numpy(ql), torch.from_numpy(tl), self.cfg,
-                                     half_band=hb, band_offset=torch.from_numpy(off),
-                                     return_matrices=True)
-        return self._finish(res, chains, wins, read_len)
+        return self.extend_batch([read_seq], [chains], [anchors], ref_seq)[0]
 
-    def _finish(self, res: BandedDPResult, chains, wins, read_len):
-        out = []
-        for b, (chain, (start, _e, _)) in enumerate(zip(chains, wins)):
-            ext = traceback_banded(res, b, None, None, self.cfg)
-            ext.ref_start += start
-            ext.ref_end += start
-            ext.cigar = merge_cigar([("S", ext.read_start)] + ext.cigar +
-                                    [("S", read_len - ext.read_end)])
-            out.append(ext)
-        return out
+    def extend_batch(self, reads: Sequence[str], chains: Sequence[Sequence[Chain]],
+                     anchors: Sequence[AnchorSet], ref_seq: str) -> list:
+        """extend_chains over many reads with one forward launch per distinct
+        half-band (not one per read).
+
+        Each read's chains form one reference batch (extension.py:534-585):
+        they share the read's half-band (the max over its chains) and the
+        batch-wide x-drop, applied per read on the device.  Reads are packed
+        once (forward + reverse complement), windows are offsets into the
+        resident reference, so no per-chain copy is made; soft clips and the
+        coordinate lift (extension.py:628-640) are applied to the results."""
+        cfg = self.cfg
+        if cfg.algorithm == "wfa":
+            raise NotImplementedError("the wavefront aligner is not part of this library")
+        if not (len(reads) == len(chains) == len(anchors)):
+            raise ValueError("reads, chains and anchors must have one entry per read")
+        ma, mi, go, ge = integer_scoring(cfg)
+        out: list = [[] for _ in reads]
+        live = [r for r in range(len(reads)) if len(chains[r])]
+        if not live:
+            return out
+        ref_words, ref_masks, ref_has_n, ref_len = self._reference(ref_seq)
+        dev = self.device
+        # queries: forward and reverse complement of every read, packed once
+        seqs, lens = [], np.zeros(len(reads), np.int64)
+        for r in range(len(reads)):
+            f = encode_bases(reads[r])
+            lens[r] = len(f)
+            seqs.append(f)
+            seqs.append(reverse_complement_codes(f))
+        qw, qm, qoff, q_has_n = K.pack_host(seqs, alphabet="ref")
+        has_n = bool(q_has_n or ref_has_n)
+        qseq = torch.from_numpy(qw.view(np.int32)).to(dev)
+        qmask = torch.from_numpy(qm.view(np.int32)).to(dev) if has_n else None
+        # windows and per-read half-bands
+        by_hb: dict = {}
+        geo = {}
+        for r in live:
+            L = int(lens[r])
+            wins = [self._window(c, anchors[r], L, ref_len) for c in chains[r]]
+            hb = max(w[2] for w in wins)
+            geo[r] = wins
+            by_hb.setdefault(hb, []).append(r)
+        xd = float(cfg.x_drop)
+        for hb, rs in sorted(by_hb.items()):
+            q_off, t_off, ql, tl, bo, gofs, owner = [], [], [], [], [], [0], []
+            for r in rs:
+                L = int(lens[r])
+                for k, (c, (start, end, _)) in enumerate(zip(chains[r], geo[r])):
+                    q_off.append(int(qoff[2 * r + (0 if c.strand > 0 else 1)]))
+                    t_off.append(min(int(start), ref_len))
+                    tl.append(max(0, min(int(end), ref_len) - int(start)))
+                    ql.append(L)
+                    bo.append(int(c.ref_start - c.read_start) - int(start))
+                    owner.append((r, k, int(start)))
+                gofs.append(len(owner))
+            ql_h = np.array(ql, np.int32)
+            tl_h = np.array(tl, np.int32)
+            bo_h = np.array(bo, np.int32)
+
+            def up(a):
+                return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+            inp = K.SwInputs(qseq=qseq, tseq=ref_words, qmask=qmask,
+                             tmask=ref_masks if has_n else None,
+                             q_off=up(np.array(q_off, np.int64)),
+                             t_off=up(np.array(t_off, np.int64)), qlen=up(ql_h), tlen=up(tl_h),
+                             band_offset=up(bo_h), qlen_host=ql_h, tlen_host=tl_h,
+                             boff_host=bo_h, has_n=has_n)
+            gofs = np.array(gofs, np.int64)
+            if xd > 0:
+                fwd = K.sw_forward(inp, ma, mi, go, ge, hb, traceback=True, rowkeys=True)
+                if int(np.diff(gofs).max()) <= 64:
+                    sc, qe, te = K.sw_xdrop(fwd, inp, gofs, xd, hb)
+                else:
+                    sc_h, qe_h, te_h = K.xdrop_best(fwd, ql_h, xd, hb, bo_h, group_off=gofs)
+                    sc, qe, te = up(sc_h), up(qe_h), up(te_h)
+                tbk = K.sw_traceback(inp, fwd, sc, qe, te)
+            else:
+                fwd = K.sw_forward(inp, ma, mi, go, ge, hb, traceback=True, cigar=True)
+                sc, qe, te, tbk = fwd.score, fwd.query_end, fwd.target_end, fwd.walk
+            sc_h, qe_h, te_h = sc.cpu().numpy(), qe.cpu().numpy(), te.cpu().numpy()
+            rs_h, ts_h = tbk.read_start.cpu().numpy(), tbk.ref_start.cpu().numpy()
+            cnt = tbk.counts.cpu().numpy()
+            cigs = K.decode_cigars(tbk)
+            for p, (r, k, start) in enumerate(owner):
+                L = int(lens[r])
+                rs0, qe0 = int(rs_h[p]), int(qe_h[p])
+                ext = ExtensionResult(
+                    score=float(sc_h[p]),
+                    cigar=merge_cigar([("S", rs0)] + list(cigs[p]) + [("S", L - qe0)]),
+                    read_start=rs0, read_end=qe0, ref_start=int(ts_h[p]) + start,
+                    ref_end=int(te_h[p]) + start, n_match=int(cnt[p, 0]),
+                    n_mismatch=int(cnt[p, 1]), n_insertion=int(cnt[p, 2]),
+                    n_deletion=int(cnt[p, 3]))
+                out[r].append((k, ext))
+        return [[e for _, e in sorted(lst, key=lambda x: x[0])] for lst in out]

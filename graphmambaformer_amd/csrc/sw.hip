@@ -606,12 +606,14 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 const int tk = rk - (int)((uint32_t)(i * mi) << 5);
                 if (tk > (bestK | 31)) { bestK = tk; besti = i; }
                 if constexpr (RK) {
-                    // lane-local (max H, first d) of this row, stored per block;
-                    // the host takes the max over a row's lanes (keys order by H,
-                    // then by smaller d)
+                    // the row's (max H, first d) as one key (keys order by H, then
+                    // by smaller d): lane-local key, then a max over the group;
+                    // the group's last lane ends up holding it
                     const int htrue = (rk >> 5) - i * mi;
                     const int d = gl * C + 31 - (rk & 31);
-                    rkeys[r] = (int)(((uint32_t)htrue << 11) | (uint32_t)(2047 - d));
+                    const int key = lane_live ? (int)(((uint32_t)htrue << 11) | (uint32_t)(2047 - d))
+                                              : INT_MIN;
+                    rkeys[r] = seg_scan_max<G>(key);
                 }
             };
             // Hp[k] is overwritten in pass 2 only after pass 1 consumed it, and
@@ -625,11 +627,10 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 #pragma unroll
                 for (int r = 0; r < 8; ++r) row(std::true_type{}, r);
             }
-            if (RK && has && lane_live) {
+            if (RK && has && gl == G - 1) {
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
-                    if (i0 + r <= qlen)
-                        a.rowkey[rk_base + (int64_t)(i0 + r - 1) * (lstar + 1) + gl] = rkeys[r];
+                    if (i0 + r <= qlen) a.rowkey[rk_base + (int64_t)(i0 + r - 1)] = rkeys[r];
             }
             if (TB && has && lane_live && b < nblk_mine) {
                 uint32_t *dst = a.tb + tb_base + ((int64_t)b * (lstar + 1) + gl) * C;
@@ -694,6 +695,56 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 }
 
 __global__ void reset_counter_kernel(uint32_t *c) { *c = 0u; }
+
+// Batch-wide x-drop (extension.py:186-193) per group of pairs: the reference
+// stops a batch after the first row where every pair's best-so-far minus the
+// row's best exceeds x_drop; the result is the first strict maximum over the
+// rows up to and including that one.  One wavefront per group, lane k = pair
+// group_off[g] + k (groups of at most 64 pairs); rows are read 16 at a time.
+__global__ __launch_bounds__(64) void sw_xdrop_kernel(const int32_t *rowkey, const int64_t *rk_off,
+                                                      const int32_t *qlen, const int32_t *boff,
+                                                      const int32_t *group_off, int32_t thr,
+                                                      int32_t hb, int32_t *score, int32_t *qend,
+                                                      int32_t *tend)
+{
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int p0 = group_off[g];
+    const int n = min(group_off[g + 1] - p0, 64);
+    const bool act = lane < n;
+    const int p = p0 + (act ? lane : 0);
+    const int ql = act ? qlen[p] : 0;
+    const int64_t base = act ? rk_off[p] : 0;
+    const int maxq = wave_max(ql);
+    const uint64_t need = __builtin_amdgcn_ballot_w64(act);
+    int seen = 0, bh = 0, bi = 0, bd = 0;
+    bool stop = false;
+    for (int i0 = 1; i0 <= maxq && !stop; i0 += 16) {
+        int kk[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = i0 + r;
+            kk[r] = (act && i <= ql) ? rowkey[base + i - 1] : 0;  // rows past qlen: H = 0
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = i0 + r;
+            if (i > maxq) break;
+            const int h = kk[r] >> 11;
+            if (h > bh) { bh = h; bi = i; bd = 2047 - (kk[r] & 2047); }
+            seen = max(seen, h);
+            if (thr >= 0 && (__builtin_amdgcn_ballot_w64(act && seen - h > thr) & need) == need) {
+                stop = true;
+                break;
+            }
+        }
+    }
+    if (act) {
+        score[p] = bh;
+        qend[p] = bh > 0 ? bi : 0;
+        tend[p] = bh > 0 ? bi + boff[p] - hb + bd : 0;
+    }
+}
 
 // -------------------------------------------------------------------------
 // int16 codes -> 2-bit stream + N mask; one thread per output word
@@ -1020,6 +1071,27 @@ extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params 
                                const gmfa_sw_out *out, void *stream)
 {
     return sw_launch(batch, params, out, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int gmfa_sw_xdrop(const int32_t *rowkey, const int64_t *rowkey_off,
+                             const int32_t *qlen, const int32_t *band_offset,
+                             const int32_t *group_off, int32_t n_groups, int32_t max_group,
+                             int32_t x_drop, int32_t half_band, int32_t *score,
+                             int32_t *query_end, int32_t *target_end, void *stream)
+{
+    GMFA_REQUIRE(n_groups >= 0, "negative n_groups");
+    if (n_groups == 0) return GMFA_OK;
+    GMFA_REQUIRE(max_group >= 1 && max_group <= 64,
+                 "groups must hold 1..64 pairs (max_group %d)", max_group);
+    GMFA_REQUIRE(half_band >= 0 && half_band <= 1023, "half_band %d outside [0, 1023]", half_band);
+    GMFA_REQUIRE(rowkey && rowkey_off && qlen && band_offset && group_off && score && query_end &&
+                     target_end,
+                 "NULL pointer");
+    hipLaunchKernelGGL(sw_xdrop_kernel, dim3(n_groups), dim3(64), 0, (hipStream_t)stream, rowkey,
+                       rowkey_off, qlen, band_offset, group_off, x_drop < 0 ? -1 : x_drop,
+                       half_band, score, query_end, target_end);
+    GMFA_CHECK_HIP(hipGetLastError());
+    return GMFA_OK;
 }
 
 extern "C" int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_params *params,

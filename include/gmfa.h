@@ -14,6 +14,8 @@
  *   gmfa_sw_forward    graphmambaformer/alignment/extension.py:78-204
  *                      banded_affine_sw_batch (and the NVIDIA tier
  *                      accel/cuda_kernels.py:504-566 banded_sw)
+ *   gmfa_sw_xdrop      graphmambaformer/alignment/extension.py:186-193
+ *                      (batch-wide x-drop, one group per reference batch)
  *   gmfa_sw_traceback  graphmambaformer/alignment/extension.py:207-302
  *                      traceback_banded
  *   gmfa_pack_codes    graphmambaformer/alignment/seeding.py:46-80 code
@@ -97,10 +99,9 @@ typedef struct gmfa_sw_out {
     int32_t *target_end;   /* [n_pairs] 1-based column of the best cell */
     uint32_t *tb;          /* traceback planes (GMFA_SW_TRACEBACK) */
     const int64_t *tb_off; /* [n_pairs] dword offset of each pair's planes */
-    int32_t *rowkey;       /* per-row keys (GMFA_SW_ROWKEYS): qlen x lanes_stored
-                              int32 per pair, key = H << 11 | (2047 - d) of a
-                              lane's first row maximum; max over a row = the row's
-                              (max H, first column) */
+    int32_t *rowkey;       /* per-row keys (GMFA_SW_ROWKEYS): qlen int32 per pair,
+                              key of row i = H << 11 | (2047 - d) for the row's
+                              maximum H at its first band diagonal d */
     const int64_t *rowkey_off; /* [n_pairs] element offset of each pair's keys */
     uint32_t *work_counter;    /* one device uint32 of scratch for the work queue */
     /* GMFA_SW_CIGAR outputs, same layout as gmfa_sw_traceback's */
@@ -149,6 +150,20 @@ int gmfa_pack_codes(const int16_t *codes, int64_t B, int64_t L, const int32_t *l
 /* Forward pass: scores, ends and (optionally) traceback planes / row keys. */
 int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
                     const gmfa_sw_out *out, void *stream);
+
+/* Batch-wide x-drop (graphmambaformer/alignment/extension.py:186-193) per
+ * group of pairs, from the per-row keys of a GMFA_SW_ROWKEYS forward pass.
+ * Group g = pairs group_off[g] .. group_off[g+1]-1 (device int32 array, at
+ * most max_group <= 64 pairs per group), one group = one reference batch:
+ * after the first row where every pair of the group has best-so-far - row
+ * best > x_drop, the group stops; each pair's result is then its first strict
+ * maximum over the rows up to that one.  x_drop = floor of the reference's
+ * x_drop, or -1 when the reference's x_drop <= 0 (no stop).  Writes score,
+ * 1-based query_end / target_end (0 when the score is 0). */
+int gmfa_sw_xdrop(const int32_t *rowkey, const int64_t *rowkey_off, const int32_t *qlen,
+                  const int32_t *band_offset, const int32_t *group_off, int32_t n_groups,
+                  int32_t max_group, int32_t x_drop, int32_t half_band, int32_t *score,
+                  int32_t *query_end, int32_t *target_end, void *stream);
 
 /* Walk the traceback planes from (query_end, target_end) of every pair and
  * write BAM-style CIGAR runs (len << 4 | op, forward order) at

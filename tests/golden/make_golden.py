@@ -20,6 +20,11 @@ else.  Cases target what the brute-force and known-answer checks leave open:
                     DP order, _backtrack and select, incl. equal-f ties.
 * extend.json       ExtensionEngine.extend_chains (extension.py:511-641):
                     per-read shared half-band geometry, soft clips, lifts.
+* seeding.json      hash64 / pack_kmers / minimizer_mask (seeding.py:83-199)
+                    and whole GPUKmerIndex.query outputs (seeding.py:720-798,
+                    its portable lookup path) incl. N bases, repeats, max_occ.
+
+    python tests/golden/make_golden.py [file.json ...]   # default: all files
 """
 
 from __future__ import annotations
@@ -251,16 +256,74 @@ def extend_cases(R, rng):
     return cases
 
 
+def seeding_cases(rng):
+    sys.path.insert(0, REF)
+    from graphmambaformer.alignment import seeding as SD
+
+    bases = np.array(list("ACGTN"))
+    out = {}
+    vals = [0, 1, 2, 3, (1 << 30) - 1, (1 << 62) - 1] + \
+        [int(x) for x in rng.integers(0, 1 << 62, size=40, dtype=np.int64)]
+    out["hash64"] = {"in": vals, "out": [int(x) for x in SD.hash64(np.array(vals, np.int64))]}
+    masks = []
+    for n, (k, w) in enumerate([(5, 1), (7, 4), (15, 10), (21, 5), (11, 16), (3, 3)]):
+        L = int(rng.integers(20, 200))
+        codes = rng.integers(0, 4, size=L)
+        if n % 2:
+            codes[rng.integers(0, L, size=2)] = 4
+        seq = "".join(bases[codes])
+        packed, valid = SD.pack_kmers(SD.encode_bases(seq), k)
+        sel = valid & SD.minimizer_mask(SD.hash64(packed), w)
+        masks.append({"seq": seq, "k": k, "w": w, "packed": [int(x) for x in packed],
+                      "valid": np.flatnonzero(valid).tolist(),
+                      "selected": np.flatnonzero(sel).tolist()})
+    out["kmers"] = masks
+    queries = []
+    for n, (k, w, occ) in enumerate([(15, 10, 0), (11, 5, 3), (7, 4, 0), (19, 8, 0)]):
+        unit = rng.integers(0, 4, size=300)
+        codes = np.concatenate([rng.integers(0, 4, size=6000), np.tile(unit[:37], 40),
+                                rng.integers(0, 4, size=6000), unit, unit,
+                                rng.integers(0, 4, size=3000)])
+        codes[rng.integers(0, len(codes), size=30)] = 4
+        ref = "".join(bases[codes])
+        idx = SD.GPUKmerIndex(SD.encode_bases(ref), k=k, window=w, max_occ=occ, device="cpu")
+        reads = []
+        for r in range(6):
+            if r == 5:
+                read = ref[100:100 + k - 1]  # shorter than k
+            else:
+                s0 = int(rng.integers(0, len(ref) - 1500))
+                read = list(ref[s0:s0 + int(rng.integers(200, 1500))])
+                for _ in range(int(rng.integers(0, 12))):
+                    read[int(rng.integers(0, len(read)))] = "ACGTN"[int(rng.integers(0, 5))]
+                read = "".join(read)
+                if r == 3:
+                    read = "".join({"A": "T", "C": "G", "G": "C", "T": "A"}.get(c, "N")
+                                   for c in reversed(read))
+            rp, fp, ln = idx.query(SD.encode_bases(read))
+            reads.append({"read": read, "read_pos": rp.tolist(), "ref_pos": fp.tolist(),
+                          "length": ln.tolist()})
+        queries.append({"ref": ref, "k": k, "window": w, "max_occ": occ,
+                        "n_keys": int(len(idx.keys)), "reads": reads})
+    out["queries"] = queries
+    return out
+
+
 def main():
+    want = set(sys.argv[1:])
     R = _imports()
     rng = np.random.default_rng(20261203)
     out = {
-        "sw_ties.json": sw_cases(R, rng),
-        "chain_dp.json": chain_cases(R, rng),
-        "chainer.json": chainer_cases(R, rng),
-        "extend.json": extend_cases(R, rng),
+        "sw_ties.json": lambda: sw_cases(R, rng),
+        "chain_dp.json": lambda: chain_cases(R, rng),
+        "chainer.json": lambda: chainer_cases(R, rng),
+        "extend.json": lambda: extend_cases(R, rng),
+        "seeding.json": lambda: seeding_cases(np.random.default_rng(20261204)),
     }
-    for name, data in out.items():
+    for name, make in out.items():
+        data = make()  # in file order: every file draws from the shared stream
+        if want and name not in want:
+            continue
         with open(os.path.join(HERE, name), "w") as fh:
             json.dump(data, fh, separators=(",", ":"))
         print(name, os.path.getsize(os.path.join(HERE, name)), "bytes")

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 import torch
 
+from oracle import oracle as O
 from tests.golden.load import fhex, load
 
 pytestmark = pytest.mark.gpu
@@ -179,3 +180,69 @@ def test_invalid_inputs_raise():
     with pytest.raises(ValueError):
         banded_affine_sw_batch(q, q, torch.tensor([4]), torch.tensor([4]),
                                ExtensionConfig(match_score=1.5), half_band=2)
+
+
+def _batch_case(seed, n_reads, x_drop, with_n):
+    """Reads sampled from one reference (both strands, decoy chains, anchor
+    spreads that give different half-bands per read)."""
+    from graphmambaformer_amd import workloads as WL
+    from graphmambaformer_amd.alignment import AnchorSet, Chain
+
+    rng = np.random.default_rng(seed)
+    ref = rng.integers(0, 4, size=300_000).astype(np.uint8)
+    if with_n:
+        ref[1000:1040] = 4
+    alpha = np.array(list("ACGTN"))
+    comp = np.array([3, 2, 1, 0, 4])
+    ref_s = "".join(alpha[ref])
+    reads, chains, anchors, ora = [], [], [], []
+    for r in range(n_reads):
+        L = int(rng.integers(150, 2500))
+        pos = int(rng.integers(0, len(ref) - L - 10)) if r % 7 else 900
+        seg = WL.mutate(rng, np.minimum(ref[pos:pos + L], 3), 0.03, 0.5, 3)
+        strand = 1 if r % 2 == 0 else -1
+        fwd_codes = seg if strand > 0 else comp[seg[::-1]]
+        reads.append("".join(alpha[fwd_codes]))
+        spread = int(rng.integers(0, 30))
+        dg = np.array([pos, pos + spread, pos + spread // 2], np.int64)
+        a = AnchorSet.from_lists([0, 50, 100], (dg + [0, 50, 100]).tolist(), [15, 15, 15],
+                                 [strand] * 3, read_len=len(seg), ref_len=len(ref))
+        cs = [dict(strand=strand, read_start=0, ref_start=pos, anchor_idx=[0, 1, 2])]
+        if r % 3 == 0:  # a decoy chain somewhere else, other strand
+            d = int(rng.integers(0, len(ref) - L))
+            cs.append(dict(strand=-strand, read_start=10, ref_start=d, anchor_idx=[2]))
+        chains.append([Chain(anchor_idx=np.array(c["anchor_idx"]), score=30.0,
+                             strand=c["strand"], read_start=c["read_start"], read_end=L,
+                             ref_start=c["ref_start"], ref_end=c["ref_start"] + L) for c in cs])
+        anchors.append(a)
+        ora.append((cs, a.diagonal))
+    return ref_s, reads, chains, anchors, ora
+
+
+@pytest.mark.parametrize("x_drop,with_n", [(600.0, False), (25.0, True), (0.0, True)])
+def test_extension_engine_batch_vs_oracle(x_drop, with_n):
+    """extend_batch over many reads (one launch per half-band, per-read x-drop
+    on the device, windows as offsets into the resident reference) equals the
+    oracle's per-read extend_chains."""
+    from graphmambaformer_amd.alignment import ExtensionEngine
+    from graphmambaformer_amd.config import ExtensionConfig
+
+    _say("case: extend_batch", x_drop, with_n)
+    ref_s, reads, chains, anchors, ora = _batch_case(int(x_drop) + 5, 36, x_drop, with_n)
+    cfg = ExtensionConfig(half_band=16, max_half_band=40, flank=60, x_drop=x_drop)
+    eng = ExtensionEngine(cfg, device=DEV)
+    got = eng.extend_batch(reads, chains, anchors, ref_s)
+    for r, (cs, diag) in enumerate(ora):
+        exp = O.extend_chains(reads[r], cs, diag, ref_s, half_band=16, max_half_band=40,
+                              flank=60, max_window=cfg.max_window, ma=2, mi=4, go=6, ge=2,
+                              x_drop=x_drop)
+        assert len(got[r]) == len(exp), r
+        for g, e in zip(got[r], exp):
+            ctx = (r, x_drop)
+            assert g.score == e["score"], ctx
+            assert [list(x) for x in g.cigar] == [list(x) for x in e["cigar"]], ctx
+            assert (g.read_start, g.read_end, g.ref_start, g.ref_end) == (
+                e["read_start"], e["read_end"], e["ref_start"], e["ref_end"]), ctx
+    # single-read calls agree with the batch
+    one = eng.extend_chains(reads[3], chains[3], anchors[3], ref_s)
+    assert [(x.score, x.cigar) for x in one] == [(x.score, x.cigar) for x in got[3]]

@@ -181,6 +181,11 @@ def test_golden_sw_ties_with_batch_xdrop():
         fwd = k.sw_forward(inp, ma, mi, go, ge, hb, traceback=True, rowkeys=True)
         sc, qe, te = k.xdrop_best(fwd, inp.qlen_host, case["x_drop"], hb, inp.boff_host)
         dev = inp.device
+        if inp.n_pairs <= 64:  # the device x-drop agrees with the host one
+            dsc, dqe, dte = k.sw_xdrop(fwd, inp, np.array([0, inp.n_pairs]), case["x_drop"], hb)
+            assert np.array_equal(dsc.cpu().numpy(), sc), n
+            assert np.array_equal(dqe.cpu().numpy(), qe), n
+            assert np.array_equal(dte.cpu().numpy(), te), n
         tbk = k.sw_traceback(inp, fwd, torch.from_numpy(sc).to(dev), torch.from_numpy(qe).to(dev),
                              torch.from_numpy(te).to(dev))
         cig = k.decode_cigars(tbk)
