@@ -59,6 +59,17 @@ class SwTile(ctypes.Structure):
                 ("half_band", _i32)]
 
 
+class KmerIndex(ctypes.Structure):
+    _fields_ = [("keys", _P), ("offsets", _P), ("positions", _P), ("dir", _P),
+                ("n_keys", _i64), ("k", _i32), ("dir_bits", _i32)]
+
+
+class KmerReads(ctypes.Structure):
+    _fields_ = [("seq", _P), ("mask", _P), ("n_words", _i64), ("read_off", _P),
+                ("read_len", _P), ("pos_off", _P), ("tile_read", _P), ("tile_start", _P),
+                ("n_reads", _i32), ("n_tiles", _i32), ("window", _i32), ("reserved", _i32)]
+
+
 class ServeClass(ctypes.Structure):
     _fields_ = [("batch", SwBatch), ("params", SwParams), ("out", SwOut),
                 ("slab_words", _i64), ("n_slots", _i32), ("reserved", _i32),
@@ -90,6 +101,11 @@ SIGNATURES = {
     "gmfa_chain_dp_i32": (ctypes.c_int, [_P, _P, _P, _P, _i32, _i32, _i64, _i32, _i32, _P,
                                          _P, _P]),
     "gmfa_chain_backtrack": (_i64, [_i64, _P, _P, _i64, ctypes.c_double, _P, _P, _P]),
+    "gmfa_kmer_lookup": (ctypes.c_int, [ctypes.POINTER(KmerIndex), _P, _i64, _P, _P, _P]),
+    "gmfa_kmer_seed": (ctypes.c_int, [ctypes.POINTER(KmerIndex), ctypes.POINTER(KmerReads), _P,
+                                      _P, _P, _P, _P, _P]),
+    "gmfa_kmer_emit": (ctypes.c_int, [ctypes.POINTER(KmerIndex), ctypes.POINTER(KmerReads), _P,
+                                      _P, _P, _P, _P, _P, _P]),
     "gmfa_serve_run": (ctypes.c_int, [ctypes.POINTER(ServeClass), _i32, ctypes.c_double,
                                       ctypes.POINTER(ServeReport)]),
 }

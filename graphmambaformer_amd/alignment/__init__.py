@@ -1,4 +1,4 @@
-"""Drop-in stage modules (chaining + extension) of the alignment path."""
+"""Drop-in stage modules (seeding lookup, chaining, extension) of the alignment path."""
 
 from .chaining import AffineChainer, chain_dp_batched, chain_dp_numpy  # noqa: F401
 from .extension import (  # noqa: F401
@@ -7,5 +7,15 @@ from .extension import (  # noqa: F401
     banded_affine_sw_batch,
     traceback_banded,
 )
-from .seeding import N_CODE, decode_bases, encode_bases, reverse_complement_codes  # noqa: F401
+from .seeding import (  # noqa: F401
+    N_CODE,
+    GPUKmerIndex,
+    KmerTable,
+    decode_bases,
+    encode_bases,
+    hash64,
+    minimizer_mask,
+    pack_kmers,
+    reverse_complement_codes,
+)
 from .types import AnchorSet, Chain, ExtensionResult, merge_cigar, run_length_encode  # noqa: F401

@@ -26,6 +26,10 @@
  *   gmfa_chain_dp_i32  the same DP on the integer profile
  *   gmfa_chain_backtrack graphmambaformer/alignment/chaining.py:530-576
  *                      AffineChainer._backtrack (host code)
+ *   gmfa_kmer_lookup   graphmambaformer/alignment/seeding.py:751-775
+ *                      GPUKmerIndex.lookup (accel/cuda_kernels.py:44-74)
+ *   gmfa_kmer_seed / gmfa_kmer_emit  seeding.py:777-798 GPUKmerIndex.query
+ *                      for a batch of reads
  *   gmfa_serve_run     the per-read extension calls of
  *                      graphmambaformer/alignment/pipeline.py:355-363
  *                      (ExtensionEngine.extend_chains, extension.py:511-641,
@@ -200,6 +204,63 @@ int gmfa_chain_dp_i32(const int64_t *read_end, const int64_t *ref_end, const int
 int64_t gmfa_chain_backtrack(int64_t n, const double *f, const int64_t *parent,
                              int64_t min_anchors, double min_score, int64_t *members,
                              int64_t *chain_start, double *chain_score);
+
+/* ---------------------------------------------------------------------------
+ * Stage-1 k-mer seeding (graphmambaformer/alignment/seeding.py:720-798
+ * GPUKmerIndex; accel/cuda_kernels.py:44-74 kmer_lookup).  The table is the
+ * reference's KmerTable (sorted unique packed k-mers, CSR positions) plus a
+ * directory over the top dir_bits of the 2k-bit key: dir[b] = first key
+ * index with key >> (2k - dir_bits) >= b, for b in [0, 2^dir_bits].
+ * ------------------------------------------------------------------------- */
+typedef struct gmfa_kmer_index {
+    const int64_t *keys;      /* [n_keys] sorted unique packed k-mers (device) */
+    const int64_t *offsets;   /* [n_keys + 1] CSR offsets into positions */
+    const int64_t *positions; /* reference offsets */
+    const int32_t *dir;       /* [2^dir_bits + 1] bucket starts */
+    int64_t n_keys;
+    int32_t k;                /* 1..31 */
+    int32_t dir_bits;         /* 0..min(2k, 28) */
+} gmfa_kmer_index;
+
+/* Reads of one seeding batch: a 2-bit stream + optional N mask (the layout of
+ * gmfa_sw_batch), per-read base offsets and lengths, k-mer position offsets
+ * pos_off[r] = sum over r' < r of max(0, len - k + 1) (n_reads + 1 entries)
+ * and 256-position tiles (tile t covers positions tile_start[t] .. +255 of
+ * read tile_read[t]; every position of every read in exactly one tile). */
+typedef struct gmfa_kmer_reads {
+    const uint32_t *seq;
+    const uint32_t *mask;     /* NULL: no N */
+    int64_t n_words;
+    const int64_t *read_off;
+    const int32_t *read_len;
+    const int64_t *pos_off;
+    const int32_t *tile_read;
+    const int32_t *tile_start;
+    int32_t n_reads;
+    int32_t n_tiles;
+    int32_t window;           /* minimizer window, 1..128 */
+    int32_t reserved;
+} gmfa_kmer_reads;
+
+/* (start, count) of each query k-mer in the table, (0, 0) if absent. */
+int gmfa_kmer_lookup(const gmfa_kmer_index *index, const int64_t *queries, int64_t n,
+                     int64_t *out_start, int32_t *out_count, void *stream);
+
+/* Minimizer selection + lookup of every read k-mer: per position (start,
+ * count) (count 0 unless selected, valid and present), pos_aoff = exclusive
+ * scan of count within its read, read_total[r], and read_anchor_off[0..n_reads]
+ * = exclusive scan of read_total (read_anchor_off[n_reads] = all anchors). */
+int gmfa_kmer_seed(const gmfa_kmer_index *index, const gmfa_kmer_reads *reads,
+                   int64_t *pos_start, int32_t *pos_count, int64_t *pos_aoff,
+                   int64_t *read_total, int64_t *read_anchor_off, void *stream);
+
+/* Anchors of every read at read_anchor_off[r]: read_pos (k-mer position in
+ * the read) and ref_pos (reference offset), in position order and table order
+ * within a position, as GPUKmerIndex.query returns them. */
+int gmfa_kmer_emit(const gmfa_kmer_index *index, const gmfa_kmer_reads *reads,
+                   const int64_t *pos_start, const int32_t *pos_count, const int64_t *pos_aoff,
+                   const int64_t *read_anchor_off, int64_t *read_pos, int64_t *ref_pos,
+                   void *stream);
 
 /* ---------------------------------------------------------------------------
  * Continuous serving.  One class = requests sharing one band width and

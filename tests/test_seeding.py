@@ -98,3 +98,33 @@ def test_kmer_query_brute_force():
             exp = [(int(p), q) for p in np.flatnonzero(qs) for q in d.get(int(qp[p]), [])]
             assert list(zip(rp.tolist(), fp.tolist())) == exp
             assert (ln == k).all()
+
+
+def test_product_host_functions_match_oracle():
+    """The library's host-side pack / hash / minimizer / table build (used to
+    build the device index) equal the oracle on random inputs."""
+    from graphmambaformer_amd.alignment import seeding as P
+
+    rng = np.random.default_rng(4)
+    v = rng.integers(0, 1 << 62, size=500, dtype=np.int64)
+    assert np.array_equal(P.hash64(v), S.hash64(v))
+    for _ in range(60):
+        L = int(rng.integers(1, 300))
+        c = rng.integers(-1, 7, size=L).astype(np.int8)
+        k = int(rng.integers(1, 25))
+        a, b = P.pack_kmers(c, k), S.pack_kmers(c, k)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        h = rng.integers(0, 6, size=L).astype(np.uint64)
+        w = int(rng.integers(1, 40))
+        assert np.array_equal(P.minimizer_mask(h, w), S.minimizer_mask(h, w))
+    ref = rng.integers(1, 5, size=30000).astype(np.int8)
+    ref[::997] = 5
+    ref[5000:5600] = np.tile(ref[:20], 30)
+    for k, w, occ in ((11, 5, 0), (15, 10, 3), (7, 1, 0)):
+        pk, vd = P.pack_kmers(ref, k)
+        t1 = P.KmerTable.build(pk, vd & P.minimizer_mask(P.hash64(pk), w), k, max_occ=occ)
+        t2 = S.build_index(ref, k, w, occ)
+        assert np.array_equal(t1.keys, t2.keys) and np.array_equal(t1.offsets, t2.offsets)
+        assert np.array_equal(t1.positions, t2.positions)
+        q = np.concatenate([t2.keys[::7], rng.integers(-5, 1 << (2 * k), size=200)])
+        assert all(np.array_equal(x, y) for x, y in zip(t1.lookup(q), t2.lookup(q)))
