@@ -214,15 +214,13 @@ class GPUKmerIndex:
         """(read_pos, ref_pos, length) int64 of one read (seeding.py:777-798)."""
         return self.query_batch([read_codes])[0]
 
-    def query_batch_device(self, reads):
-        """Seed many reads at once.  Returns device tensors (read_pos, ref_pos)
-        of all anchors, read by read, and the host offsets [n_reads + 1]."""
-        import ctypes
-
+    def prepare(self, reads):
+        """Pack and upload a batch of reads (codes 1..5) with its tiling; the
+        returned batch can be seeded repeatedly by ``seed``."""
         import torch
 
         from .. import kernels as K
-        from .._lib import KmerReads, check, lib, ptr, stream_of
+        from .._lib import KmerReads, ptr
 
         dev = self.device
         k = self.k
@@ -236,39 +234,60 @@ class GPUKmerIndex:
         tile_read = np.repeat(np.arange(n, dtype=np.int32), ntile)
         tile_start = (np.arange(int(ntile.sum()), dtype=np.int64) -
                       np.repeat(np.cumsum(ntile) - ntile, ntile)).astype(np.int32) * 256
-        w, msk, roff, has_n = K.pack_host(reads, alphabet="ref") if n else (
-            np.zeros(1, np.uint32), np.zeros(1, np.uint32), np.zeros(0, np.int64), False)
+        if n:
+            w, msk, roff, has_n = K.pack_host(reads, alphabet="ref")
+        else:
+            w, msk, roff, has_n = (np.zeros(1, np.uint32), np.zeros(1, np.uint32),
+                                   np.zeros(0, np.int64), False)
 
         def up(a):
             return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
-        seq = up(w.view(np.int32))
-        mask = up(msk.view(np.int32)) if has_n else None
+        bufs = {"seq": up(w.view(np.int32)), "mask": up(msk.view(np.int32)) if has_n else None,
+                "roff": up(roff), "len": up(lens.astype(np.int32)), "poff": up(pos_off),
+                "tr": up(tile_read), "ts": up(tile_start)}
         rd = KmerReads()
-        rd.seq, rd.mask, rd.n_words = ptr(seq), ptr(mask), int(seq.numel())
-        d_roff, d_len, d_poff = up(roff), up(lens.astype(np.int32)), up(pos_off)
-        d_tr, d_ts = up(tile_read), up(tile_start)
-        rd.read_off, rd.read_len, rd.pos_off = ptr(d_roff), ptr(d_len), ptr(d_poff)
-        rd.tile_read, rd.tile_start = ptr(d_tr), ptr(d_ts)
+        rd.seq, rd.mask, rd.n_words = ptr(bufs["seq"]), ptr(bufs["mask"]), int(bufs["seq"].numel())
+        rd.read_off, rd.read_len, rd.pos_off = ptr(bufs["roff"]), ptr(bufs["len"]), ptr(bufs["poff"])
+        rd.tile_read, rd.tile_start = ptr(bufs["tr"]), ptr(bufs["ts"])
         rd.n_reads, rd.n_tiles, rd.window = n, int(len(tile_read)), self.window
         npos = int(pos_off[-1])
-        pstart = torch.empty(max(npos, 1), dtype=torch.int64, device=dev)
-        pcount = torch.empty(max(npos, 1), dtype=torch.int32, device=dev)
-        paoff = torch.empty(max(npos, 1), dtype=torch.int64, device=dev)
-        rtot = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
-        raoff = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        bufs["pstart"] = torch.empty(max(npos, 1), dtype=torch.int64, device=dev)
+        bufs["pcount"] = torch.empty(max(npos, 1), dtype=torch.int32, device=dev)
+        bufs["paoff"] = torch.empty(max(npos, 1), dtype=torch.int64, device=dev)
+        bufs["rtot"] = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+        bufs["raoff"] = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        return rd, bufs
+
+    def seed(self, prepared):
+        """Seed a prepared batch: device (read_pos, ref_pos) of all anchors,
+        read by read, and the host offsets [n_reads + 1] (one readback of the
+        anchor counts sizes the output)."""
+        import ctypes
+
+        import torch
+
+        from .._lib import check, lib, ptr, stream_of
+
+        rd, b = prepared
+        dev = self.device
         ix = self._index()
         s = stream_of(dev)
-        check(lib().gmfa_kmer_seed(ctypes.byref(ix), ctypes.byref(rd), ptr(pstart), ptr(pcount),
-                                   ptr(paoff), ptr(rtot), ptr(raoff), s), "gmfa_kmer_seed")
-        offs = raoff.cpu().numpy()
+        check(lib().gmfa_kmer_seed(ctypes.byref(ix), ctypes.byref(rd), ptr(b["pstart"]),
+                                   ptr(b["pcount"]), ptr(b["paoff"]), ptr(b["rtot"]),
+                                   ptr(b["raoff"]), s), "gmfa_kmer_seed")
+        offs = b["raoff"].cpu().numpy()
         total = int(offs[-1])
         read_pos = torch.empty(max(total, 1), dtype=torch.int64, device=dev)
         ref_pos = torch.empty(max(total, 1), dtype=torch.int64, device=dev)
-        check(lib().gmfa_kmer_emit(ctypes.byref(ix), ctypes.byref(rd), ptr(pstart), ptr(pcount),
-                                   ptr(paoff), ptr(raoff), ptr(read_pos), ptr(ref_pos), s),
-              "gmfa_kmer_emit")
+        check(lib().gmfa_kmer_emit(ctypes.byref(ix), ctypes.byref(rd), ptr(b["pstart"]),
+                                   ptr(b["pcount"]), ptr(b["paoff"]), ptr(b["raoff"]),
+                                   ptr(read_pos), ptr(ref_pos), s), "gmfa_kmer_emit")
         return read_pos[:total], ref_pos[:total], offs
+
+    def query_batch_device(self, reads):
+        """Seed many reads at once (pack, upload, seed)."""
+        return self.seed(self.prepare(reads))
 
     def query_batch(self, reads):
         """Per read (read_pos, ref_pos, length) int64 host arrays."""

@@ -309,3 +309,16 @@ def poisson_arrivals(n: int, rate: float, seed: int = 6) -> np.ndarray:
     """Arrival times (s) of a Poisson stream of n requests at `rate` per second."""
     rng = np.random.default_rng(seed)
     return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def seeding_reads(seed: int = 7, ref_len: int = 46_000_000, n_reads: int = 4096,
+                  read_len: int = 10_000):
+    """A random reference (codes 1..4) and reads sampled from it with 1% errors
+    (half substitutions, half 1-3 bp indels); for k-mer seeding."""
+    rng = np.random.default_rng(seed)
+    ref = rng.integers(0, 4, size=ref_len, dtype=np.uint8)
+    reads = []
+    for _ in range(n_reads):
+        p = int(rng.integers(0, ref_len - read_len))
+        reads.append(mutate(rng, ref[p:p + read_len], 0.01, 0.5, 3).astype(np.int8) + 1)
+    return (ref + 1).astype(np.int8), reads
