@@ -267,21 +267,59 @@ class AffineChainer:
         return segs
 
     def chain_batch(self, anchor_sets: Sequence[AnchorSet], contexts=None, device=None) -> list:
-        """Chains per read, every (read, strand) DP in one kernel launch."""
+        """Chains per read (chaining.py:644-745): the strand split and weights on
+        the host, then sort into DP order, DP, backtrack and select for every
+        (read, strand) segment of the batch on the device, one readback."""
         contexts = list(contexts or [None] * len(anchor_sets))
-        segs = self._segments(anchor_sets, contexts)
-        results: list = [[] for _ in anchor_sets]
-        if segs:
-            seg_off = np.zeros(len(segs) + 1, np.int64)
-            np.cumsum([len(s[2]) for s in segs], out=seg_off[1:])
-            re_ = np.concatenate([s[2].read_end for s in segs]).astype(np.int64)
-            fe = np.concatenate([s[2].ref_end for s in segs]).astype(np.int64)
-            w = np.concatenate([s[4] for s in segs])
-            f, par = _run_segments(re_, fe, w, seg_off, self.cfg)
-            for k, (r, strand, ordered, gidx, _w) in enumerate(segs):
-                a, z = seg_off[k], seg_off[k + 1]
-                results[r].extend(self._backtrack(ordered, f[a:z], par[a:z], strand, gidx))
-        return [self.select(c) for c in results]
+        nr = len(anchor_sets)
+        cols = {k: [] for k in ("re", "fe", "rp", "fp", "w", "id")}
+        seg_off, seg_strand = [0], []
+        read_seg = np.full((nr, 2), -1, np.int32)
+        read_base = np.zeros(nr, np.int64)
+        for r, (anchors, ctx) in enumerate(zip(anchor_sets, contexts)):
+            read_base[r] = seg_off[-1]
+            if len(anchors) == 0:
+                continue
+            self._check_context(ctx)
+            for h, strand in enumerate((1, -1)):
+                subset, indices = anchors.for_strand(strand)
+                if len(subset) == 0 or len(subset) < self.cfg.min_chain_anchors:
+                    continue
+                read_seg[r, h] = len(seg_strand)
+                seg_strand.append(strand)
+                cols["re"].append(subset.read_end)
+                cols["fe"].append(subset.ref_end)
+                cols["rp"].append(subset.read_pos)
+                cols["fp"].append(subset.ref_pos)
+                cols["w"].append(self._weights(subset, ctx))
+                cols["id"].append(indices)
+                seg_off.append(seg_off[-1] + len(subset))
+        if not seg_strand:
+            return [[] for _ in anchor_sets]
+        cat = {k: np.concatenate(v) for k, v in cols.items()}
+        dev = _device()
+        res = K.chain_pipeline(_as_int64(cat["re"], "read_end"), _as_int64(cat["fe"], "ref_end"),
+                               _as_int64(cat["rp"], "read_pos"), _as_int64(cat["fp"], "ref_pos"),
+                               cat["w"].astype(np.float64), cat["id"].astype(np.int64),
+                               np.array(seg_off, np.int64), np.array(seg_strand, np.int32),
+                               read_seg, read_base, _lookback(self.cfg), int(self.cfg.max_gap),
+                               penalty_table(self.cfg, dev), self.cfg.min_chain_anchors,
+                               self.cfg.min_chain_score, self.cfg.secondary_score_ratio,
+                               self.cfg.secondary_overlap, self.cfg.max_chains, dev)
+        out = []
+        for r in range(nr):
+            chains = []
+            for x in range(int(res.n_kept[r])):
+                c = int(res.kept[r, x])
+                o, m = int(res.mem[c, 0]), int(res.mem[c, 1])
+                sp = res.span[c]
+                ch = Chain(anchor_idx=res.members[o:o + m].copy(), score=float(res.score[c]),
+                           strand=int(res.strand[c]), read_start=int(sp[0]), read_end=int(sp[1]),
+                           ref_start=int(sp[2]), ref_end=int(sp[3]))
+                ch.is_primary = x == 0
+                chains.append(ch)
+            out.append(chains)
+        return out
 
     def chain(self, anchors: AnchorSet, ctx=None, device=None) -> list:
         """chaining.py:620-642."""

@@ -477,3 +477,76 @@ def chain_backtrack(f: np.ndarray, parent: np.ndarray, min_anchors: int, min_sco
                                           starts.ctypes.data, scores.ctypes.data),
                "gmfa_chain_backtrack")
     return [(members[starts[c]:starts[c + 1]], float(scores[c])) for c in range(nc)]
+
+
+@dataclass
+class ChainBatchResult:
+    """Chains of a batch of reads from the device pipeline (host arrays)."""
+
+    n_kept: np.ndarray     # [n_reads]
+    kept: np.ndarray       # [n_reads, max_chains] chain record indices
+    score: np.ndarray      # per chain record
+    span: np.ndarray       # [records, 4] read_start, read_end, ref_start, ref_end
+    mem: np.ndarray        # [records, 2] member offset, count
+    members: np.ndarray    # anchor ids (read-local), read order per chain
+    strand: np.ndarray     # per chain record: +1 / -1
+
+
+def chain_pipeline(read_end: np.ndarray, ref_end: np.ndarray, read_pos: np.ndarray,
+                   ref_pos: np.ndarray, weight: np.ndarray, anchor_id: np.ndarray,
+                   seg_off: np.ndarray, seg_strand: np.ndarray, read_seg: np.ndarray,
+                   read_base: np.ndarray, max_lookback: int, max_gap: int, pen_lut: torch.Tensor,
+                   min_anchors: int, min_score: float, score_ratio: float, max_overlap: float,
+                   max_chains: int, device) -> ChainBatchResult:
+    """Sort into DP order, DP, backtrack and select for a batch of segments,
+    all on the device; one readback of the results.  Arrays hold the
+    segments' anchors back to back in their original order."""
+    dev = torch.device(device)
+    n = int(len(read_end))
+    ns = int(len(seg_off) - 1)
+    nr = int(len(read_seg))
+
+    def up(a, dt=None):
+        return torch.from_numpy(np.ascontiguousarray(a if dt is None else a.astype(dt))).to(dev)
+
+    d_so = up(seg_off, np.int64)
+    src = [up(read_end, np.int64), up(ref_end, np.int64), up(weight, np.float64),
+           up(read_pos, np.int64), up(ref_pos, np.int64), up(anchor_id, np.int64)]
+    o_re, o_fe = torch.empty_like(src[0]), torch.empty_like(src[1])
+    o_w, o_rp = torch.empty_like(src[2]), torch.empty_like(src[3])
+    o_fp, o_id = torch.empty_like(src[4]), torch.empty_like(src[5])
+    scratch = torch.empty(max(2 * n, 1), dtype=torch.int32, device=dev)
+    s = stream_of(dev)
+    check(lib().gmfa_chain_sort(ptr(d_so), ns, n, *[ptr(t) for t in src], ptr(scratch),
+                                ptr(o_re), ptr(o_fe), ptr(o_w), ptr(o_rp), ptr(o_fp), ptr(o_id),
+                                s), "gmfa_chain_sort")
+    f, parent = chain_dp(o_re, o_fe, o_w, d_so, max_lookback, max_gap, pen_lut=pen_lut)
+    claimed = torch.empty(max(n, 1), dtype=torch.uint8, device=dev)
+    n_chains = torch.zeros(max(ns, 1), dtype=torch.int32, device=dev)
+    c_score = torch.zeros(max(n, 1), dtype=torch.float64, device=dev)
+    c_span = torch.zeros((max(n, 1), 4), dtype=torch.int64, device=dev)
+    c_mem = torch.zeros((max(n, 1), 2), dtype=torch.int64, device=dev)
+    members = torch.zeros(max(n, 1), dtype=torch.int64, device=dev)
+    bad = torch.zeros(1, dtype=torch.int32, device=dev)
+    check(lib().gmfa_chain_backtrack_device(ptr(d_so), ns, n, ptr(f), ptr(parent), ptr(o_rp),
+                                            ptr(o_re), ptr(o_fp), ptr(o_fe), ptr(o_id),
+                                            int(min_anchors), float(min_score), ptr(scratch),
+                                            ptr(claimed), ptr(n_chains), ptr(c_score),
+                                            ptr(c_span), ptr(c_mem), ptr(members), ptr(bad), s),
+          "gmfa_chain_backtrack_device")
+    d_rs = up(read_seg, np.int32)
+    d_rb = up(read_base, np.int64)
+    sel_scratch = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    n_kept = torch.zeros(max(nr, 1), dtype=torch.int32, device=dev)
+    kept = torch.zeros((max(nr, 1), int(max_chains)), dtype=torch.int64, device=dev)
+    check(lib().gmfa_chain_select(nr, ptr(d_rs), ptr(d_so), ptr(n_chains), ptr(c_score),
+                                  ptr(c_span), float(score_ratio), float(max_overlap),
+                                  int(max_chains), ptr(sel_scratch), ptr(d_rb), ptr(n_kept),
+                                  ptr(kept), s), "gmfa_chain_select")
+    if int(bad.item()) != 0:
+        raise RuntimeError("chaining DP produced a parent outside its segment")
+    seg_of_rec = np.repeat(np.arange(ns), np.diff(np.asarray(seg_off, np.int64)))
+    return ChainBatchResult(n_kept=n_kept.cpu().numpy()[:nr], kept=kept.cpu().numpy()[:nr],
+                            score=c_score.cpu().numpy()[:n], span=c_span.cpu().numpy()[:n],
+                            mem=c_mem.cpu().numpy()[:n], members=members.cpu().numpy()[:n],
+                            strand=np.asarray(seg_strand)[seg_of_rec] if n else np.zeros(0))

@@ -26,6 +26,9 @@
  *   gmfa_chain_dp_i32  the same DP on the integer profile
  *   gmfa_chain_backtrack graphmambaformer/alignment/chaining.py:530-576
  *                      AffineChainer._backtrack (host code)
+ *   gmfa_chain_sort    graphmambaformer/alignment/types.py:149-152 (DP order)
+ *   gmfa_chain_backtrack_device / gmfa_chain_select  chaining.py:530-617
+ *                      (_backtrack and select on the device)
  *   gmfa_kmer_lookup   graphmambaformer/alignment/seeding.py:751-775
  *                      GPUKmerIndex.lookup (accel/cuda_kernels.py:44-74)
  *   gmfa_kmer_seed / gmfa_kmer_emit  seeding.py:777-798 GPUKmerIndex.query
@@ -204,6 +207,49 @@ int gmfa_chain_dp_i32(const int64_t *read_end, const int64_t *ref_end, const int
 int64_t gmfa_chain_backtrack(int64_t n, const double *f, const int64_t *parent,
                              int64_t min_anchors, double min_score, int64_t *members,
                              int64_t *chain_start, double *chain_score);
+
+/* ---------------------------------------------------------------------------
+ * Chaining around the DP, on the device, for a batch of segments (a segment =
+ * one (read, strand) anchor set; segment s = elements seg_off[s] ..
+ * seg_off[s+1]-1, all arrays device memory).
+ * ------------------------------------------------------------------------- */
+
+/* AnchorSet.sorted_by_ref (graphmambaformer/alignment/types.py:149-152):
+ * stable sort of each segment by (ref_end, read_end); the six arrays are
+ * gathered into DP order.  scratch: 2n int32. */
+int gmfa_chain_sort(const int64_t *seg_off, int32_t n_segments, int64_t n,
+                    const int64_t *read_end, const int64_t *ref_end, const double *weight,
+                    const int64_t *read_pos, const int64_t *ref_pos, const int64_t *anchor_id,
+                    int32_t *scratch, int64_t *o_read_end, int64_t *o_ref_end, double *o_weight,
+                    int64_t *o_read_pos, int64_t *o_ref_pos, int64_t *o_anchor_id, void *stream);
+
+/* AffineChainer._backtrack (graphmambaformer/alignment/chaining.py:530-576)
+ * per segment on the device (f / parent of gmfa_chain_dp_f64, DP order,
+ * segment-local parents).  Chain records of segment s are stored from index
+ * seg_off[s] on: n_chains[s] of them, chain_score, chain_span (read_start,
+ * read_end, ref_start, ref_end), chain_members (absolute offset into members,
+ * count); members hold anchor_id values in read order.  scratch: 2n int32,
+ * claimed: n bytes; *bad_parent (zeroed by the caller) is set if a parent
+ * leaves its segment. */
+int gmfa_chain_backtrack_device(const int64_t *seg_off, int32_t n_segments, int64_t n,
+                                const double *f, const int64_t *parent, const int64_t *read_pos,
+                                const int64_t *read_end, const int64_t *ref_pos,
+                                const int64_t *ref_end, const int64_t *anchor_id,
+                                int64_t min_anchors, double min_score, int32_t *scratch,
+                                uint8_t *claimed, int32_t *n_chains, double *chain_score,
+                                int64_t *chain_span, int64_t *chain_members, int64_t *members,
+                                int32_t *bad_parent, void *stream);
+
+/* AffineChainer.select (chaining.py:579-617) per read: candidates are the
+ * chains of segment read_seg[2r] (strand +1) then read_seg[2r+1] (strand -1;
+ * -1 = no segment); kept[r * max_chains + x] = chain record indices, the
+ * primary first, n_kept[r] of them.  scratch: at read_base[r] room for the
+ * read's candidates (its anchor count suffices). */
+int gmfa_chain_select(int32_t n_reads, const int32_t *read_seg, const int64_t *seg_off,
+                      const int32_t *n_chains, const double *chain_score,
+                      const int64_t *chain_span, double score_ratio, double max_overlap,
+                      int32_t max_chains, int64_t *scratch, const int64_t *read_base,
+                      int32_t *n_kept, int64_t *kept, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Stage-1 k-mer seeding (graphmambaformer/alignment/seeding.py:720-798

@@ -246,3 +246,60 @@ def test_extension_engine_batch_vs_oracle(x_drop, with_n):
     # single-read calls agree with the batch
     one = eng.extend_chains(reads[3], chains[3], anchors[3], ref_s)
     assert [(x.score, x.cigar) for x in one] == [(x.score, x.cigar) for x in got[3]]
+
+
+def _anchor_read(rng, n_anchors, read_len=8000, with_ties=False):
+    """Anchors of one read: collinear runs on both strands, decoys, and (with
+    ties) duplicated anchors and equal spacings so equal f values occur."""
+    rp, fp, ln, st = [], [], [], []
+    for strand in (1, -1):
+        for _ in range(int(rng.integers(1, 4))):
+            start = int(rng.integers(0, 40_000_000))
+            k = int(rng.integers(2, max(3, n_anchors // 3)))
+            pos = np.sort(rng.integers(0, read_len - 20, size=k))
+            jitter = rng.integers(-2, 3, size=k) if not with_ties else np.zeros(k, np.int64)
+            rp += pos.tolist()
+            fp += (start + pos + jitter).tolist()
+            ln += [15] * k
+            st += [strand] * k
+    nd = max(1, n_anchors // 10)
+    rp += rng.integers(0, read_len - 20, size=nd).tolist()
+    fp += rng.integers(0, 40_000_000, size=nd).tolist()
+    ln += rng.integers(11, 20, size=nd).tolist()
+    st += rng.choice([1, -1], size=nd).tolist()
+    if with_ties and len(rp) > 4:
+        for x in rng.integers(0, len(rp), size=4):
+            rp.append(rp[x]), fp.append(fp[x]), ln.append(ln[x]), st.append(st[x])
+    return (np.array(rp), np.array(fp), np.array(ln), np.array(st))
+
+
+@pytest.mark.parametrize("min_score,lookback,ratio,max_chains",
+                         [(20.0, 64, 0.6, 8), (0.0, 17, 0.0, 64), (200.0, 50, 0.6, 8),
+                          (10.0, 33, 0.1, 3)])
+def test_affine_chainer_device_pipeline_vs_oracle(min_score, lookback, ratio, max_chains):
+    """Sort, DP, backtrack and select all on the device for a batch of reads
+    (ragged, both strands, ties, one read with 20,000 anchors) equal the
+    oracle's per-read chaining."""
+    from graphmambaformer_amd.alignment import AffineChainer, AnchorSet
+    from graphmambaformer_amd.config import ChainingConfig
+
+    _say("case: device chaining pipeline", min_score, lookback)
+    rng = np.random.default_rng(int(min_score) + lookback)
+    reads = [_anchor_read(rng, int(rng.integers(1, 400)), with_ties=(r % 3 == 0))
+             for r in range(120)]
+    reads.append(_anchor_read(rng, 20_000))
+    reads.append((np.zeros(0, np.int64),) * 4)
+    cfg = ChainingConfig(max_lookback=lookback, min_chain_score=min_score,
+                         secondary_score_ratio=ratio, max_chains=max_chains)
+    sets = [AnchorSet.from_lists(rp, fp, ln, st, read_len=8000) for rp, fp, ln, st in reads]
+    got = AffineChainer(cfg).chain_batch(sets)
+    for r, (rp, fp, ln, st) in enumerate(reads):
+        exp = O.chain_read(rp, fp, ln, st, max_lookback=lookback, min_chain_score=min_score,
+                           secondary_score_ratio=ratio, max_chains=max_chains)
+        assert len(got[r]) == len(exp), r
+        for g, e in zip(got[r], exp):
+            assert g.anchor_idx.tolist() == e["anchor_idx"].tolist(), r
+            assert g.score == e["score"], r
+            assert (g.strand, g.read_start, g.read_end, g.ref_start, g.ref_end, g.is_primary) == (
+                e["strand"], e["read_start"], e["read_end"], e["ref_start"], e["ref_end"],
+                e["is_primary"]), r
