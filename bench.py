@@ -561,6 +561,41 @@ def main():
         O.chain_dp_i32(wc.read_end[a:b], wc.ref_end[a:b], wc.weight[a:b], wc.max_lookback,
                        wc.max_gap, wc.gap_open, wc.gap_extend)
     ccpu = time.perf_counter() - t1
+    # anchors -> chains on the device: sort into DP order, DP (float64 reference
+    # defaults), backtrack, select; host arrays in, chain records out
+    from graphmambaformer_amd.alignment.chaining import penalty_table
+    from graphmambaformer_amd.config import ChainingConfig
+
+    ccfg = ChainingConfig(max_lookback=wc.max_lookback, max_gap=wc.max_gap)
+    nrd = wc.n_reads
+    seg_of = np.repeat(np.arange(nrd), np.diff(wc.seg_off))
+    shuf = np.lexsort((np.random.default_rng(0).random(len(seg_of)), seg_of))  # unsorted input
+    rp_ = wc.read_end[shuf] - 15
+    fp_ = wc.ref_end[shuf] - 15
+    ids = (np.arange(len(seg_of)) - wc.seg_off[seg_of])[shuf]
+    read_seg = np.stack([np.arange(nrd), np.full(nrd, -1)], axis=1).astype(np.int32)
+    lut64 = penalty_table(ccfg, dev)
+
+    def pipeline():
+        return K.chain_pipeline(rp_ + 15, fp_ + 15, rp_, fp_, np.full(len(rp_), 15.0), ids,
+                                wc.seg_off, np.ones(nrd, np.int32), read_seg, wc.seg_off[:-1],
+                                ccfg.max_lookback, ccfg.max_gap, lut64, ccfg.min_chain_anchors,
+                                ccfg.min_chain_score, ccfg.secondary_score_ratio,
+                                ccfg.secondary_overlap, ccfg.max_chains, dev)
+
+    pipeline()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(3):
+        pres = pipeline()
+    pms = (time.perf_counter() - t1) / 3 * 1e3
+    sub = np.arange(0, nrd, 16)
+    t1 = time.perf_counter()
+    for r in sub:
+        m = seg_of[shuf] == r
+        O.chain_read(rp_[m], fp_[m], np.full(int(m.sum()), 15), np.ones(int(m.sum()), np.int64),
+                     max_lookback=ccfg.max_lookback, max_gap=ccfg.max_gap)
+    cpu_pipe = (time.perf_counter() - t1) / len(sub)
     result["chains"] = {
         "reads_per_s": round(wc.n_reads / (cms / 1e3), 1),
         "pair_evals_per_s": round(wc.pair_evaluations() / (cms / 1e3), 1),
@@ -569,9 +604,15 @@ def main():
         "f64_reads_per_s": round(wc.n_reads / (cms64 / 1e3), 1),
         "cpu_reads_per_s": round(wc.n_reads / ccpu, 1),
         "cpu_threads": 1,
+        "anchors_to_chains_ms": round(pms, 3),
+        "anchors_to_chains_reads_per_s": round(nrd / (pms / 1e3), 1),
+        "anchors_to_chains_kept": int(pres.n_kept.sum()),
+        "cpu_anchors_to_chains_reads_per_s": round(1.0 / cpu_pipe, 1),
         "workload": "4096 reads, Poisson(400) anchors, lookback 50, max_gap 5000, "
                     "int profile 6/1 (and float64 6/0.05/0.5), seed 1; CPU: whole batch, "
-                    "int profile, one thread",
+                    "int profile, one thread; anchors_to_chains: unsorted anchors in, "
+                    "sort + float64 DP + backtrack + select on the device, host arrays in and "
+                    "chain records out (CPU: the oracle's chain_read on every 16th read)",
     }
     log(f"[bench] chains: {result['chains']}")
 

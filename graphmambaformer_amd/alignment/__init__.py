@@ -4,6 +4,7 @@ from .chaining import AffineChainer, chain_dp_batched, chain_dp_numpy  # noqa: F
 from .extension import (  # noqa: F401
     BandedDPResult,
     ExtensionEngine,
+    WavefrontAligner,
     banded_affine_sw_batch,
     traceback_banded,
 )

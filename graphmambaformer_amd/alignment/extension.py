@@ -13,9 +13,13 @@ sarakh1999/GraphMambaFormer:
       the device (one launch) and caches the CIGARs; ``query`` / ``target`` are
       taken to be the sequences the forward pass saw, as at every reference
       call site.
+  WavefrontAligner(cfg).align(query, target)
+      extension.py:308-463: global unit-cost alignment, distance + CIGAR,
+      RuntimeError past wfa_max_distance; align_batch for many pairs.
   ExtensionEngine(cfg, device, backend="hip").extend_chains(...)
-      extension.py:469-641: per-chain windows on the chain diagonal, one batch
-      per read with a shared half-band, soft clips, coordinate lift.
+      extension.py:469-730: per-chain windows on the chain diagonal, one batch
+      per read with a shared half-band, soft clips, coordinate lift; the WFA
+      branch with its per-chain banded fallback; extend_batch for many reads.
 
 Scores must be integer valued with gap_open >= gap_extend (the int32 DP is
 exact there, and the reference's closed-form E assumes it).  Base codes are the
@@ -171,6 +175,35 @@ def traceback_banded(result: BandedDPResult, index: int, query: np.ndarray, targ
                            n_mismatch=int(c[1]), n_insertion=int(c[2]), n_deletion=int(c[3]))
 
 
+class WavefrontAligner:
+    """Global unit-cost alignment (extension.py:308-463) on the device.
+
+    ``align(query, target)`` returns ``(edit_distance, cigar)`` and raises
+    RuntimeError past ``cfg.wfa_max_distance`` (the caller's cue to fall back
+    to banded DP), as the reference does; ``align_batch`` aligns many pairs in
+    two launches and returns None for the pairs over the budget."""
+
+    def __init__(self, cfg: ExtensionConfig | None = None, device=None):
+        self.cfg = cfg or ExtensionConfig()
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+
+    def align_batch(self, queries: Sequence[np.ndarray], targets: Sequence[np.ndarray]) -> list:
+        if len(queries) != len(targets):
+            raise ValueError("one target per query")
+        if not queries:
+            return []
+        inp = K.upload_pairs([np.asarray(q) for q in queries], [np.asarray(t) for t in targets],
+                             np.zeros(len(queries), np.int32), self.device, alphabet="ref")
+        dist, cigs, _ = K.wfa_align(inp, int(self.cfg.wfa_max_distance))
+        return [None if d < 0 else (int(d), list(c)) for d, c in zip(dist, cigs)]
+
+    def align(self, query: np.ndarray, target: np.ndarray):
+        res = self.align_batch([query], [target])[0]
+        if res is None:
+            raise RuntimeError(f"WFA exceeded wfa_max_distance={self.cfg.wfa_max_distance}")
+        return res
+
+
 class ExtensionEngine:
     """Stage 3 entry point (extension.py:469-641) on the HIP kernels."""
 
@@ -206,6 +239,79 @@ class ExtensionEngine:
                                torch.from_numpy(m.view(np.int32)).to(dev), has_n, len(codes))
         return self._ref_cache[1:]
 
+    def _extend_wfa_batch(self, reads, chains, anchors, ref_seq):
+        """The WFA branch (extension.py:643-730) for every chain of every read:
+        global alignment of the read against the window at the chain's
+        read-zero diagonal, length read_len + the chain's net indel; chains over
+        wfa_max_distance are extended with banded SW alone, as the reference
+        falls back one chain at a time."""
+        import dataclasses
+
+        cfg = self.cfg
+        ref_words, ref_masks, ref_has_n, ref_len = self._reference(ref_seq)
+        dev = self.device
+        seqs, lens = [], []
+        for r in range(len(reads)):
+            f = encode_bases(reads[r])
+            lens.append(len(f))
+            seqs.append(f)
+            seqs.append(reverse_complement_codes(f))
+        qw, qm, qoff, q_has_n = K.pack_host(seqs, alphabet="ref") if seqs else (
+            np.zeros(1, np.uint32), np.zeros(1, np.uint32), np.zeros(0, np.int64), False)
+        has_n = bool(q_has_n or ref_has_n)
+        owner, q_off, t_off, ql, tl = [], [], [], [], []
+        for r in range(len(reads)):
+            L = lens[r]
+            for k, c in enumerate(chains[r]):
+                start = int(np.clip(c.ref_start - c.read_start, 0, max(ref_len - 1, 0)))
+                net = (c.ref_end - c.ref_start) - (c.read_end - c.read_start)
+                end = min(ref_len, start + max(1, L + net))
+                owner.append((r, k, start))
+                q_off.append(int(qoff[2 * r + (0 if c.strand > 0 else 1)]))
+                t_off.append(start)
+                ql.append(L)
+                tl.append(max(0, end - start))
+        out: list = [[None] * len(chains[r]) for r in range(len(reads))]
+        if not owner:
+            return out
+
+        def up(a):
+            return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+        ql_h, tl_h = np.array(ql, np.int32), np.array(tl, np.int32)
+        inp = K.SwInputs(qseq=up(qw.view(np.int32)), tseq=ref_words,
+                         qmask=up(qm.view(np.int32)) if has_n else None,
+                         tmask=ref_masks if has_n else None, q_off=up(np.array(q_off, np.int64)),
+                         t_off=up(np.array(t_off, np.int64)), qlen=up(ql_h), tlen=up(tl_h),
+                         band_offset=up(np.zeros(len(ql), np.int32)), qlen_host=ql_h,
+                         tlen_host=tl_h, boff_host=np.zeros(len(ql), np.int32), has_n=has_n)
+        dist, cigs, cnt = K.wfa_align(inp, int(cfg.wfa_max_distance))
+        fallback = []
+        for p, (r, k, start) in enumerate(owner):
+            if dist[p] < 0:
+                fallback.append((r, k))
+                continue
+            cig = cigs[p]
+            gap = sum(cfg.gap_open + max(n - 1, 0) * cfg.gap_extend for op, n in cig
+                      if op in ("I", "D"))
+            score = (int(cnt[p, 0]) * cfg.match_score - int(cnt[p, 1]) * cfg.mismatch_penalty
+                     - gap)
+            out[r][k] = ExtensionResult(score=float(score), cigar=list(cig), read_start=0,
+                                        read_end=int(ql[p]), ref_start=start,
+                                        ref_end=start + int(tl[p]), n_match=int(cnt[p, 0]),
+                                        n_mismatch=int(cnt[p, 1]), n_insertion=int(cnt[p, 2]),
+                                        n_deletion=int(cnt[p, 3]))
+        if fallback:
+            sw = ExtensionEngine(dataclasses.replace(cfg, algorithm="banded_sw"), device=dev,
+                                 backend=self.backend)
+            sw._ref_cache = self._ref_cache
+            res = sw.extend_batch([reads[r] for r, _ in fallback],
+                                  [[chains[r][k]] for r, k in fallback],
+                                  [anchors[r] for r, _ in fallback], ref_seq)
+            for (r, k), one in zip(fallback, res):
+                out[r][k] = one[0]
+        return out
+
     def extend_chains(self, read_seq: str, chains: Sequence[Chain], anchors: AnchorSet,
                       ref_seq: str) -> list:
         """One ExtensionResult per chain (extension.py:511-641)."""
@@ -225,10 +331,10 @@ class ExtensionEngine:
         resident reference, so no per-chain copy is made; soft clips and the
         coordinate lift (extension.py:628-640) are applied to the results."""
         cfg = self.cfg
-        if cfg.algorithm == "wfa":
-            raise NotImplementedError("the wavefront aligner is not part of this library")
         if not (len(reads) == len(chains) == len(anchors)):
             raise ValueError("reads, chains and anchors must have one entry per read")
+        if cfg.algorithm == "wfa":
+            return self._extend_wfa_batch(reads, chains, anchors, ref_seq)
         ma, mi, go, ge = integer_scoring(cfg)
         out: list = [[] for _ in reads]
         live = [r for r in range(len(reads)) if len(chains[r])]

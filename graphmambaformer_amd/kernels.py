@@ -550,3 +550,51 @@ def chain_pipeline(read_end: np.ndarray, ref_end: np.ndarray, read_pos: np.ndarr
                             score=c_score.cpu().numpy()[:n], span=c_span.cpu().numpy()[:n],
                             mem=c_mem.cpu().numpy()[:n], members=members.cpu().numpy()[:n],
                             strand=np.asarray(seg_strand)[seg_of_rec] if n else np.zeros(0))
+
+
+def wfa_align(inp: SwInputs, max_distance: int):
+    """Batched unit-cost wavefront alignment (extension.py:308-463) of every
+    pair of ``inp``: distance (-1 above max_distance) and, for the others,
+    CIGAR runs and op counts.  Two launches: the distances, then (history
+    sized per pair) the alignments."""
+    dev = inp.device
+    n = inp.n_pairs
+    ql = inp.qlen_host.astype(np.int64)
+    tl = inp.tlen_host.astype(np.int64)
+    dist = torch.full((max(n, 1),), -1, dtype=torch.int32, device=dev)
+    if n == 0:
+        return np.zeros(0, np.int32), [], np.zeros((0, 4), np.int32)
+    cap = int(min(int(max_distance), int(np.maximum(ql, tl).max())))
+    scratch = torch.empty(n * 2 * (2 * cap + 3), dtype=torch.int32, device=dev)
+    b = inp.batch_struct()
+    s = stream_of(dev)
+    check(lib().gmfa_wfa_distance(ctypes.byref(b), int(max_distance), cap, ptr(scratch),
+                                  ptr(dist), s), "gmfa_wfa_distance")
+    d_h = dist.cpu().numpy()[:n]
+    del scratch
+    both = (ql > 0) & (tl > 0) & (d_h >= 0)
+    hsz = np.where(both, (d_h.astype(np.int64) + 1) ** 2, 0)
+    hoff = np.zeros(n + 1, np.int64)
+    np.cumsum(hsz, out=hoff[1:])
+    hist = torch.empty(max(int(hoff[-1]), 1), dtype=torch.int32, device=dev)
+    cap_c = ql + tl + 1
+    coff = np.zeros(n + 1, np.int64)
+    np.cumsum(cap_c, out=coff[1:])
+    cig = torch.zeros(max(int(coff[-1]), 1), dtype=torch.int32, device=dev)
+    ncig = torch.zeros(n, dtype=torch.int32, device=dev)
+    counts = torch.zeros((n, 4), dtype=torch.int32, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    d_hoff = torch.from_numpy(hoff[:-1]).to(dev)
+    d_coff = torch.from_numpy(coff[:-1]).to(dev)
+    check(lib().gmfa_wfa_align(ctypes.byref(b), ptr(dist), ptr(d_hoff), ptr(hist), ptr(d_coff),
+                               ptr(cig), ptr(ncig), ptr(counts), ptr(status), s), "gmfa_wfa_align")
+    if int(status.item()) != 0:
+        raise RuntimeError("WFA traceback left the first wavefront")
+    tbk = SwTraceback(cigar=cig, cigar_off=d_coff, n_cigar=ncig, read_start=ncig, ref_start=ncig,
+                      counts=counts)
+    cigs = decode_cigars(tbk)
+    cnt = counts.cpu().numpy()
+    for p in range(n):
+        if d_h[p] < 0:
+            cigs[p] = None
+    return d_h, cigs, cnt

@@ -26,6 +26,8 @@
  *   gmfa_chain_dp_i32  the same DP on the integer profile
  *   gmfa_chain_backtrack graphmambaformer/alignment/chaining.py:530-576
  *                      AffineChainer._backtrack (host code)
+ *   gmfa_wfa_distance / gmfa_wfa_align  graphmambaformer/alignment/extension.py:308-463
+ *                      WavefrontAligner.align for a batch of pairs
  *   gmfa_chain_sort    graphmambaformer/alignment/types.py:149-152 (DP order)
  *   gmfa_chain_backtrack_device / gmfa_chain_select  chaining.py:530-617
  *                      (_backtrack and select on the device)
@@ -207,6 +209,27 @@ int gmfa_chain_dp_i32(const int64_t *read_end, const int64_t *ref_end, const int
 int64_t gmfa_chain_backtrack(int64_t n, const double *f, const int64_t *parent,
                              int64_t min_anchors, double min_score, int64_t *members,
                              int64_t *chain_start, double *chain_score);
+
+/* ---------------------------------------------------------------------------
+ * Unit-cost wavefront alignment (graphmambaformer/alignment/extension.py:308-463
+ * WavefrontAligner; accel/cuda_kernels.py:290-357 wfa_distance) for a batch
+ * of pairs in the gmfa_sw_batch layout (band fields unused).
+ * ------------------------------------------------------------------------- */
+
+/* Edit distance of every pair, -1 when above max_distance (the reference's
+ * RuntimeError).  cap >= min(max_distance, max over pairs of max(qlen, tlen));
+ * scratch: 2 * (2 cap + 3) int32 per pair. */
+int gmfa_wfa_distance(const gmfa_sw_batch *batch, int32_t max_distance, int32_t cap,
+                      int32_t *scratch, int32_t *distance, void *stream);
+
+/* CIGAR of every pair with distance >= 0: BAM runs (len << 4 | op) at
+ * cigar[cigar_off[p]] (capacity qlen + tlen + 1), n_cigar[p] runs, counts
+ * (=, X, I, D).  hist: (distance + 1)^2 int32 per pair with both sides
+ * non-empty, at hist_off[p].  *status (zeroed by the caller) is set if a
+ * traceback leaves the first wavefront (the reference would fail there). */
+int gmfa_wfa_align(const gmfa_sw_batch *batch, const int32_t *distance, const int64_t *hist_off,
+                   int32_t *hist, const int64_t *cigar_off, uint32_t *cigar, int32_t *n_cigar,
+                   int32_t *counts, int32_t *status, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Chaining around the DP, on the device, for a batch of segments (a segment =

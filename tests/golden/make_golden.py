@@ -20,6 +20,9 @@ else.  Cases target what the brute-force and known-answer checks leave open:
                     DP order, _backtrack and select, incl. equal-f ties.
 * extend.json       ExtensionEngine.extend_chains (extension.py:511-641):
                     per-read shared half-band geometry, soft clips, lifts.
+* wfa.json          WavefrontAligner.align (extension.py:308-463) incl. N
+                    bases, empty sequences, ties and the over-budget signal,
+                    and ExtensionEngine's WFA branch (extension.py:643-730).
 * seeding.json      hash64 / pack_kmers / minimizer_mask (seeding.py:83-199)
                     and whole GPUKmerIndex.query outputs (seeding.py:720-798,
                     its portable lookup path) incl. N bases, repeats, max_occ.
@@ -309,6 +312,86 @@ def seeding_cases(rng):
     return out
 
 
+def wfa_cases(rng):
+    sys.path.insert(0, REF)
+    from graphmambaformer.alignment.extension import ExtensionEngine, WavefrontAligner
+    from graphmambaformer.alignment.seeding import encode_bases
+    from graphmambaformer.alignment.types import AnchorSet, Chain
+    from graphmambaformer.config import ExtensionConfig
+
+    bases = "ACGT"
+    out = {"align": [], "extend": []}
+
+    def mutate(a, n_ed, alphabet="ACGT"):
+        b = list(a)
+        for _ in range(n_ed):
+            if not b:
+                b.append(alphabet[int(rng.integers(0, len(alphabet)))])
+                continue
+            pos = int(rng.integers(0, len(b)))
+            act = int(rng.integers(0, 3))
+            if act == 0:
+                b[pos] = alphabet[int(rng.integers(0, len(alphabet)))]
+            elif act == 1:
+                b.pop(pos)
+            else:
+                b.insert(pos, alphabet[int(rng.integers(0, len(alphabet)))])
+        return "".join(b)
+
+    pairs = [("", ""), ("", "ACG"), ("ACGT", ""), ("AN", "AN"), ("ACGT", "TGCA"),
+             ("A", "A"), ("A", "C"), ("NNNN", "NNNN"), ("ACGTN", "ACGTA")]
+    for n in range(120):
+        L = int(rng.integers(1, 60)) if n < 90 else int(rng.integers(200, 900))
+        a = "".join(bases[int(x)] for x in rng.integers(0, 4, size=L))
+        if n % 9 == 0:
+            a = "".join("ACGTN"[int(x)] for x in rng.integers(0, 5, size=L))
+        if n % 7 == 0:  # homopolymer / tandem content, ties in the traceback
+            a = "".join(bases[int(x)] * int(rng.integers(1, 6)) for x in rng.integers(0, 4, size=L))
+        b = mutate(a, int(rng.integers(0, max(2, L // 8))), "ACGTN" if n % 11 == 0 else "ACGT")
+        pairs.append((a, b))
+    for n in range(10):  # unrelated: over a small budget
+        pairs.append(("".join(bases[int(x)] for x in rng.integers(0, 4, size=40)),
+                      "".join(bases[int(x)] for x in rng.integers(0, 4, size=40))))
+    for n, (a, b) in enumerate(pairs):
+        maxd = 6 if n >= len(pairs) - 10 else 4096
+        wfa = WavefrontAligner(ExtensionConfig(algorithm="wfa", wfa_max_distance=maxd))
+        try:
+            d, cig = wfa.align(encode_bases(a), encode_bases(b))
+            out["align"].append({"q": a, "t": b, "max_distance": maxd, "distance": int(d),
+                                 "cigar": [[op, int(k)] for op, k in cig]})
+        except RuntimeError:
+            out["align"].append({"q": a, "t": b, "max_distance": maxd, "distance": None,
+                                 "cigar": None})
+    # ExtensionEngine WFA branch, incl. the banded fallback past the budget
+    ref = "".join(bases[int(x)] for x in rng.integers(0, 4, size=4000))
+    for n in range(12):
+        s0 = int(rng.integers(0, 3000))
+        L = int(rng.integers(50, 700))
+        read = mutate(ref[s0:s0 + L], int(rng.integers(0, max(2, L // 30))))
+        strand = 1
+        if n % 4 == 3:
+            read = "".join({"A": "T", "C": "G", "G": "C", "T": "A"}.get(c, "N")
+                           for c in reversed(read))
+            strand = -1
+        maxd = 3 if n % 5 == 4 else 4096
+        cfg = ExtensionConfig(algorithm="wfa", wfa_max_distance=maxd, half_band=24, flank=40)
+        anchors = AnchorSet.from_lists([0, 20], [s0, s0 + 20], [15, 15], [strand, strand],
+                                       read_len=len(read), ref_len=len(ref))
+        chains = [Chain(anchor_idx=np.array([0, 1]), score=40.0, strand=strand, read_start=0,
+                        read_end=int(rng.integers(30, len(read) + 1)), ref_start=s0,
+                        ref_end=s0 + int(rng.integers(30, L + 5)))]
+        res = ExtensionEngine(cfg).extend_chains(read, chains, anchors, ref)
+        out["extend"].append({
+            "ref": ref, "read": read, "max_distance": maxd,
+            "chain": dict(strand=strand, read_start=chains[0].read_start,
+                          read_end=chains[0].read_end, ref_start=chains[0].ref_start,
+                          ref_end=chains[0].ref_end),
+            "out": [dict(score=float(e.score), cigar=[[op, int(k)] for op, k in e.cigar],
+                         read_start=e.read_start, read_end=e.read_end, ref_start=e.ref_start,
+                         ref_end=e.ref_end) for e in res]})
+    return out
+
+
 def main():
     want = set(sys.argv[1:])
     R = _imports()
@@ -319,6 +402,7 @@ def main():
         "chainer.json": lambda: chainer_cases(R, rng),
         "extend.json": lambda: extend_cases(R, rng),
         "seeding.json": lambda: seeding_cases(np.random.default_rng(20261204)),
+        "wfa.json": lambda: wfa_cases(np.random.default_rng(20261205)),
     }
     for name, make in out.items():
         data = make()  # in file order: every file draws from the shared stream
