@@ -101,6 +101,7 @@ SIGNATURES = {
     "gmfa_chain_dp_i32": (ctypes.c_int, [_P, _P, _P, _P, _i32, _i32, _i64, _i32, _i32, _P,
                                          _P, _P]),
     "gmfa_chain_backtrack": (_i64, [_i64, _P, _P, _i64, ctypes.c_double, _P, _P, _P]),
+    "gmfa_cigar_bam": (ctypes.c_int, [_i64] + [_P] * 14),
     "gmfa_wfa_distance": (ctypes.c_int, [ctypes.POINTER(SwBatch), _i32, _i32, _P, _P, _P]),
     "gmfa_wfa_align": (ctypes.c_int, [ctypes.POINTER(SwBatch)] + [_P] * 9),
     "gmfa_chain_sort": (ctypes.c_int, [_P, _i32, _i64] + [_P] * 13 + [_P]),

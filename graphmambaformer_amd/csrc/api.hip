@@ -89,3 +89,84 @@ extern "C" int64_t gmfa_chain_backtrack(int64_t n, const double *f, const int64_
     }
     return nc;
 }
+
+// ---------------------------------------------------------------------------
+// BAM CIGAR records (graphmambaformer/data/formats.py:52, 343-382 _make_segment,
+// with ExtensionEngine's soft clips, extension.py:628-640)
+// ---------------------------------------------------------------------------
+namespace gmfa {
+
+__device__ __forceinline__ bool query_op(uint32_t op)
+{
+    return op == 0 || op == GMFA_OP_I || op == GMFA_OP_S || op == GMFA_OP_EQ || op == GMFA_OP_X;
+}
+
+// One record per thread: S(read_start) + runs + S(read_len - read_end), zero
+// runs dropped and equal neighbours merged; a record is mapped with a
+// non-empty CIGAR, a positive reference span and a sequence; reverse-strand
+// mapped records are reversed to the forward strand and flagged 0x10; a
+// query-consuming length other than the read length unmaps (0x4, no CIGAR).
+__global__ void cigar_bam_kernel(int64_t n, const uint32_t *cigar, const int64_t *cig_off,
+                                 const int32_t *n_cigar, const int32_t *read_start,
+                                 const int32_t *read_end, const int32_t *read_len,
+                                 const int64_t *ref_start, const int64_t *ref_end,
+                                 const int32_t *strand, uint32_t *out, const int64_t *out_off,
+                                 int32_t *out_n, int32_t *flag)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t *in = cigar + cig_off[r];
+    uint32_t *o = out + out_off[r];
+    int nr = 0;
+    auto put = [&](uint32_t op, uint32_t len) {
+        if (len == 0) return;
+        if (nr > 0 && (o[nr - 1] & 15u) == op) o[nr - 1] += len << 4;
+        else o[nr++] = (len << 4) | op;
+    };
+    put(GMFA_OP_S, (uint32_t)max(read_start[r], 0));
+    for (int x = 0; x < n_cigar[r]; ++x) put(in[x] & 15u, in[x] >> 4);
+    put(GMFA_OP_S, (uint32_t)max(read_len[r] - read_end[r], 0));
+    int64_t qlen = 0;
+    for (int x = 0; x < nr; ++x)
+        if (query_op(o[x] & 15u)) qlen += o[x] >> 4;
+    // the reference sets the reverse flag (and reverses) before the SAM length
+    // check, so an unmapped reverse-strand record keeps 0x10
+    bool mapped = nr > 0 && ref_end[r] > ref_start[r] && read_len[r] > 0;
+    int f = 0;
+    if (mapped && strand[r] < 0) {
+        f |= 0x10;
+        for (int x = 0, y = nr - 1; x < y; ++x, --y) {
+            const uint32_t t = o[x];
+            o[x] = o[y];
+            o[y] = t;
+        }
+    }
+    if (mapped && qlen != (int64_t)read_len[r]) mapped = false;
+    if (!mapped) {
+        f |= 0x4;
+        nr = 0;
+    }
+    out_n[r] = nr;
+    flag[r] = f;
+}
+
+}  // namespace gmfa
+
+extern "C" int gmfa_cigar_bam(int64_t n, const uint32_t *cigar, const int64_t *cigar_off,
+                              const int32_t *n_cigar, const int32_t *read_start,
+                              const int32_t *read_end, const int32_t *read_len,
+                              const int64_t *ref_start, const int64_t *ref_end,
+                              const int32_t *strand, uint32_t *out, const int64_t *out_off,
+                              int32_t *out_n, int32_t *flag, void *stream)
+{
+    GMFA_REQUIRE(n >= 0, "negative n");
+    if (n == 0) return GMFA_OK;
+    GMFA_REQUIRE(cigar && cigar_off && n_cigar && read_start && read_end && read_len && ref_start &&
+                     ref_end && strand && out && out_off && out_n && flag,
+                 "NULL pointer");
+    hipLaunchKernelGGL(gmfa::cigar_bam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, cigar, cigar_off, n_cigar, read_start, read_end,
+                       read_len, ref_start, ref_end, strand, out, out_off, out_n, flag);
+    GMFA_CHECK_HIP(hipGetLastError());
+    return GMFA_OK;
+}

@@ -598,3 +598,33 @@ def wfa_align(inp: SwInputs, max_distance: int):
         if d_h[p] < 0:
             cigs[p] = None
     return d_h, cigs, cnt
+
+
+def cigar_bam(cigar: torch.Tensor, cigar_off: torch.Tensor, n_cigar: torch.Tensor,
+              read_start: torch.Tensor, read_end: torch.Tensor, read_len: torch.Tensor,
+              ref_start: torch.Tensor, ref_end: torch.Tensor, strand: torch.Tensor,
+              n_cigar_host: np.ndarray):
+    """BAM CIGAR records on the device (gmfa_cigar_bam): returns the runs
+    (len << 4 | op, int32 view), their offsets, counts and flags."""
+    dev = cigar.device
+    n = int(len(n_cigar_host))
+    cap = np.asarray(n_cigar_host, np.int64) + 2
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(cap, out=off[1:])
+    out = torch.zeros(max(int(off[-1]), 1), dtype=torch.int32, device=dev)
+    d_off = torch.from_numpy(off[:-1]).to(dev)
+    out_n = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+    flag = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+
+    def i32(t):
+        return t.to(device=dev, dtype=torch.int32).contiguous()
+
+    def i64(t):
+        return t.to(device=dev, dtype=torch.int64).contiguous()
+
+    args = [i32(read_start), i32(read_end), i32(read_len), i64(ref_start), i64(ref_end),
+            i32(strand)]
+    check(lib().gmfa_cigar_bam(n, ptr(cigar), ptr(i64(cigar_off)), ptr(i32(n_cigar)),
+                               *[ptr(t) for t in args], ptr(out), ptr(d_off), ptr(out_n),
+                               ptr(flag), stream_of(dev)), "gmfa_cigar_bam")
+    return out, d_off, out_n[:n], flag[:n]

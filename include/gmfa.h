@@ -28,6 +28,8 @@
  *                      AffineChainer._backtrack (host code)
  *   gmfa_wfa_distance / gmfa_wfa_align  graphmambaformer/alignment/extension.py:308-463
  *                      WavefrontAligner.align for a batch of pairs
+ *   gmfa_cigar_bam     graphmambaformer/data/formats.py:52, 343-382 CIGAR of a
+ *                      BAM record (soft clips, strand, SAM length rule)
  *   gmfa_chain_sort    graphmambaformer/alignment/types.py:149-152 (DP order)
  *   gmfa_chain_backtrack_device / gmfa_chain_select  chaining.py:530-617
  *                      (_backtrack and select on the device)
@@ -209,6 +211,19 @@ int gmfa_chain_dp_i32(const int64_t *read_end, const int64_t *ref_end, const int
 int64_t gmfa_chain_backtrack(int64_t n, const double *f, const int64_t *parent,
                              int64_t min_anchors, double min_score, int64_t *members,
                              int64_t *chain_start, double *chain_score);
+
+/* BAM CIGAR records (graphmambaformer/data/formats.py:52, 343-382, with the
+ * soft clips of extension.py:628-640): per record, S(read_start) + the runs +
+ * S(read_len - read_end) merged, in the BAM binary encoding (len << 4 | op);
+ * a mapped record (runs, ref_end > ref_start, read_len > 0) on strand -1 is
+ * reversed to the forward strand with flag 0x10; if its query-consuming
+ * length differs from read_len it becomes unmapped: flag |= 0x4, no CIGAR.
+ * out[out_off[r]] needs room for n_cigar[r] + 2 runs. */
+int gmfa_cigar_bam(int64_t n, const uint32_t *cigar, const int64_t *cigar_off,
+                   const int32_t *n_cigar, const int32_t *read_start, const int32_t *read_end,
+                   const int32_t *read_len, const int64_t *ref_start, const int64_t *ref_end,
+                   const int32_t *strand, uint32_t *out, const int64_t *out_off, int32_t *out_n,
+                   int32_t *flag, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Unit-cost wavefront alignment (graphmambaformer/alignment/extension.py:308-463

@@ -97,3 +97,56 @@ def test_extension_engine_wfa_golden():
             assert [list(x) for x in g[0].cigar] == e["cigar"], k
             assert (g[0].read_start, g[0].read_end, g[0].ref_start, g[0].ref_end) == (
                 e["read_start"], e["read_end"], e["ref_start"], e["ref_end"]), k
+
+
+def test_cigar_bam_records_vs_oracle():
+    """gmfa_cigar_bam on random CIGARs (clips, strands, length mismatches)."""
+    import torch
+
+    from graphmambaformer_amd import kernels as K
+    from oracle import bam as B
+
+    _say("case: BAM CIGAR records")
+    rng = np.random.default_rng(12)
+    ops = ["=", "X", "I", "D"]
+    code = {"=": 7, "X": 8, "I": 1, "D": 2}
+    recs = []
+    for r in range(500):
+        cig = []
+        for _ in range(int(rng.integers(0, 8))):
+            op = ops[int(rng.integers(0, 4))]
+            n = int(rng.integers(1, 20))
+            if cig and cig[-1][0] == op:
+                cig[-1] = (op, cig[-1][1] + n)
+            else:
+                cig.append((op, n))
+        qcons = sum(n for op, n in cig if op in "=XI")
+        rs = int(rng.integers(0, 5))
+        L = rs + qcons + int(rng.integers(0, 5))
+        re_ = rs + qcons
+        if r % 9 == 0:
+            L += 1  # clip length disagrees with the read: the length rule applies
+        fs = int(rng.integers(0, 1000))
+        fe = fs + sum(n for op, n in cig if op in "=XD") + (0 if r % 13 else 0)
+        recs.append((cig, rs, re_, L - (1 if r % 9 == 0 else 0), L, fs, fe,
+                     1 if r % 2 else -1))
+    runs, offs, nc = [], [0], []
+    for cig, *_ in recs:
+        runs += [(n << 4) | code[op] for op, n in cig]
+        nc.append(len(cig))
+        offs.append(len(runs))
+    dev = torch.device(DEV)
+    t = lambda a, dt: torch.tensor(a, dtype=dt, device=dev)  # noqa: E731
+    out, d_off, out_n, flag = K.cigar_bam(
+        t(runs or [0], torch.int64).to(torch.int32), t(offs[:-1], torch.int64),
+        t(nc, torch.int32), t([x[1] for x in recs], torch.int32),
+        t([x[3] for x in recs], torch.int32), t([x[4] for x in recs], torch.int32),
+        t([x[5] for x in recs], torch.int64), t([x[6] for x in recs], torch.int64),
+        t([x[7] for x in recs], torch.int32), np.array(nc))
+    out = out.cpu().numpy().view(np.uint32)
+    d_off, out_n, flag = d_off.cpu().numpy(), out_n.cpu().numpy(), flag.cpu().numpy()
+    for r, (cig, rs, re_, L_seen, L, fs, fe, st) in enumerate(recs):
+        full = B.record_cigar(cig, rs, re_, L)
+        exp_runs, exp_flag = B.bam_record(full, L, fs, fe, st)
+        got = out[d_off[r]:d_off[r] + out_n[r]].tolist()
+        assert got == exp_runs and int(flag[r]) == exp_flag, r
