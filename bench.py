@@ -275,6 +275,44 @@ def seeding_extra(torch, WL, dev):
     return res
 
 
+def wfa_extra(torch, K, WL, dev):
+    """Batched wavefront alignment (distance + CIGAR) of 4,096 10 kb pairs
+    with 1% errors vs the oracle's WFA on a sample."""
+    from oracle import wfa as OW
+
+    rng = np.random.default_rng(8)
+    qs, ts = [], []
+    for _ in range(4096):
+        t = rng.integers(0, 4, size=10_000).astype(np.uint8)
+        qs.append(WL.mutate(rng, t, 0.01, 0.5, 3).astype(np.int8) + 1)
+        ts.append(t.astype(np.int8) + 1)
+    inp = K.upload_pairs(qs, ts, np.zeros(len(qs), np.int32), dev, alphabet="ref")
+    K.wfa_align(inp, 4096)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        dist, _, _ = K.wfa_align(inp, 4096)
+    gpu_s = (time.perf_counter() - t1) / reps
+    sub = range(0, len(qs), 256)
+    t1 = time.perf_counter()
+    for p in sub:
+        OW.wfa_align(qs[p], ts[p], 4096)
+    cpu_s = (time.perf_counter() - t1) / len(sub)
+    res = {
+        "pairs_per_s": round(len(qs) / gpu_s, 1),
+        "ms": round(gpu_s * 1e3, 3),
+        "mean_distance": round(float(np.mean(dist)), 2),
+        "cpu_pairs_per_s": round(1.0 / cpu_s, 2),
+        "cpu_threads": 1,
+        "workload": "4096 pairs x 10 kb, 1% errors (half 1-3 bp indels), max_distance 4096, "
+                    "seed 8; GPU: distance pass + history/traceback pass + CIGAR readback; CPU: "
+                    "the oracle's (pure Python) WFA on every 256th pair",
+    }
+    log(f"[bench] wfa: {res}")
+    return res
+
+
 def host_info():
     model = "unknown"
     try:
@@ -618,6 +656,7 @@ def main():
 
     # ----------------------------------------------------------------- seeding
     result["seeding"] = seeding_extra(torch, WL, dev)
+    result["wfa"] = wfa_extra(torch, K, WL, dev)
 
     print(json.dumps(result), flush=True)
 

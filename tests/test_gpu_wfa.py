@@ -122,14 +122,15 @@ def test_cigar_bam_records_vs_oracle():
                 cig.append((op, n))
         qcons = sum(n for op, n in cig if op in "=XI")
         rs = int(rng.integers(0, 5))
-        L = rs + qcons + int(rng.integers(0, 5))
         re_ = rs + qcons
-        if r % 9 == 0:
-            L += 1  # clip length disagrees with the read: the length rule applies
+        L = re_ + int(rng.integers(0, 5))
+        if r % 9 == 0 and re_ > 0:
+            L = re_ - 1  # read end past the read: the right clip vanishes, length rule fails
         fs = int(rng.integers(0, 1000))
-        fe = fs + sum(n for op, n in cig if op in "=XD") + (0 if r % 13 else 0)
-        recs.append((cig, rs, re_, L - (1 if r % 9 == 0 else 0), L, fs, fe,
-                     1 if r % 2 else -1))
+        fe = fs + sum(n for op, n in cig if op in "=XD")
+        if r % 13 == 0:
+            fe = fs  # empty reference span: unmapped
+        recs.append((cig, rs, re_, L, fs, fe, 1 if r % 2 else -1))
     runs, offs, nc = [], [0], []
     for cig, *_ in recs:
         runs += [(n << 4) | code[op] for op, n in cig]
@@ -140,13 +141,16 @@ def test_cigar_bam_records_vs_oracle():
     out, d_off, out_n, flag = K.cigar_bam(
         t(runs or [0], torch.int64).to(torch.int32), t(offs[:-1], torch.int64),
         t(nc, torch.int32), t([x[1] for x in recs], torch.int32),
-        t([x[3] for x in recs], torch.int32), t([x[4] for x in recs], torch.int32),
-        t([x[5] for x in recs], torch.int64), t([x[6] for x in recs], torch.int64),
-        t([x[7] for x in recs], torch.int32), np.array(nc))
+        t([x[2] for x in recs], torch.int32), t([x[3] for x in recs], torch.int32),
+        t([x[4] for x in recs], torch.int64), t([x[5] for x in recs], torch.int64),
+        t([x[6] for x in recs], torch.int32), np.array(nc))
     out = out.cpu().numpy().view(np.uint32)
     d_off, out_n, flag = d_off.cpu().numpy(), out_n.cpu().numpy(), flag.cpu().numpy()
-    for r, (cig, rs, re_, L_seen, L, fs, fe, st) in enumerate(recs):
+    n_unmapped = 0
+    for r, (cig, rs, re_, L, fs, fe, st) in enumerate(recs):
         full = B.record_cigar(cig, rs, re_, L)
         exp_runs, exp_flag = B.bam_record(full, L, fs, fe, st)
         got = out[d_off[r]:d_off[r] + out_n[r]].tolist()
         assert got == exp_runs and int(flag[r]) == exp_flag, r
+        n_unmapped += bool(exp_flag & 4)
+    assert 0 < n_unmapped < len(recs)
