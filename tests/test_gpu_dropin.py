@@ -303,3 +303,43 @@ def test_affine_chainer_device_pipeline_vs_oracle(min_score, lookback, ratio, ma
             assert (g.strand, g.read_start, g.read_end, g.ref_start, g.ref_end, g.is_primary) == (
                 e["strand"], e["read_start"], e["read_end"], e["ref_start"], e["ref_end"],
                 e["is_primary"]), r
+
+
+def test_tier_kmer_lookup_and_wfa_distance():
+    """cuda_kernels.kmer_lookup / wfa_distance signatures: the KAT of
+    tests/test_accel.py:336-345 (ACGTAC vs ACGTTAC -> 1), random pairs vs the
+    oracle, and table lookups vs the oracle's KmerTable."""
+    from graphmambaformer_amd.accel import kmer_lookup, wfa_distance
+    from oracle import seeding as S
+    from oracle import wfa as W
+
+    _say("case: tier kmer_lookup / wfa_distance")
+    q = torch.tensor(O.encode("ACGTAC"), dtype=torch.int8, device=DEV)[None]
+    t = torch.tensor(O.encode("ACGTTAC"), dtype=torch.int8, device=DEV)[None]
+    d = wfa_distance(q, t, torch.tensor([6], device=DEV), torch.tensor([7], device=DEV), 4)
+    assert int(d[0]) == 1
+    rng = np.random.default_rng(21)
+    qs = [rng.integers(1, 6, size=int(rng.integers(0, 80))) for _ in range(40)]
+    ts = [np.concatenate([x[:len(x) // 2], rng.integers(1, 5, size=int(rng.integers(0, 6))),
+                          x[len(x) // 2:]]) for x in qs]
+    M = max(len(x) for x in qs) + 1
+    N = max(len(x) for x in ts) + 1
+    Q = np.full((40, M), -1, np.int8)
+    T = np.full((40, N), -1, np.int8)
+    for b in range(40):
+        Q[b, :len(qs[b])] = qs[b]
+        T[b, :len(ts[b])] = ts[b]
+    d = wfa_distance(torch.from_numpy(Q).to(DEV), torch.from_numpy(T).to(DEV),
+                     torch.tensor([len(x) for x in qs]), torch.tensor([len(x) for x in ts]), 3)
+    for b in range(40):
+        e = W.wfa_align(qs[b], ts[b], 3)
+        assert int(d[b]) == (-1 if e is None else e[0]), b
+    ref = rng.integers(1, 5, size=20000).astype(np.int8)
+    tab = S.build_index(ref, 13, 6, 0)
+    qk = np.concatenate([tab.keys[::3], rng.integers(0, 1 << 26, size=500)]).astype(np.int64)
+    st, ct = kmer_lookup(torch.from_numpy(tab.keys).to(DEV),
+                         torch.from_numpy(tab.offsets.astype(np.int32)).to(DEV),
+                         torch.from_numpy(qk).to(DEV))
+    es, ec = tab.lookup(qk)
+    assert np.array_equal(st.cpu().numpy(), es.astype(np.int32))
+    assert np.array_equal(ct.cpu().numpy(), ec.astype(np.int32))
