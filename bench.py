@@ -287,13 +287,18 @@ def wfa_extra(torch, K, WL, dev):
         qs.append(WL.mutate(rng, t, 0.01, 0.5, 3).astype(np.int8) + 1)
         ts.append(t.astype(np.int8) + 1)
     inp = K.upload_pairs(qs, ts, np.zeros(len(qs), np.int32), dev, alphabet="ref")
-    K.wfa_align(inp, 4096)
+    K.wfa_align_device(inp, 4096)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     reps = 3
     for _ in range(reps):
-        dist, _, _ = K.wfa_align(inp, 4096)
+        r = K.wfa_align_device(inp, 4096)
+    torch.cuda.synchronize()
     gpu_s = (time.perf_counter() - t1) / reps
+    dist = r.distance
+    t1 = time.perf_counter()
+    K.wfa_align(inp, 4096)
+    host_s = time.perf_counter() - t1
     sub = range(0, len(qs), 256)
     t1 = time.perf_counter()
     for p in sub:
@@ -302,12 +307,15 @@ def wfa_extra(torch, K, WL, dev):
     res = {
         "pairs_per_s": round(len(qs) / gpu_s, 1),
         "ms": round(gpu_s * 1e3, 3),
+        "with_host_cigars_ms": round(host_s * 1e3, 3),
         "mean_distance": round(float(np.mean(dist)), 2),
         "cpu_pairs_per_s": round(1.0 / cpu_s, 2),
         "cpu_threads": 1,
         "workload": "4096 pairs x 10 kb, 1% errors (half 1-3 bp indels), max_distance 4096, "
-                    "seed 8; GPU: distance pass + history/traceback pass + CIGAR readback; CPU: "
-                    "the oracle's (pure Python) WFA on every 256th pair",
+                    "seed 8; pairs resident on the device; GPU: distance pass + distance readback + "
+                    "history/traceback pass, CIGARs left on the device (with_host_cigars_ms adds "
+                    "the CIGAR readback and decoding into Python lists); CPU: the oracle's (pure "
+                    "Python) WFA on every 256th pair",
     }
     log(f"[bench] wfa: {res}")
     return res
@@ -626,7 +634,17 @@ def main():
     t1 = time.perf_counter()
     for _ in range(3):
         pres = pipeline()
-    pms = (time.perf_counter() - t1) / 3 * 1e3
+    pms_host = (time.perf_counter() - t1) / 3 * 1e3
+    up_ = K.chain_upload(rp_ + 15, fp_ + 15, rp_, fp_, np.full(len(rp_), 15.0), ids, wc.seg_off,
+                         read_seg, wc.seg_off[:-1], dev)
+
+    def device_run():
+        return K.chain_run(up_, ccfg.max_lookback, ccfg.max_gap, lut64, ccfg.min_chain_anchors,
+                           ccfg.min_chain_score, ccfg.secondary_score_ratio,
+                           ccfg.secondary_overlap, ccfg.max_chains)
+
+    device_run()
+    pms = time_events(torch, device_run, 5)
     sub = np.arange(0, nrd, 16)
     t1 = time.perf_counter()
     for r in sub:
@@ -644,13 +662,15 @@ def main():
         "cpu_threads": 1,
         "anchors_to_chains_ms": round(pms, 3),
         "anchors_to_chains_reads_per_s": round(nrd / (pms / 1e3), 1),
+        "anchors_to_chains_with_host_ms": round(pms_host, 3),
         "anchors_to_chains_kept": int(pres.n_kept.sum()),
         "cpu_anchors_to_chains_reads_per_s": round(1.0 / cpu_pipe, 1),
         "workload": "4096 reads, Poisson(400) anchors, lookback 50, max_gap 5000, "
                     "int profile 6/1 (and float64 6/0.05/0.5), seed 1; CPU: whole batch, "
-                    "int profile, one thread; anchors_to_chains: unsorted anchors in, "
-                    "sort + float64 DP + backtrack + select on the device, host arrays in and "
-                    "chain records out (CPU: the oracle's chain_read on every 16th read)",
+                    "int profile, one thread; anchors_to_chains: unsorted anchors resident on "
+                    "the device, sort + float64 DP + backtrack + select (with_host adds the "
+                    "uploads and the readback of the chain records); CPU: the oracle's "
+                    "chain_read on every 16th read",
     }
     log(f"[bench] chains: {result['chains']}")
 

@@ -341,14 +341,31 @@ def sw_traceback(inp: SwInputs, fwd: SwForward, score=None, query_end=None, targ
 
 
 def decode_cigars(tbk: SwTraceback) -> list[list[tuple[str, int]]]:
-    """Device CIGAR runs -> list of [(op, n)] per pair (one D2H copy)."""
-    cig = tbk.cigar.cpu().numpy().view(np.uint32)
-    off = tbk.cigar_off.cpu().numpy()
-    nc = tbk.n_cigar.cpu().numpy()
+    """Device CIGAR runs -> list of [(op, n)] per pair (the runs are gathered
+    on the device, so only they cross to the host)."""
+    off = tbk.cigar_off.to(torch.int64)
+    nc = tbk.n_cigar.to(torch.int64)
+    n = int(nc.numel())
+    if n == 0:
+        return []
+    total = int(nc.sum().item())
+    if total:
+        starts = torch.cumsum(nc, 0) - nc
+        owner = torch.repeat_interleave(torch.arange(n, device=nc.device), nc)
+        within = torch.arange(total, device=nc.device) - starts[owner]
+        runs = tbk.cigar[off[owner] + within].cpu().numpy().view(np.uint32)
+    else:
+        runs = np.zeros(0, np.uint32)
+    nc_h = nc.cpu().numpy()
+    ops = (runs & 15).astype(np.int64)
+    lens = (runs >> 4).astype(np.int64)
+    chars = np.array([_lib.OP_CHAR.get(c, "?") for c in range(16)])[ops]
     out = []
-    for p in range(len(nc)):
-        runs = cig[off[p]:off[p] + nc[p]]
-        out.append([(_lib.OP_CHAR[int(r & 15)], int(r >> 4)) for r in runs])
+    x = 0
+    for p in range(n):
+        k = int(nc_h[p])
+        out.append(list(zip(chars[x:x + k].tolist(), lens[x:x + k].tolist())))
+        x += k
     return out
 
 
@@ -492,95 +509,122 @@ class ChainBatchResult:
     strand: np.ndarray     # per chain record: +1 / -1
 
 
+def chain_upload(read_end, ref_end, read_pos, ref_pos, weight, anchor_id, seg_off, read_seg,
+                 read_base, device) -> dict:
+    """Copy a batch of segments (anchors back to back in their original order)
+    to the device for chain_run."""
+    dev = torch.device(device)
+
+    def up(a, dt):
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a).astype(dt))).to(dev)
+
+    return {"re": up(read_end, np.int64), "fe": up(ref_end, np.int64),
+            "w": up(weight, np.float64), "rp": up(read_pos, np.int64),
+            "fp": up(ref_pos, np.int64), "id": up(anchor_id, np.int64),
+            "seg_off": up(seg_off, np.int64), "read_seg": up(read_seg, np.int32),
+            "read_base": up(read_base, np.int64), "n": int(len(read_end)),
+            "n_seg": int(len(seg_off) - 1), "n_reads": int(len(read_seg))}
+
+
+def chain_run(u: dict, max_lookback: int, max_gap: int, pen_lut: torch.Tensor, min_anchors: int,
+              min_score: float, score_ratio: float, max_overlap: float, max_chains: int) -> dict:
+    """Sort into DP order, DP, backtrack and select for an uploaded batch, all
+    on the device (results stay there)."""
+    n, ns, nr = u["n"], u["n_seg"], u["n_reads"]
+    dev = u["re"].device
+    src = [u["re"], u["fe"], u["w"], u["rp"], u["fp"], u["id"]]
+    o_re, o_fe = torch.empty_like(src[0]), torch.empty_like(src[1])
+    o_w, o_rp = torch.empty_like(src[2]), torch.empty_like(src[3])
+    o_fp, o_id = torch.empty_like(src[4]), torch.empty_like(src[5])
+    scratch = torch.empty(max(2 * n, 1), dtype=torch.int32, device=dev)
+    s = stream_of(dev)
+    check(lib().gmfa_chain_sort(ptr(u["seg_off"]), ns, n, *[ptr(t) for t in src], ptr(scratch),
+                                ptr(o_re), ptr(o_fe), ptr(o_w), ptr(o_rp), ptr(o_fp), ptr(o_id),
+                                s), "gmfa_chain_sort")
+    f, parent = chain_dp(o_re, o_fe, o_w, u["seg_off"], max_lookback, max_gap, pen_lut=pen_lut)
+    r = {"claimed": torch.empty(max(n, 1), dtype=torch.uint8, device=dev),
+         "n_chains": torch.zeros(max(ns, 1), dtype=torch.int32, device=dev),
+         "score": torch.zeros(max(n, 1), dtype=torch.float64, device=dev),
+         "span": torch.zeros((max(n, 1), 4), dtype=torch.int64, device=dev),
+         "mem": torch.zeros((max(n, 1), 2), dtype=torch.int64, device=dev),
+         "members": torch.zeros(max(n, 1), dtype=torch.int64, device=dev),
+         "bad": torch.zeros(1, dtype=torch.int32, device=dev),
+         "n_kept": torch.zeros(max(nr, 1), dtype=torch.int32, device=dev),
+         "kept": torch.zeros((max(nr, 1), int(max_chains)), dtype=torch.int64, device=dev)}
+    check(lib().gmfa_chain_backtrack_device(ptr(u["seg_off"]), ns, n, ptr(f), ptr(parent),
+                                            ptr(o_rp), ptr(o_re), ptr(o_fp), ptr(o_fe), ptr(o_id),
+                                            int(min_anchors), float(min_score), ptr(scratch),
+                                            ptr(r["claimed"]), ptr(r["n_chains"]),
+                                            ptr(r["score"]), ptr(r["span"]), ptr(r["mem"]),
+                                            ptr(r["members"]), ptr(r["bad"]), s),
+          "gmfa_chain_backtrack_device")
+    sel_scratch = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    check(lib().gmfa_chain_select(nr, ptr(u["read_seg"]), ptr(u["seg_off"]), ptr(r["n_chains"]),
+                                  ptr(r["score"]), ptr(r["span"]), float(score_ratio),
+                                  float(max_overlap), int(max_chains), ptr(sel_scratch),
+                                  ptr(u["read_base"]), ptr(r["n_kept"]), ptr(r["kept"]), s),
+          "gmfa_chain_select")
+    return r
+
+
 def chain_pipeline(read_end: np.ndarray, ref_end: np.ndarray, read_pos: np.ndarray,
                    ref_pos: np.ndarray, weight: np.ndarray, anchor_id: np.ndarray,
                    seg_off: np.ndarray, seg_strand: np.ndarray, read_seg: np.ndarray,
                    read_base: np.ndarray, max_lookback: int, max_gap: int, pen_lut: torch.Tensor,
                    min_anchors: int, min_score: float, score_ratio: float, max_overlap: float,
                    max_chains: int, device) -> ChainBatchResult:
-    """Sort into DP order, DP, backtrack and select for a batch of segments,
-    all on the device; one readback of the results.  Arrays hold the
-    segments' anchors back to back in their original order."""
-    dev = torch.device(device)
-    n = int(len(read_end))
-    ns = int(len(seg_off) - 1)
-    nr = int(len(read_seg))
-
-    def up(a, dt=None):
-        return torch.from_numpy(np.ascontiguousarray(a if dt is None else a.astype(dt))).to(dev)
-
-    d_so = up(seg_off, np.int64)
-    src = [up(read_end, np.int64), up(ref_end, np.int64), up(weight, np.float64),
-           up(read_pos, np.int64), up(ref_pos, np.int64), up(anchor_id, np.int64)]
-    o_re, o_fe = torch.empty_like(src[0]), torch.empty_like(src[1])
-    o_w, o_rp = torch.empty_like(src[2]), torch.empty_like(src[3])
-    o_fp, o_id = torch.empty_like(src[4]), torch.empty_like(src[5])
-    scratch = torch.empty(max(2 * n, 1), dtype=torch.int32, device=dev)
-    s = stream_of(dev)
-    check(lib().gmfa_chain_sort(ptr(d_so), ns, n, *[ptr(t) for t in src], ptr(scratch),
-                                ptr(o_re), ptr(o_fe), ptr(o_w), ptr(o_rp), ptr(o_fp), ptr(o_id),
-                                s), "gmfa_chain_sort")
-    f, parent = chain_dp(o_re, o_fe, o_w, d_so, max_lookback, max_gap, pen_lut=pen_lut)
-    claimed = torch.empty(max(n, 1), dtype=torch.uint8, device=dev)
-    n_chains = torch.zeros(max(ns, 1), dtype=torch.int32, device=dev)
-    c_score = torch.zeros(max(n, 1), dtype=torch.float64, device=dev)
-    c_span = torch.zeros((max(n, 1), 4), dtype=torch.int64, device=dev)
-    c_mem = torch.zeros((max(n, 1), 2), dtype=torch.int64, device=dev)
-    members = torch.zeros(max(n, 1), dtype=torch.int64, device=dev)
-    bad = torch.zeros(1, dtype=torch.int32, device=dev)
-    check(lib().gmfa_chain_backtrack_device(ptr(d_so), ns, n, ptr(f), ptr(parent), ptr(o_rp),
-                                            ptr(o_re), ptr(o_fp), ptr(o_fe), ptr(o_id),
-                                            int(min_anchors), float(min_score), ptr(scratch),
-                                            ptr(claimed), ptr(n_chains), ptr(c_score),
-                                            ptr(c_span), ptr(c_mem), ptr(members), ptr(bad), s),
-          "gmfa_chain_backtrack_device")
-    d_rs = up(read_seg, np.int32)
-    d_rb = up(read_base, np.int64)
-    sel_scratch = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
-    n_kept = torch.zeros(max(nr, 1), dtype=torch.int32, device=dev)
-    kept = torch.zeros((max(nr, 1), int(max_chains)), dtype=torch.int64, device=dev)
-    check(lib().gmfa_chain_select(nr, ptr(d_rs), ptr(d_so), ptr(n_chains), ptr(c_score),
-                                  ptr(c_span), float(score_ratio), float(max_overlap),
-                                  int(max_chains), ptr(sel_scratch), ptr(d_rb), ptr(n_kept),
-                                  ptr(kept), s), "gmfa_chain_select")
-    if int(bad.item()) != 0:
+    """Upload, chain_run, one readback of the results."""
+    u = chain_upload(read_end, ref_end, read_pos, ref_pos, weight, anchor_id, seg_off, read_seg,
+                     read_base, device)
+    r = chain_run(u, max_lookback, max_gap, pen_lut, min_anchors, min_score, score_ratio,
+                  max_overlap, max_chains)
+    if int(r["bad"].item()) != 0:
         raise RuntimeError("chaining DP produced a parent outside its segment")
+    n, ns, nr = u["n"], u["n_seg"], u["n_reads"]
     seg_of_rec = np.repeat(np.arange(ns), np.diff(np.asarray(seg_off, np.int64)))
-    return ChainBatchResult(n_kept=n_kept.cpu().numpy()[:nr], kept=kept.cpu().numpy()[:nr],
-                            score=c_score.cpu().numpy()[:n], span=c_span.cpu().numpy()[:n],
-                            mem=c_mem.cpu().numpy()[:n], members=members.cpu().numpy()[:n],
+    return ChainBatchResult(n_kept=r["n_kept"].cpu().numpy()[:nr],
+                            kept=r["kept"].cpu().numpy()[:nr], score=r["score"].cpu().numpy()[:n],
+                            span=r["span"].cpu().numpy()[:n], mem=r["mem"].cpu().numpy()[:n],
+                            members=r["members"].cpu().numpy()[:n],
                             strand=np.asarray(seg_strand)[seg_of_rec] if n else np.zeros(0))
 
 
-def wfa_align(inp: SwInputs, max_distance: int):
+@dataclass
+class WfaResult:
+    distance: np.ndarray  # host, -1 above the budget
+    walk: "SwTraceback"   # device CIGAR runs / counts (read/ref starts unused)
+    status: torch.Tensor  # device int32, nonzero if a traceback left the first wavefront
+
+
+def wfa_align_device(inp: SwInputs, max_distance: int) -> WfaResult:
     """Batched unit-cost wavefront alignment (extension.py:308-463) of every
-    pair of ``inp``: distance (-1 above max_distance) and, for the others,
-    CIGAR runs and op counts.  Two launches: the distances, then (history
-    sized per pair) the alignments."""
+    pair of ``inp``, results left on the device.  Two launches: the
+    distances (read back to size the histories), then the alignments."""
     dev = inp.device
     n = inp.n_pairs
     ql = inp.qlen_host.astype(np.int64)
     tl = inp.tlen_host.astype(np.int64)
-    dist = torch.full((max(n, 1),), -1, dtype=torch.int32, device=dev)
     if n == 0:
-        return np.zeros(0, np.int32), [], np.zeros((0, 4), np.int32)
+        z = torch.zeros(0, dtype=torch.int32, device=dev)
+        return WfaResult(np.zeros(0, np.int32), SwTraceback(z, z.long(), z, z, z, z.view(0, 4)),
+                         torch.zeros(1, dtype=torch.int32, device=dev))
+    dist = torch.empty(n, dtype=torch.int32, device=dev)
     cap = int(min(int(max_distance), int(np.maximum(ql, tl).max())))
     scratch = torch.empty(n * 2 * (2 * cap + 3), dtype=torch.int32, device=dev)
     b = inp.batch_struct()
     s = stream_of(dev)
     check(lib().gmfa_wfa_distance(ctypes.byref(b), int(max_distance), cap, ptr(scratch),
                                   ptr(dist), s), "gmfa_wfa_distance")
-    d_h = dist.cpu().numpy()[:n]
+    d_h = dist.cpu().numpy()
     del scratch
     both = (ql > 0) & (tl > 0) & (d_h >= 0)
     hsz = np.where(both, (d_h.astype(np.int64) + 1) ** 2, 0)
     hoff = np.zeros(n + 1, np.int64)
     np.cumsum(hsz, out=hoff[1:])
     hist = torch.empty(max(int(hoff[-1]), 1), dtype=torch.int32, device=dev)
-    cap_c = ql + tl + 1
     coff = np.zeros(n + 1, np.int64)
-    np.cumsum(cap_c, out=coff[1:])
-    cig = torch.zeros(max(int(coff[-1]), 1), dtype=torch.int32, device=dev)
+    np.cumsum(ql + tl + 1, out=coff[1:])
+    cig = torch.empty(max(int(coff[-1]), 1), dtype=torch.int32, device=dev)
     ncig = torch.zeros(n, dtype=torch.int32, device=dev)
     counts = torch.zeros((n, 4), dtype=torch.int32, device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -588,16 +632,23 @@ def wfa_align(inp: SwInputs, max_distance: int):
     d_coff = torch.from_numpy(coff[:-1]).to(dev)
     check(lib().gmfa_wfa_align(ctypes.byref(b), ptr(dist), ptr(d_hoff), ptr(hist), ptr(d_coff),
                                ptr(cig), ptr(ncig), ptr(counts), ptr(status), s), "gmfa_wfa_align")
-    if int(status.item()) != 0:
+    walk = SwTraceback(cigar=cig, cigar_off=d_coff, n_cigar=ncig, read_start=ncig,
+                       ref_start=ncig, counts=counts)
+    return WfaResult(distance=d_h, walk=walk, status=status)
+
+
+def wfa_align(inp: SwInputs, max_distance: int):
+    """wfa_align_device + host decoding: (distance, CIGARs (None above the
+    budget), op counts)."""
+    r = wfa_align_device(inp, max_distance)
+    n = len(r.distance)
+    if n == 0:
+        return r.distance, [], np.zeros((0, 4), np.int32)
+    if int(r.status.item()) != 0:
         raise RuntimeError("WFA traceback left the first wavefront")
-    tbk = SwTraceback(cigar=cig, cigar_off=d_coff, n_cigar=ncig, read_start=ncig, ref_start=ncig,
-                      counts=counts)
-    cigs = decode_cigars(tbk)
-    cnt = counts.cpu().numpy()
-    for p in range(n):
-        if d_h[p] < 0:
-            cigs[p] = None
-    return d_h, cigs, cnt
+    cigs = decode_cigars(r.walk)
+    cnt = r.walk.counts.cpu().numpy()
+    return r.distance, [None if r.distance[p] < 0 else cigs[p] for p in range(n)], cnt
 
 
 def cigar_bam(cigar: torch.Tensor, cigar_off: torch.Tensor, n_cigar: torch.Tensor,
