@@ -236,3 +236,31 @@ def test_chain_dp_i32_segments_vs_oracle():
                                 w.max_gap, w.gap_open, w.gap_extend)
         assert np.array_equal(f[a:b], ef), s
         assert np.array_equal(par[a:b], ep), s
+
+
+def test_sw_edge_cases_vs_oracle():
+    """Empty query or target, a band of one diagonal, bands lying entirely
+    beside the target, all-N sequences, one-base pairs."""
+    rng = np.random.default_rng(31)
+    qs, ts, offs = [], [], []
+    base = rng.integers(1, 5, size=120).astype(np.int16)
+
+    def add(q, t, o):
+        qs.append(np.asarray(q, np.int16))
+        ts.append(np.asarray(t, np.int16))
+        offs.append(int(o))
+
+    add([], base[:50], 0)
+    add(base[:50], [], 0)
+    add([], [], 0)
+    add(base[:1], base[:1], 0)
+    add(base[:1], base[1:2], 0)
+    add(base[:60], base[:60], 500)    # band far right of the target
+    add(base[:60], base[:60], -500)   # band far left
+    add(np.full(40, 5), np.full(40, 5), 0)
+    add(base[:80], np.concatenate([base[:40], np.full(5, 5), base[40:80]]), 3)
+    for hb in (0, 1, 3, 64):
+        _say("case: SW edge cases hb", hb)
+        for fused in (False, True):
+            got = _gpu_align(qs, ts, offs, hb, (2, 4, 4, 2), fused=fused)
+            _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"edge{hb}")
