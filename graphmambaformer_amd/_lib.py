@@ -95,7 +95,7 @@ SIGNATURES = {
     "gmfa_sw_traceback": (ctypes.c_int, [ctypes.POINTER(SwBatch), ctypes.POINTER(SwParams),
                                          _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gmfa_sw_xdrop": (ctypes.c_int, [_P, _P, _P, _P, _P, _i32, _i32, _i32, _i32, _P, _P, _P,
-                                     _P]),
+                                     _P, _P]),
     "gmfa_chain_dp_f64": (ctypes.c_int, [_P, _P, _P, _P, _i32, _i32, _i64, _P, _P, _P, _P,
                                          _P]),
     "gmfa_chain_dp_i32": (ctypes.c_int, [_P, _P, _P, _P, _i32, _i32, _i64, _i32, _i32, _P,

@@ -140,13 +140,8 @@ def banded_affine_sw_batch(query: torch.Tensor, target: torch.Tensor, query_len:
     inp = _inputs_from_codes(query, target, qlen_h, tlen_h, off_h, dev)
     xdrop = float(cfg.x_drop) > 0
     fwd = K.sw_forward(inp, ma, mi, go, ge, hb, traceback=return_matrices, rowkeys=xdrop)
-    if xdrop and B <= 64:
+    if xdrop:
         sc_d, qe_d, te_d = K.sw_xdrop(fwd, inp, np.array([0, B]), float(cfg.x_drop), hb)
-    elif xdrop:
-        sc, qe, te = K.xdrop_best(fwd, inp.qlen_host, float(cfg.x_drop), hb, inp.boff_host)
-        sc_d = torch.from_numpy(sc).to(dev)
-        qe_d = torch.from_numpy(qe).to(dev)
-        te_d = torch.from_numpy(te).to(dev)
     else:
         sc_d, qe_d, te_d = fwd.score, fwd.query_end, fwd.target_end
     planes = None
@@ -391,11 +386,7 @@ class ExtensionEngine:
             gofs = np.array(gofs, np.int64)
             if xd > 0:
                 fwd = K.sw_forward(inp, ma, mi, go, ge, hb, traceback=True, rowkeys=True)
-                if int(np.diff(gofs).max()) <= 64:
-                    sc, qe, te = K.sw_xdrop(fwd, inp, gofs, xd, hb)
-                else:
-                    sc_h, qe_h, te_h = K.xdrop_best(fwd, ql_h, xd, hb, bo_h, group_off=gofs)
-                    sc, qe, te = up(sc_h), up(qe_h), up(te_h)
+                sc, qe, te = K.sw_xdrop(fwd, inp, gofs, xd, hb)
                 tbk = K.sw_traceback(inp, fwd, sc, qe, te)
             else:
                 fwd = K.sw_forward(inp, ma, mi, go, ge, hb, traceback=True, cigar=True)

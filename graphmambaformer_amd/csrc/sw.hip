@@ -696,6 +696,54 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 
 __global__ void reset_counter_kernel(uint32_t *c) { *c = 0u; }
 
+// The same x-drop for groups of any size: one 256-thread workgroup per group,
+// each thread owning pairs p0 + tid, p0 + tid + 256, ...; per-pair state
+// (best H, its row, its diagonal, best so far) in scratch (4 int32 per pair);
+// rows advance together with one barrier each, which also decides the stop.
+__global__ __launch_bounds__(256) void sw_xdrop_wide_kernel(
+    const int32_t *rowkey, const int64_t *rk_off, const int32_t *qlen, const int32_t *boff,
+    const int32_t *group_off, int32_t thr, int32_t hb, int32_t *st, int32_t *score,
+    int32_t *qend, int32_t *tend)
+{
+    __shared__ int red[256];
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int p0 = group_off[g], p1 = group_off[g + 1];
+    int mq = 0;
+    for (int p = p0 + tid; p < p1; p += 256) {
+        mq = max(mq, qlen[p]);
+        st[4 * (int64_t)p + 0] = 0;  // best H
+        st[4 * (int64_t)p + 1] = 0;  // its row
+        st[4 * (int64_t)p + 2] = 0;  // its diagonal
+        st[4 * (int64_t)p + 3] = 0;  // best so far of the rows seen (x-drop reference)
+    }
+    red[tid] = mq;
+    __syncthreads();
+    for (int o = 128; o >= 1; o >>= 1) {
+        if (tid < o) red[tid] = max(red[tid], red[tid + o]);
+        __syncthreads();
+    }
+    const int maxq = red[0];
+    for (int i = 1; i <= maxq; ++i) {
+        int live = 0;
+        for (int p = p0 + tid; p < p1; p += 256) {
+            const int key = i <= qlen[p] ? rowkey[rk_off[p] + i - 1] : 0;
+            const int h = key >> 11;
+            int32_t *sp = st + 4 * (int64_t)p;
+            if (h > sp[0]) { sp[0] = h; sp[1] = i; sp[2] = 2047 - (key & 2047); }
+            const int seen = max(sp[3], h);
+            sp[3] = seen;
+            if (thr < 0 || seen - h <= thr) live = 1;
+        }
+        if (!__syncthreads_or(live)) break;
+    }
+    for (int p = p0 + tid; p < p1; p += 256) {
+        const int32_t *sp = st + 4 * (int64_t)p;
+        score[p] = sp[0];
+        qend[p] = sp[0] > 0 ? sp[1] : 0;
+        tend[p] = sp[0] > 0 ? sp[1] + boff[p] - hb + sp[2] : 0;
+    }
+}
+
 // Batch-wide x-drop (extension.py:186-193) per group of pairs: the reference
 // stops a batch after the first row where every pair's best-so-far minus the
 // row's best exceeds x_drop; the result is the first strict maximum over the
@@ -1076,20 +1124,28 @@ extern "C" int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params 
 extern "C" int gmfa_sw_xdrop(const int32_t *rowkey, const int64_t *rowkey_off,
                              const int32_t *qlen, const int32_t *band_offset,
                              const int32_t *group_off, int32_t n_groups, int32_t max_group,
-                             int32_t x_drop, int32_t half_band, int32_t *score,
-                             int32_t *query_end, int32_t *target_end, void *stream)
+                             int32_t x_drop, int32_t half_band, int32_t *scratch,
+                             int32_t *score, int32_t *query_end, int32_t *target_end,
+                             void *stream)
 {
     GMFA_REQUIRE(n_groups >= 0, "negative n_groups");
     if (n_groups == 0) return GMFA_OK;
-    GMFA_REQUIRE(max_group >= 1 && max_group <= 64,
-                 "groups must hold 1..64 pairs (max_group %d)", max_group);
+    GMFA_REQUIRE(max_group >= 1, "max_group must be positive (got %d)", max_group);
     GMFA_REQUIRE(half_band >= 0 && half_band <= 1023, "half_band %d outside [0, 1023]", half_band);
     GMFA_REQUIRE(rowkey && rowkey_off && qlen && band_offset && group_off && score && query_end &&
                      target_end,
                  "NULL pointer");
-    hipLaunchKernelGGL(sw_xdrop_kernel, dim3(n_groups), dim3(64), 0, (hipStream_t)stream, rowkey,
-                       rowkey_off, qlen, band_offset, group_off, x_drop < 0 ? -1 : x_drop,
-                       half_band, score, query_end, target_end);
+    const int thr = x_drop < 0 ? -1 : x_drop;
+    if (max_group <= 64) {
+        hipLaunchKernelGGL(sw_xdrop_kernel, dim3(n_groups), dim3(64), 0, (hipStream_t)stream,
+                           rowkey, rowkey_off, qlen, band_offset, group_off, thr, half_band, score,
+                           query_end, target_end);
+    } else {
+        GMFA_REQUIRE(scratch, "groups above 64 pairs need scratch (4 int32 per pair)");
+        hipLaunchKernelGGL(sw_xdrop_wide_kernel, dim3(n_groups), dim3(256), 0,
+                           (hipStream_t)stream, rowkey, rowkey_off, qlen, band_offset, group_off,
+                           thr, half_band, scratch, score, query_end, target_end);
+    }
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }

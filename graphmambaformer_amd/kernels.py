@@ -371,8 +371,8 @@ def decode_cigars(tbk: SwTraceback) -> list[list[tuple[str, int]]]:
 
 def sw_xdrop(fwd: SwForward, inp: SwInputs, group_off: np.ndarray, x_drop: float, hb: int):
     """Batch-wide x-drop on the device, one reference batch per group
-    (pairs group_off[g] .. group_off[g+1]-1, at most 64 each).  Returns device
-    int32 (score, query_end, target_end)."""
+    (pairs group_off[g] .. group_off[g+1]-1).  Returns device int32 (score,
+    query_end, target_end)."""
     if fwd.rowkey is None:
         raise ValueError("x-drop needs the forward pass with rowkeys=True")
     go = np.asarray(group_off, dtype=np.int64)
@@ -388,60 +388,13 @@ def sw_xdrop(fwd: SwForward, inp: SwInputs, group_off: np.ndarray, x_drop: float
     if ng == 0 or n == 0:
         return score, qe, te
     mg = int(sizes.max())
-    if mg > 64:
-        raise ValueError("x-drop groups hold at most 64 pairs")
     thr = int(np.floor(x_drop)) if x_drop > 0 else -1
     rk_off = torch.from_numpy(fwd.rowkey_off_host[:-1].astype(np.int64)).to(dev)
     god = torch.from_numpy(go.astype(np.int32)).to(dev)
+    scratch = torch.empty(4 * n, dtype=torch.int32, device=dev) if mg > 64 else None
     check(lib().gmfa_sw_xdrop(ptr(fwd.rowkey), ptr(rk_off), ptr(inp.qlen), ptr(inp.band_offset),
-                              ptr(god), ng, max(mg, 1), thr, int(hb), ptr(score), ptr(qe), ptr(te),
-                              stream_of(dev)), "gmfa_sw_xdrop")
-    return score, qe, te
-
-
-def xdrop_best(fwd: SwForward, qlen_host: np.ndarray, x_drop: float, hb: int,
-               boff_host: np.ndarray, group_off: Optional[np.ndarray] = None):
-    """Batch-wide x-drop (extension.py:186-193) from per-row keys, on the host.
-
-    The forward pass computed every row of every pair; the reference stops a
-    batch after the first row i where every pair has
-    row_max_seen - row_best > x_drop, so the best cell is the first strict
-    maximum over rows <= i.  ``group_off``: batches as pair ranges (default:
-    all pairs are one batch).  Returns host arrays (score, qend, tend)."""
-    n = len(qlen_host)
-    keys = fwd.rowkey.cpu().numpy()
-    offs = fwd.rowkey_off_host
-    go = np.array([0, n], np.int64) if group_off is None else np.asarray(group_off, np.int64)
-    score = np.zeros(n, dtype=np.int32)
-    qe = np.zeros(n, dtype=np.int32)
-    te = np.zeros(n, dtype=np.int32)
-    for g in range(len(go) - 1):
-        ps = range(int(go[g]), int(go[g + 1]))
-        maxq = max((int(qlen_host[p]) for p in ps), default=0)
-        if maxq == 0:
-            continue
-        rb = np.zeros((len(ps), maxq), dtype=np.int64)
-        rd = np.zeros((len(ps), maxq), dtype=np.int64)
-        for x, p in enumerate(ps):
-            q = int(qlen_host[p])
-            k = keys[offs[p]:offs[p] + q].astype(np.int64)
-            rb[x, :q] = k >> 11
-            rd[x, :q] = 2047 - (k & 2047)
-        stop = maxq
-        if x_drop > 0:
-            thr = int(np.floor(x_drop))
-            seen = np.maximum.accumulate(rb, axis=1)
-            hit = np.flatnonzero(((seen - rb) > thr).all(axis=0))
-            if len(hit):
-                stop = int(hit[0]) + 1
-        for x, p in enumerate(ps):
-            r = rb[x, :stop]
-            if r.max() <= 0:
-                continue
-            i = int(np.argmax(r))  # first row reaching the max = first strict improvement
-            score[p] = int(r[i])
-            qe[p] = i + 1
-            te[p] = i + 1 + int(boff_host[p]) - hb + int(rd[x, i])
+                              ptr(god), ng, max(mg, 1), thr, int(hb), ptr(scratch), ptr(score),
+                              ptr(qe), ptr(te), stream_of(dev)), "gmfa_sw_xdrop")
     return score, qe, te
 
 

@@ -165,16 +165,17 @@ int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
 /* Batch-wide x-drop (graphmambaformer/alignment/extension.py:186-193) per
  * group of pairs, from the per-row keys of a GMFA_SW_ROWKEYS forward pass.
  * Group g = pairs group_off[g] .. group_off[g+1]-1 (device int32 array, at
- * most max_group <= 64 pairs per group), one group = one reference batch:
+ * most max_group pairs per group), one group = one reference batch:
  * after the first row where every pair of the group has best-so-far - row
  * best > x_drop, the group stops; each pair's result is then its first strict
  * maximum over the rows up to that one.  x_drop = floor of the reference's
- * x_drop, or -1 when the reference's x_drop <= 0 (no stop).  Writes score,
- * 1-based query_end / target_end (0 when the score is 0). */
+ * x_drop, or -1 when the reference's x_drop <= 0 (no stop).  Groups above 64
+ * pairs need scratch (4 int32 per pair; NULL is fine otherwise).  Writes
+ * score, 1-based query_end / target_end (0 when the score is 0). */
 int gmfa_sw_xdrop(const int32_t *rowkey, const int64_t *rowkey_off, const int32_t *qlen,
                   const int32_t *band_offset, const int32_t *group_off, int32_t n_groups,
-                  int32_t max_group, int32_t x_drop, int32_t half_band, int32_t *score,
-                  int32_t *query_end, int32_t *target_end, void *stream);
+                  int32_t max_group, int32_t x_drop, int32_t half_band, int32_t *scratch,
+                  int32_t *score, int32_t *query_end, int32_t *target_end, void *stream);
 
 /* Walk the traceback planes from (query_end, target_end) of every pair and
  * write BAM-style CIGAR runs (len << 4 | op, forward order) at

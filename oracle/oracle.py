@@ -382,3 +382,48 @@ def extend_chains(read_seq, chains, anchor_diagonals, ref_seq, half_band=64,
         e["cigar"] = merged
         out.append(e)
     return out
+
+
+def xdrop_from_rowkeys(keys: np.ndarray, offs: np.ndarray, qlen_host: np.ndarray, x_drop: float,
+                       hb: int, boff_host: np.ndarray, group_off=None):
+    """Batch-wide x-drop (extension.py:186-193) from the forward kernel's
+    per-row keys (key = H << 11 | 2047 - d, qlen keys per pair at offs[p]).
+
+    The forward pass computed every row of every pair; the reference stops a
+    batch after the first row i where every pair has
+    row_max_seen - row_best > x_drop, so the best cell is the first strict
+    maximum over rows <= i.  ``group_off``: batches as pair ranges (default:
+    all pairs are one batch).  Returns host arrays (score, qend, tend)."""
+    n = len(qlen_host)
+    go = np.array([0, n], np.int64) if group_off is None else np.asarray(group_off, np.int64)
+    score = np.zeros(n, dtype=np.int32)
+    qe = np.zeros(n, dtype=np.int32)
+    te = np.zeros(n, dtype=np.int32)
+    for g in range(len(go) - 1):
+        ps = range(int(go[g]), int(go[g + 1]))
+        maxq = max((int(qlen_host[p]) for p in ps), default=0)
+        if maxq == 0:
+            continue
+        rb = np.zeros((len(ps), maxq), dtype=np.int64)
+        rd = np.zeros((len(ps), maxq), dtype=np.int64)
+        for x, p in enumerate(ps):
+            q = int(qlen_host[p])
+            k = keys[offs[p]:offs[p] + q].astype(np.int64)
+            rb[x, :q] = k >> 11
+            rd[x, :q] = 2047 - (k & 2047)
+        stop = maxq
+        if x_drop > 0:
+            thr = int(np.floor(x_drop))
+            seen = np.maximum.accumulate(rb, axis=1)
+            hit = np.flatnonzero(((seen - rb) > thr).all(axis=0))
+            if len(hit):
+                stop = int(hit[0]) + 1
+        for x, p in enumerate(ps):
+            r = rb[x, :stop]
+            if r.max() <= 0:
+                continue
+            i = int(np.argmax(r))  # first row reaching the max = first strict improvement
+            score[p] = int(r[i])
+            qe[p] = i + 1
+            te[p] = i + 1 + int(boff_host[p]) - hb + int(rd[x, i])
+    return score, qe, te

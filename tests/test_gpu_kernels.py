@@ -179,13 +179,14 @@ def test_golden_sw_ties_with_batch_xdrop():
         inp = k.upload_pairs(qs, ts, case["off"], DEV, alphabet="ref")
         hb = case["half_band"]
         fwd = k.sw_forward(inp, ma, mi, go, ge, hb, traceback=True, rowkeys=True)
-        sc, qe, te = k.xdrop_best(fwd, inp.qlen_host, case["x_drop"], hb, inp.boff_host)
+        sc, qe, te = O.xdrop_from_rowkeys(fwd.rowkey.cpu().numpy(), fwd.rowkey_off_host,
+                                          inp.qlen_host, case["x_drop"], hb, inp.boff_host)
         dev = inp.device
-        if inp.n_pairs <= 64:  # the device x-drop agrees with the host one
-            dsc, dqe, dte = k.sw_xdrop(fwd, inp, np.array([0, inp.n_pairs]), case["x_drop"], hb)
-            assert np.array_equal(dsc.cpu().numpy(), sc), n
-            assert np.array_equal(dqe.cpu().numpy(), qe), n
-            assert np.array_equal(dte.cpu().numpy(), te), n
+        # the device x-drop (the product path) agrees with the host restatement
+        dsc, dqe, dte = k.sw_xdrop(fwd, inp, np.array([0, inp.n_pairs]), case["x_drop"], hb)
+        assert np.array_equal(dsc.cpu().numpy(), sc), n
+        assert np.array_equal(dqe.cpu().numpy(), qe), n
+        assert np.array_equal(dte.cpu().numpy(), te), n
         tbk = k.sw_traceback(inp, fwd, torch.from_numpy(sc).to(dev), torch.from_numpy(qe).to(dev),
                              torch.from_numpy(te).to(dev))
         cig = k.decode_cigars(tbk)
@@ -264,3 +265,37 @@ def test_sw_edge_cases_vs_oracle():
         for fused in (False, True):
             got = _gpu_align(qs, ts, offs, hb, (2, 4, 4, 2), fused=fused)
             _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"edge{hb}")
+
+
+def test_xdrop_wide_groups_vs_oracle():
+    """Groups above 64 pairs (the 256-thread x-drop kernel) and mixed group
+    sizes in one call, against the oracle's x-drop on the same pairs."""
+    k = K()
+    rng = np.random.default_rng(41)
+    qs, ts, offs = [], [], []
+    for p in range(300):
+        q, t = _random_pair(rng, int(rng.integers(20, 160)))
+        qs.append(q)
+        ts.append(t)
+        offs.append(int(rng.integers(0, 20)))
+    inp = k.upload_pairs(qs, ts, offs, DEV, alphabet="ref")
+    hb = 24
+    fwd = k.sw_forward(inp, 2, 4, 4, 2, hb, traceback=True, rowkeys=True)
+    for groups in ([0, 300], [0, 10, 200, 201, 300], [0, 64, 65, 300]):
+        for xd in (6.0, 30.0, 0.0):
+            _say("case: x-drop groups", groups, xd)
+            go = np.array(groups)
+            sc, qe, te = k.sw_xdrop(fwd, inp, go, xd, hb)
+            for a, b in zip(go[:-1], go[1:]):
+                M = max(len(x) for x in qs[a:b])
+                N = max(len(x) for x in ts[a:b])
+                Q = np.full((b - a, M), -1, np.int16)
+                T = np.full((b - a, N), -1, np.int16)
+                for x in range(a, b):
+                    Q[x - a, :len(qs[x])] = qs[x]
+                    T[x - a, :len(ts[x])] = ts[x]
+                res = O.sw_banded(Q, T, [len(x) for x in qs[a:b]], [len(x) for x in ts[a:b]],
+                                  offs[a:b], hb, 2, 4, 4, 2, x_drop=xd)
+                assert np.array_equal(sc.cpu().numpy()[a:b], res["score"]), (groups, xd)
+                assert np.array_equal(qe.cpu().numpy()[a:b], res["query_end"]), (groups, xd)
+                assert np.array_equal(te.cpu().numpy()[a:b], res["target_end"]), (groups, xd)
