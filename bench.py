@@ -78,13 +78,13 @@ class BatchRun:
     order, tile and every buffer preallocated; ``step()`` is one forward
     launch (+ the walk, fused or as a second launch)."""
 
-    def __init__(self, torch, K, w, dev, fused=True):
+    def __init__(self, torch, K, w, dev, fused=True, tile=None):
         self.w, self.K, self.fused = w, K, fused
         self.cells = w.cells()
         self.total_cells = int(self.cells.sum())
         self.inp = K.upload_pairs(w.queries, w.targets, w.band_offset, dev, alphabet="2bit")
         self.order = torch.from_numpy(lpt_order(self.cells)).to(dev)
-        self.tile = K.resolve_tile(w.half_band)
+        self.tile = tile or K.resolve_tile(w.half_band)
         self.ws = K.make_workspace(self.inp, self.tile, cigar=True)
         torch.cuda.synchronize()
 
@@ -380,7 +380,16 @@ def main():
     log(f"[bench] sw_long generated: {w.n_pairs} pairs, {w.cells().sum():.3e} cells, "
         f"{time.time() - t0:.1f}s")
     fused = args.walk == "fused"
-    run = BatchRun(torch, K, w, dev, fused)
+    # tile autotuning on every 16th pair (by cell count) of the batch
+    sub_t = WL.subsample(w, np.argsort(w.cells(), kind="stable")[::16])
+    inp_t = K.upload_pairs(sub_t.queries, sub_t.targets, sub_t.band_offset, dev, alphabet="2bit")
+    tuned, tune_ms = K.autotune_tile(inp_t, w.half_band, scoring(w),
+                                     order=torch.from_numpy(lpt_order(sub_t.cells())).to(dev))
+    del inp_t
+    torch.cuda.empty_cache()
+    log(f"[bench] autotune (ms per sample launch): {tune_ms} -> "
+        f"{tuned.cells_per_lane}x{tuned.lanes_per_pair}")
+    run = BatchRun(torch, K, w, dev, fused, tile=tuned)
     total_cells, cells, tile = run.total_cells, run.cells, run.tile
     log(f"[bench] resident: tb planes {run.ws['tb_bytes'] / 1e9:.1f} GB, tile "
         f"{tile.cells_per_lane}x{tile.lanes_per_pair}")
@@ -446,6 +455,7 @@ def main():
             "cells": total_cells,
             "half_band": w.half_band,
             "tile": [tile.cells_per_lane, tile.lanes_per_pair],
+            "autotune_ms": {f"{c}x{g}": v for (c, g), v in tune_ms.items()},
             "step": "sw_forward (scores, ends, 4-bit planes) + CIGAR walk of every pair "
                     f"({args.walk})",
         },
@@ -545,8 +555,14 @@ def main():
     for lo, hi in zip(edges[:-1], edges[1:]):
         idx = np.flatnonzero((lens >= lo) & (lens < hi))
         if len(idx):
+            rb = BatchRun(torch, K, WL.subsample(wn, idx), dev, True)
+            rb.step()
+            bms = time_events(torch, rb.step, 2)
             bins.append({"len": [int(lo), int(hi)], "pairs": int(len(idx)),
-                         "cells": int(ncells[idx].sum())})
+                         "cells": int(ncells[idx].sum()), "ms": round(bms, 3),
+                         "gcups": round(float(ncells[idx].sum()) / (bms / 1e3) / 1e9, 2)})
+            del rb
+    torch.cuda.empty_cache()
     tile_n = [rn.tile.cells_per_lane, rn.tile.lanes_per_pair]
     del rn
     torch.cuda.empty_cache()
@@ -567,6 +583,8 @@ def main():
                       f"{int(ncells[subn].sum()):.3e} cells), {csn:.1f} s",
         "tile": tile_n,
         "length_bins": bins,
+        "length_bins_note": "each bin of 8 log-spaced length bins run as its own batch "
+                            "(one launch, fused walk)",
         "workload": "8192 pairs, lognormal(8 kb, 1.0) clipped 0.5-100 kb, 6% errors "
                     "(homopolymer-weighted indels), half_band 751, seed 3, fused walk",
     }

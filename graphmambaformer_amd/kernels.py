@@ -632,3 +632,34 @@ def cigar_bam(cigar: torch.Tensor, cigar_off: torch.Tensor, n_cigar: torch.Tenso
                                *[ptr(t) for t in args], ptr(out), ptr(d_off), ptr(out_n),
                                ptr(flag), stream_of(dev)), "gmfa_cigar_bam")
     return out, d_off, out_n[:n], flag[:n]
+
+
+def autotune_tile(inp: SwInputs, half_band: int, scoring: tuple, reps: int = 2,
+                  order: Optional[torch.Tensor] = None):
+    """Time every compiled tile that holds the band (the exact/specialised
+    variant where one exists) on ``inp`` with the fused walk; returns the
+    fastest tile and {(cells_per_lane, lanes_per_pair): ms}."""
+    W = 2 * int(half_band) + 1
+    timings = {}
+    best, best_ms = None, None
+    for C, G in list_tiles():
+        if C * G < W:
+            continue
+        tile = resolve_tile(half_band, C, G)
+        ws = make_workspace(inp, tile, cigar=True)
+        sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
+                   workspace=ws, cigar=True)
+        s = torch.cuda.current_stream(inp.device)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(s)
+        for _ in range(reps):
+            sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
+                       workspace=ws, cigar=True)
+        ev[1].record(s)
+        torch.cuda.synchronize(inp.device)
+        ms = ev[0].elapsed_time(ev[1]) / reps
+        timings[(C, G)] = round(ms, 4)
+        if best_ms is None or ms < best_ms:
+            best, best_ms = tile, ms
+        del ws
+    return best, timings
