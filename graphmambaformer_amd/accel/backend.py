@@ -14,6 +14,6 @@ def kernel_backend(stage: str, stage_backends: dict | None = None) -> str:
     override = (stage_backends or {}).get(stage)
     if override and override != "auto":
         return override
-    if stage in ("chaining", "extension") and kernels_available():
+    if stage in ("seeding", "chaining", "extension") and kernels_available():
         return "hip"
     return "torch"
