@@ -140,7 +140,7 @@ def cpu_gcups(w, idx, threads):
     return float(w.cells()[idx].sum()) / s / 1e9, s
 
 
-SERVE_SLOTS = (512, 2048, 1024)  # persistent wavefronts of the short / long / noisy classes
+SERVE_SLOTS = (512, 1024, 2048)  # persistent wavefronts of the short / long / noisy classes
 
 
 def serving_extra(torch, WL, dev, n_req, load):
