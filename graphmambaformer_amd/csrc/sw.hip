@@ -797,6 +797,41 @@ __global__ __launch_bounds__(64) void sw_xdrop_kernel(const int32_t *rowkey, con
 // -------------------------------------------------------------------------
 // int16 codes -> 2-bit stream + N mask; one thread per output word
 // -------------------------------------------------------------------------
+// A flat byte stream of codes -> 2-bit stream + N mask, one thread per output
+// word.  two_bit: codes 0..3 (any other value is rejected); else reference codes
+// (1..4 = ACGT, 5 = N -> mask, 0 or above 5 rejected).  flags bit 0: a rejected
+// code, bit 1: an N.
+__global__ void pack_flat_kernel(const uint8_t *codes, int64_t n, int32_t two_bit, uint32_t *seq,
+                                 uint32_t *mask, int32_t *flags)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nw = (n + 15) >> 4;
+    if (w >= nw) return;
+    uint32_t sv = 0u, mv = 0u;
+    int f = 0;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+        const int64_t e = w * 16 + x;
+        if (e >= n) {
+            mv |= 1u << (2 * x);  // padding never matches
+            continue;
+        }
+        const int c = codes[e];
+        if (two_bit) {
+            if (c > 3) f |= 1;
+            sv |= (uint32_t)(c & 3) << (2 * x);
+        } else if (c >= 1 && c <= 4) {
+            sv |= (uint32_t)(c - 1) << (2 * x);
+        } else {
+            mv |= 1u << (2 * x);
+            f |= (c == 5) ? 2 : 1;
+        }
+    }
+    seq[w] = sv;
+    mask[w] = mv;
+    if (f) atomicOr(flags, f);
+}
+
 __global__ void pack_codes_kernel(const int16_t *codes, int64_t L, const int32_t *lens,
                                   const int64_t *word_off, uint32_t *seq, uint32_t *mask,
                                   int32_t *bad)
@@ -969,6 +1004,19 @@ extern "C" int64_t gmfa_sw_tb_bytes(int32_t qlen, const gmfa_sw_tile *tile)
     if (!tile || qlen < 0) return GMFA_E_INVALID;
     const int64_t nblk = ((int64_t)qlen + 7) / 8;
     return nblk * tile->lanes_stored * tile->cells_per_lane * 4;
+}
+
+extern "C" int gmfa_pack_flat(const uint8_t *codes, int64_t n, int32_t two_bit, uint32_t *seq,
+                              uint32_t *mask, int32_t *flags, void *stream)
+{
+    GMFA_REQUIRE(n >= 0, "negative length");
+    if (n == 0) return GMFA_OK;
+    GMFA_REQUIRE(codes && seq && mask && flags, "NULL pointer");
+    const int64_t nw = (n + 15) / 16;
+    hipLaunchKernelGGL(pack_flat_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, codes, n, two_bit, seq, mask, flags);
+    GMFA_CHECK_HIP(hipGetLastError());
+    return GMFA_OK;
 }
 
 extern "C" int gmfa_pack_codes(const int16_t *codes, int64_t B, int64_t L, const int32_t *lens,

@@ -111,9 +111,58 @@ class SwInputs:
         return b
 
 
+def pack_stream(seqs: Sequence[np.ndarray], device, alphabet: str = "ref"):
+    """Sequences back to back (no word alignment) as one byte stream, copied
+    once and packed on the device (gmfa_pack_flat).  Returns (words, masks,
+    base offsets int64, has_n); codes as in pack_host, negative codes are not
+    accepted here."""
+    dev = torch.device(device)
+    lens = np.array([len(x) for x in seqs], dtype=np.int64)
+    offs = np.zeros(len(seqs) + 1, dtype=np.int64)
+    np.cumsum(lens, out=offs[1:])
+    n = int(offs[-1])
+    flat = np.concatenate([np.asarray(x).astype(np.uint8, copy=False) for x in seqs]) \
+        if n else np.zeros(0, np.uint8)
+    nw = max((n + 15) // 16, 1)
+    seq = torch.zeros(nw, dtype=torch.int32, device=dev)
+    mask = torch.full((nw,), -1, dtype=torch.int32, device=dev)
+    flags = torch.zeros(1, dtype=torch.int32, device=dev)
+    if n:
+        codes = torch.from_numpy(flat).to(dev)
+        check(lib().gmfa_pack_flat(ptr(codes), n, 1 if alphabet == "2bit" else 0, ptr(seq),
+                                   ptr(mask), ptr(flags), stream_of(dev)), "gmfa_pack_flat")
+    f = int(flags.item())
+    if f & 1:
+        raise ValueError("base codes must be 1..4 (ACGT) or 5 (N)" if alphabet != "2bit" else
+                         "2-bit codes must be 0..3")
+    return seq, mask, offs[:-1], bool(f & 2)
+
+
 def upload_pairs(queries: Sequence[np.ndarray], targets: Sequence[np.ndarray], band_offset,
                  device, alphabet: str = "ref") -> SwInputs:
-    """Pack on the host and copy once to the device."""
+    """Copy a batch once to the device and pack it there (queries and targets
+    each one stream, back to back)."""
+    ql = np.array([len(q) for q in queries], dtype=np.int32)
+    tl = np.array([len(t) for t in targets], dtype=np.int32)
+    neg = any(len(x) and np.asarray(x).min() < 0 for x in list(queries) + list(targets))
+    if neg:  # negative padding codes inside a sequence: the host packer masks them
+        return _upload_pairs_host(queries, targets, band_offset, device, alphabet)
+    dev = torch.device(device)
+    qseq, qmask, qo, qn = pack_stream(queries, dev, alphabet)
+    tseq, tmask, to, tn = pack_stream(targets, dev, alphabet)
+    has_n = qn or tn
+    bo = np.asarray(band_offset, dtype=np.int32).reshape(len(queries))
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    return SwInputs(qseq=qseq, tseq=tseq, qmask=qmask if has_n else None,
+                    tmask=tmask if has_n else None, q_off=up(qo), t_off=up(to), qlen=up(ql),
+                    tlen=up(tl), band_offset=up(bo), qlen_host=ql, tlen_host=tl, boff_host=bo,
+                    has_n=has_n)
+
+
+def _upload_pairs_host(queries, targets, band_offset, device, alphabet):
     qw, qm, qo, qn = pack_host(queries, alphabet)
     tw, tm, to, tn = pack_host(targets, alphabet)
     has_n = qn or tn

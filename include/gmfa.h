@@ -158,6 +158,13 @@ int gmfa_pack_codes(const int16_t *codes, int64_t B, int64_t L, const int32_t *l
                     const int64_t *word_off, int64_t total_words, uint32_t *seq,
                     uint32_t *mask, int32_t *bad_codes, void *stream);
 
+/* Pack a flat stream of n byte codes (sequences back to back) into the 2-bit
+ * stream + N mask (words of 16 bases; padding past n is masked).  two_bit: codes
+ * 0..3; else reference codes (1..4, 5 = N).  *flags (zeroed by the caller): bit
+ * 0 = a code outside the alphabet, bit 1 = an N. */
+int gmfa_pack_flat(const uint8_t *codes, int64_t n, int32_t two_bit, uint32_t *seq,
+                   uint32_t *mask, int32_t *flags, void *stream);
+
 /* Forward pass: scores, ends and (optionally) traceback planes / row keys. */
 int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
                     const gmfa_sw_out *out, void *stream);
