@@ -108,3 +108,42 @@ def test_serve_run_validates_before_touching_the_device():
     assert b"nondecreasing" in L.gmfa_last_error()
     with pytest.raises(ValueError):
         _lib.check(L.gmfa_serve_run(arr, 1, 1.0, ctypes.byref(rep)))
+
+
+def test_entry_points_reject_bad_arguments_without_a_device():
+    """Every entry point validates its arguments before any HIP call, so bad
+    input is a ValueError (GMFA_E_INVALID) even on a host without a GPU."""
+    import ctypes
+
+    from graphmambaformer_amd import _lib
+
+    L = _lib.lib()
+    E = _lib.GMFA_E_INVALID
+    ix = _lib.KmerIndex()
+    assert L.gmfa_kmer_lookup(None, None, 1, None, None, None) == E
+    ix.k, ix.dir_bits = 0, 0
+    assert L.gmfa_kmer_lookup(ctypes.byref(ix), None, 1, None, None, None) == E
+    ix.k, ix.dir_bits = 15, 40  # dir_bits above 2k
+    assert L.gmfa_kmer_lookup(ctypes.byref(ix), None, 1, None, None, None) == E
+    ix.dir_bits = 8
+    rd = _lib.KmerReads()
+    rd.window = 0
+    assert L.gmfa_kmer_seed(ctypes.byref(ix), ctypes.byref(rd), None, None, None, None, None,
+                            None) == E
+    assert L.gmfa_wfa_distance(None, 10, 10, None, None, None) == E
+    b = _lib.SwBatch()
+    b.n_pairs = 1  # pointers missing
+    assert L.gmfa_wfa_distance(ctypes.byref(b), 10, 10, None, None, None) == E
+    assert L.gmfa_wfa_align(ctypes.byref(b), None, None, None, None, None, None, None, None,
+                            None) == E
+    assert L.gmfa_chain_sort(None, -1, 0, *([None] * 13), None) == E
+    assert L.gmfa_chain_backtrack_device(None, 1, -5, *([None] * 7), 1, 0.0, *([None] * 8),
+                                         None) == E
+    assert L.gmfa_chain_select(1, *([None] * 5), 0.6, 0.5, 0, *([None] * 4), None) == E
+    assert L.gmfa_cigar_bam(-1, *([None] * 13), None) == E
+    assert L.gmfa_pack_flat(None, -1, 0, None, None, None, None) == E
+    assert L.gmfa_sw_xdrop(None, None, None, None, None, 1, 0, 0, 10, None, None, None, None,
+                           None) == E
+    assert L.gmfa_sw_xdrop(None, None, None, None, None, 1, 4, 0, 5000, None, None, None, None,
+                           None) == E
+    assert b"half_band" in L.gmfa_last_error()
