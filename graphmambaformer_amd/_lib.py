@@ -20,6 +20,8 @@ GMFA_OK = 0
 GMFA_E_INVALID = -1
 GMFA_E_RUNTIME = -2
 GMFA_E_UNSUPPORTED = -3
+GMFA_E_AGAIN = -4
+GMFA_SERVE_TRUNCATED = 1
 
 SW_TRACEBACK = 1
 SW_NMASK = 2
@@ -81,6 +83,24 @@ class ServeReport(ctypes.Structure):
                 ("waves_gave_up", _i32)]
 
 
+class ServeClassCfg(ctypes.Structure):
+    _fields_ = [("params", SwParams), ("max_qlen", _i32), ("ring", _i32), ("n_slots", _i32),
+                ("cigar_cap", _i32), ("tb", _P), ("slab_words", _i64), ("stage", _P),
+                ("head", _P)]
+
+
+class ServeResultRec(ctypes.Structure):
+    _fields_ = [("ticket", _i64), ("cls", _i32), ("status", _i32), ("score", _i32),
+                ("query_end", _i32), ("target_end", _i32), ("read_start", _i32),
+                ("ref_start", _i32), ("n_cigar", _i32), ("counts", _i32 * 4),
+                ("cigar_off", _i64)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("submitted", _i64), ("returned", _i64), ("in_flight", _i64),
+                ("launches", _i64)]
+
+
 # Every exported symbol with its ctypes signature (must match include/gmfa.h).
 SIGNATURES = {
     "gmfa_last_error": (ctypes.c_char_p, []),
@@ -117,6 +137,15 @@ SIGNATURES = {
                                       _P, _P, _P, _P, _P, _P]),
     "gmfa_serve_run": (ctypes.c_int, [ctypes.POINTER(ServeClass), _i32, ctypes.c_double,
                                       ctypes.POINTER(ServeReport)]),
+    "gmfa_serve_stage_words": (_i32, [_i32]),
+    "gmfa_serve_open": (ctypes.c_int, [_P, _P, _i64, ctypes.POINTER(ServeClassCfg), _i32,
+                                       ctypes.c_double, ctypes.POINTER(_P)]),
+    "gmfa_serve_submit": (ctypes.c_int, [_P, _i32, _P, _i32, _i32, _i64, _i32, _i32,
+                                         ctypes.POINTER(_i64)]),
+    "gmfa_serve_poll": (ctypes.c_int, [_P, ctypes.POINTER(ServeResultRec), _i32, _P, _i64,
+                                       ctypes.POINTER(_i32)]),
+    "gmfa_serve_session_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServeStats)]),
+    "gmfa_serve_close": (ctypes.c_int, [_P, ctypes.POINTER(ServeReport)]),
 }
 
 _lib = None

@@ -119,7 +119,7 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
             set_error("hipStreamCreateWithFlags failed");
             return finish(GMFA_E_RUNTIME);
         }
-        ServeLaunch sl;
+        ServeLaunch sl{};
         sl.tail = &ctl_d[c].tail.v;
         sl.closed = &ctl_d[c].closed.v;
         sl.done = done_d + flag_off[c];
@@ -176,5 +176,418 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
                   timeout_s, (long long)report->completed, (long long)total);
         return GMFA_E_RUNTIME;
     }
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Online sessions (gmfa_serve_open / submit / poll / close)
+// ---------------------------------------------------------------------------
+namespace gmfa {
+namespace {
+
+struct alignas(128) CtlWord64 {
+    int64_t v;
+    int64_t pad[15];
+};
+
+struct OnlineCtl {
+    CtlWord64 tail;
+    CtlWord closed, gave_up;
+};
+
+struct OnlineClass {
+    gmfa_serve_class_cfg cfg{};
+    int32_t stage_words = 0;
+    bool nm = false;
+    hipStream_t stream = nullptr;
+    void *host = nullptr;  // one host-coherent block per class
+    OnlineCtl *ctl = nullptr;
+    ServeDesc *desc = nullptr;
+    uint32_t *q = nullptr;
+    ServeRes *res = nullptr;
+    uint32_t *cig = nullptr;
+    int64_t *done = nullptr;
+    ServeLaunch sl{};
+    gmfa_sw_batch batch{};
+    gmfa_sw_out out{};  // only tb: the outputs go to the ring
+    int64_t next = 0;                // next ticket
+    int64_t lo = 0;                  // oldest ticket not yet returned
+    std::vector<int64_t> occupant;   // ticket held by each ring entry, -1 when free
+    std::chrono::steady_clock::time_point last_query{};
+};
+
+template <class T>
+T *dev_ptr(void *host_base, void *dev_base, T *p)
+{
+    return (T *)((char *)dev_base + ((char *)p - (char *)host_base));
+}
+
+}  // namespace
+}  // namespace gmfa
+
+struct gmfa_serve_session {
+    std::vector<gmfa::OnlineClass> cls;
+    const uint32_t *ref_seq = nullptr, *ref_mask = nullptr;
+    int64_t ref_bases = 0;
+    uint64_t idle_ticks = 0;
+    int device = 0;
+    int64_t submitted = 0, returned = 0, completed = 0, launches = 0;
+};
+
+namespace gmfa {
+namespace {
+
+__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
+
+int launch_class(gmfa_serve_session *s, OnlineClass &c)
+{
+    const int rc = sw_launch(&c.batch, &c.cfg.params, &c.out, &c.sl, c.stream);
+    if (rc == GMFA_OK) ++s->launches;
+    return rc;
+}
+
+// Relaunch a class whose wavefronts have all exited (idle) while requests of
+// it are still pending.  The stream query is rate-limited (it takes the
+// runtime's stream lock) unless forced: a class that was empty until now
+// (its waves may have idled out) or a closing session.
+int ensure_running(gmfa_serve_session *s, OnlineClass &c, bool force)
+{
+    if (c.lo == c.next) return GMFA_OK;
+    const auto now = std::chrono::steady_clock::now();
+    if (!force && now - c.last_query < std::chrono::microseconds(200)) return GMFA_OK;
+    c.last_query = now;
+    const hipError_t q = hipStreamQuery(c.stream);
+    if (q == hipErrorNotReady) return GMFA_OK;
+    if (q != hipSuccess) {
+        set_error("serving class kernel failed: %s", hipGetErrorString(q));
+        return GMFA_E_RUNTIME;
+    }
+    const int32_t ring = c.cfg.ring;
+    bool pending = false;
+    for (int64_t t = c.lo; t < c.next && !pending; ++t) {
+        const int r = (int)(t % ring);
+        if (c.occupant[r] == t && __atomic_load_n(&c.done[r], __ATOMIC_ACQUIRE) != t + 1)
+            pending = true;
+    }
+    return pending ? launch_class(s, c) : GMFA_OK;
+}
+
+void free_session(gmfa_serve_session *s)
+{
+    for (auto &c : s->cls) {
+        if (c.ctl) __atomic_store_n(&c.ctl->closed.v, 1, __ATOMIC_RELEASE);
+    }
+    for (auto &c : s->cls) {
+        if (c.stream) {
+            (void)hipStreamSynchronize(c.stream);
+            (void)hipStreamDestroy(c.stream);
+            c.stream = nullptr;
+        }
+        if (c.host) {
+            (void)hipHostFree(c.host);
+            c.host = nullptr;
+        }
+    }
+    delete s;
+}
+
+}  // namespace
+}  // namespace gmfa
+
+extern "C" int32_t gmfa_serve_stage_words(int32_t max_qlen)
+{
+    return max_qlen < 0 ? 0 : serve_stage_words(max_qlen);
+}
+
+extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask,
+                               int64_t ref_bases, const gmfa_serve_class_cfg *classes,
+                               int32_t n_classes, double idle_timeout_s,
+                               gmfa_serve_session **session)
+{
+    GMFA_REQUIRE(session, "NULL session pointer");
+    *session = nullptr;
+    GMFA_REQUIRE(ref_seq && classes, "NULL argument");
+    GMFA_REQUIRE(ref_bases >= 1, "empty reference");
+    GMFA_REQUIRE(n_classes >= 1 && n_classes <= 8, "n_classes %d outside [1, 8]", n_classes);
+    GMFA_REQUIRE(idle_timeout_s > 0.0 && idle_timeout_s <= 3600.0,
+                 "idle_timeout_s must be in (0, 3600]");
+    for (int c = 0; c < n_classes; ++c) {
+        const gmfa_serve_class_cfg &k = classes[c];
+        GMFA_REQUIRE(k.max_qlen >= 0 && k.max_qlen <= (1 << 20),
+                     "class %d: max_qlen %d outside [0, 2^20]", c, k.max_qlen);
+        GMFA_REQUIRE(k.ring >= 1 && k.ring <= (1 << 20) && (k.ring & (k.ring - 1)) == 0,
+                     "class %d: ring %d is not a power of two in [1, 2^20]", c, k.ring);
+        GMFA_REQUIRE(k.n_slots >= 1, "class %d: n_slots must be positive", c);
+        GMFA_REQUIRE(k.cigar_cap >= 1 && k.cigar_cap <= (1 << 22),
+                     "class %d: cigar_cap %d outside [1, 2^22]", c, k.cigar_cap);
+        GMFA_REQUIRE(k.tb && k.stage && k.head, "class %d: NULL tb / stage / head", c);
+        GMFA_REQUIRE(!(k.params.flags & GMFA_SW_NMASK) || ref_mask,
+                     "class %d: GMFA_SW_NMASK needs ref_mask", c);
+    }
+    int dev = 0, khz = 0;
+    GMFA_CHECK_HIP(hipGetDevice(&dev));
+    GMFA_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    GMFA_REQUIRE(khz > 0, "device reports no wall-clock rate");
+    // the caller's uploads (reference, zeroed buffers) must be complete
+    GMFA_CHECK_HIP(hipDeviceSynchronize());
+
+    gmfa_serve_session *s = new gmfa_serve_session();
+    s->ref_seq = ref_seq;
+    s->ref_mask = ref_mask;
+    s->ref_bases = ref_bases;
+    s->idle_ticks = (uint64_t)(idle_timeout_s * (double)khz * 1000.0);
+    s->device = dev;
+    s->cls.resize(n_classes);
+    for (int ci = 0; ci < n_classes; ++ci) {
+        OnlineClass &c = s->cls[ci];
+        const gmfa_serve_class_cfg &k = classes[ci];
+        c.cfg = k;
+        c.nm = (k.params.flags & GMFA_SW_NMASK) != 0;
+        c.stage_words = serve_stage_words(k.max_qlen);
+        c.occupant.assign(k.ring, -1);
+        const size_t ring = (size_t)k.ring;
+        const size_t b_ctl = sizeof(OnlineCtl);
+        const size_t b_desc = sizeof(ServeDesc) * ring;
+        const size_t b_q = sizeof(uint32_t) * 2 * (size_t)c.stage_words * ring;
+        const size_t b_res = sizeof(ServeRes) * ring;
+        const size_t b_cig = (sizeof(uint32_t) * (size_t)k.cigar_cap * ring + 127) & ~(size_t)127;
+        const size_t b_done = sizeof(int64_t) * ring;
+        const size_t bytes = b_ctl + b_desc + b_q + b_res + b_cig + b_done;
+        void *host = nullptr, *devp = nullptr;
+        if (hipHostMalloc(&host, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+            gmfa::free_session(s);
+            set_error("hipHostMalloc of %zu bytes for the serving rings failed", bytes);
+            return GMFA_E_RUNTIME;
+        }
+        c.host = host;
+        std::memset(host, 0, bytes);
+        if (hipHostGetDevicePointer(&devp, host, 0) != hipSuccess) {
+            gmfa::free_session(s);
+            set_error("hipHostGetDevicePointer failed for the serving rings");
+            return GMFA_E_RUNTIME;
+        }
+        char *h = (char *)host;
+        c.ctl = (OnlineCtl *)h;
+        c.desc = (ServeDesc *)(h + b_ctl);
+        c.q = (uint32_t *)(h + b_ctl + b_desc);
+        c.res = (ServeRes *)(h + b_ctl + b_desc + b_q);
+        c.cig = (uint32_t *)(h + b_ctl + b_desc + b_q + b_res);
+        c.done = (int64_t *)(h + b_ctl + b_desc + b_q + b_res + b_cig);
+
+        c.batch = gmfa_sw_batch{};
+        c.batch.max_qlen = k.max_qlen;
+        c.batch.qseq = k.stage;
+        c.batch.qmask = c.nm ? k.stage + c.stage_words : nullptr;
+        c.batch.q_words = 2 * (int64_t)c.stage_words * k.n_slots;
+        c.batch.tseq = ref_seq;
+        c.batch.tmask = c.nm ? ref_mask : nullptr;
+        c.batch.t_words = (ref_bases + 15) / 16;
+
+        c.out = gmfa_sw_out{};
+        c.out.tb = k.tb;
+
+        ServeLaunch &sl = c.sl;
+        sl.tail = nullptr;
+        sl.closed = &dev_ptr(host, devp, c.ctl)->closed.v;
+        sl.done = nullptr;
+        sl.gave_up = &dev_ptr(host, devp, c.ctl)->gave_up.v;
+        sl.slab_words = k.slab_words;
+        sl.deadline_ticks = s->idle_ticks;
+        sl.n_slots = k.n_slots;
+        sl.ring_n = k.ring;
+        sl.ring_tail = &dev_ptr(host, devp, c.ctl)->tail.v;
+        sl.head = (unsigned long long *)k.head;
+        sl.ring_desc = dev_ptr(host, devp, c.desc);
+        sl.ring_q = dev_ptr(host, devp, c.q);
+        sl.ring_res = dev_ptr(host, devp, c.res);
+        sl.ring_cigar = dev_ptr(host, devp, c.cig);
+        sl.ring_done = dev_ptr(host, devp, c.done);
+        sl.stage = k.stage;
+        sl.stage_words = c.stage_words;
+        sl.cigar_cap = k.cigar_cap;
+
+        if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+            c.stream = nullptr;
+            gmfa::free_session(s);
+            set_error("hipStreamCreateWithFlags failed");
+            return GMFA_E_RUNTIME;
+        }
+        // the head starts at ticket 0 (a kernel, not a memset: no memset nodes
+        // on a stream a caller might capture)
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, c.stream,
+                           (unsigned long long *)k.head);
+        if (hipGetLastError() != hipSuccess) {
+            gmfa::free_session(s);
+            set_error("head reset launch failed");
+            return GMFA_E_RUNTIME;
+        }
+        const int rc = gmfa::launch_class(s, c);
+        if (rc != GMFA_OK) {
+            const std::string msg = gmfa_last_error();
+            gmfa::free_session(s);
+            set_error("%s", msg.c_str());
+            return rc;
+        }
+    }
+    *session = s;
+    return GMFA_OK;
+}
+
+extern "C" int gmfa_serve_submit(gmfa_serve_session *s, int32_t cls, const uint8_t *codes,
+                                 int32_t qlen, int32_t two_bit, int64_t t_off, int32_t tlen,
+                                 int32_t band_offset, int64_t *ticket)
+{
+    GMFA_REQUIRE(s && ticket, "NULL argument");
+    GMFA_REQUIRE(cls >= 0 && cls < (int)s->cls.size(), "class %d outside [0, %d)", cls,
+                 (int)s->cls.size());
+    OnlineClass &c = s->cls[cls];
+    GMFA_REQUIRE(qlen >= 0 && qlen <= c.cfg.max_qlen, "qlen %d outside [0, %d]", qlen,
+                 c.cfg.max_qlen);
+    GMFA_REQUIRE(qlen == 0 || codes, "NULL codes");
+    GMFA_REQUIRE(tlen >= 0 && t_off >= 0 && t_off + (int64_t)tlen <= s->ref_bases,
+                 "window [%lld, %lld) leaves the reference of %lld bases", (long long)t_off,
+                 (long long)(t_off + tlen), (long long)s->ref_bases);
+    GMFA_REQUIRE(band_offset >= -(1 << 30) && band_offset <= (1 << 30),
+                 "band_offset %d outside [-2^30, 2^30]", band_offset);
+    const int64_t t = c.next;
+    const bool was_empty = c.lo == c.next;
+    const int r = (int)(t % c.cfg.ring);
+    if (c.occupant[r] >= 0) {
+        set_error("class %d: ring of %d requests is full (poll, then submit again)", cls,
+                  c.cfg.ring);
+        return GMFA_E_AGAIN;
+    }
+    // pack into the entry: 2-bit words, then N-mask words
+    const int nw = (qlen + 15) / 16;
+    uint32_t *qw = c.q + (size_t)r * 2 * c.stage_words;
+    uint32_t *mw = qw + c.stage_words;
+    for (int w = 0; w < nw; ++w) {
+        uint32_t v = 0u, m = 0u;
+        const int b0 = 16 * w, b1 = std::min(qlen, b0 + 16);
+        for (int b = b0; b < b1; ++b) {
+            uint32_t x = codes[b];
+            uint32_t n = 0u;
+            if (two_bit) {
+                GMFA_REQUIRE(x <= 3u, "code %u at %d outside 0..3", x, b);
+            } else {
+                GMFA_REQUIRE(x >= 1u && x <= 5u, "code %u at %d outside 1..5", x, b);
+                if (x == 5u) {
+                    GMFA_REQUIRE(c.nm, "an N (code 5 at %d) needs a GMFA_SW_NMASK class", b);
+                    x = 1u;
+                    n = 1u;
+                }
+                x -= 1u;
+            }
+            v |= x << (2 * (b - b0));
+            m |= n << (2 * (b - b0));
+        }
+        qw[w] = v;
+        if (c.nm) mw[w] = m;
+    }
+    ServeDesc &d = c.desc[r];
+    d.t_off = t_off;
+    d.qlen = qlen;
+    d.tlen = tlen;
+    d.band_offset = band_offset;
+    d.q_words = nw;
+    c.occupant[r] = t;
+    c.next = t + 1;
+    __atomic_store_n(&c.ctl->tail.v, c.next, __ATOMIC_RELEASE);
+    ++s->submitted;
+    *ticket = t;
+    return ensure_running(s, c, was_empty);
+}
+
+extern "C" int gmfa_serve_poll(gmfa_serve_session *s, gmfa_serve_result *results,
+                               int32_t max_results, uint32_t *cigar, int64_t cigar_cap,
+                               int32_t *n_results)
+{
+    GMFA_REQUIRE(s && n_results, "NULL argument");
+    GMFA_REQUIRE(max_results >= 0 && (max_results == 0 || results), "bad results buffer");
+    GMFA_REQUIRE(cigar_cap >= 0 && (cigar_cap == 0 || cigar), "bad cigar buffer");
+    *n_results = 0;
+    int n = 0;
+    int64_t coff = 0;
+    for (int ci = 0; ci < (int)s->cls.size(); ++ci) {
+        OnlineClass &c = s->cls[ci];
+        const int32_t ring = c.cfg.ring, cap = c.cfg.cigar_cap;
+        for (int64_t t = c.lo; t < c.next && n < max_results; ++t) {
+            const int r = (int)(t % ring);
+            if (c.occupant[r] != t) continue;
+            if (__atomic_load_n(&c.done[r], __ATOMIC_ACQUIRE) != t + 1) continue;
+            const ServeRes &e = c.res[r];
+            const int nc = e.n_cigar;
+            const bool trunc = nc > cap;
+            const int keep = trunc ? 0 : nc;
+            if (coff + keep > cigar_cap) break;
+            gmfa_serve_result &o = results[n++];
+            o.ticket = t;
+            o.cls = ci;
+            o.status = trunc ? GMFA_SERVE_TRUNCATED : 0;
+            o.score = e.score;
+            o.query_end = e.query_end;
+            o.target_end = e.target_end;
+            o.read_start = e.read_start;
+            o.ref_start = e.ref_start;
+            o.n_cigar = nc;
+            for (int k = 0; k < 4; ++k) o.counts[k] = e.counts[k];
+            o.cigar_off = coff;
+            if (keep)
+                std::memcpy(cigar + coff, c.cig + (size_t)r * cap + (cap - keep),
+                            sizeof(uint32_t) * (size_t)keep);
+            coff += keep;
+            c.occupant[r] = -1;
+            ++s->returned;
+            ++s->completed;
+        }
+        while (c.lo < c.next && c.occupant[(int)(c.lo % ring)] != c.lo) ++c.lo;
+        const int rc = ensure_running(s, c, false);
+        if (rc != GMFA_OK) {
+            *n_results = n;
+            return rc;
+        }
+    }
+    *n_results = n;
+    return GMFA_OK;
+}
+
+extern "C" int gmfa_serve_session_stats(const gmfa_serve_session *s, gmfa_serve_stats *st)
+{
+    GMFA_REQUIRE(s && st, "NULL argument");
+    st->submitted = s->submitted;
+    st->returned = s->returned;
+    st->in_flight = s->submitted - s->returned;
+    st->launches = s->launches;
+    return GMFA_OK;
+}
+
+extern "C" int gmfa_serve_close(gmfa_serve_session *s, gmfa_serve_report *report)
+{
+    GMFA_REQUIRE(s, "NULL session");
+    // publish nothing more, let the waves drain what is published, then count
+    for (auto &c : s->cls) __atomic_store_n(&c.ctl->closed.v, 1, __ATOMIC_RELEASE);
+    int rc = GMFA_OK;
+    for (auto &c : s->cls) {
+        // a class whose waves all left on idle with requests pending runs once
+        // more to drain them (it sees closed and exits when done)
+        if (rc == GMFA_OK) rc = ensure_running(s, c, true);
+        if (hipStreamSynchronize(c.stream) != hipSuccess && rc == GMFA_OK) {
+            set_error("serving class kernel failed while draining");
+            rc = GMFA_E_RUNTIME;
+        }
+    }
+    int64_t done_unpolled = 0;
+    for (auto &c : s->cls) {
+        for (int64_t t = c.lo; t < c.next; ++t) {
+            const int r = (int)(t % c.cfg.ring);
+            if (c.occupant[r] == t && __atomic_load_n(&c.done[r], __ATOMIC_ACQUIRE) == t + 1)
+                ++done_unpolled;
+        }
+    }
+    if (report) {
+        std::memset(report, 0, sizeof(*report));
+        report->completed = s->completed + done_unpolled;
+    }
+    gmfa::free_session(s);
     return rc;
 }

@@ -90,6 +90,17 @@ struct SwKArgs {
     int32_t *serve_gave_up;       // set to 1 if a wave's wait hit deadline_ticks
     int64_t slab_words;           // traceback plane words per wave slot
     uint64_t deadline_ticks;      // bound of one idle wait (wall-clock ticks)
+    // online serving (ring_n > 0): requests arrive in a ring in host-coherent
+    // memory while the kernel runs (fields as in ServeLaunch)
+    int32_t ring_n, stage_words, cigar_cap;
+    const int64_t *ring_tail;
+    unsigned long long *head;
+    const ServeDesc *ring_desc;
+    const uint32_t *ring_q;
+    ServeRes *ring_res;
+    uint32_t *ring_cigar;
+    int64_t *ring_done;
+    uint32_t *stage;
 };
 
 // Wait until request `item` is published or the host closes the queue; returns
@@ -122,6 +133,55 @@ __device__ __forceinline__ int serve_wait(const int32_t *tail_p, const int32_t *
                 __hip_atomic_store(gave_up, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             return tail;
         }
+        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(127);
+        nap = nap < 16 ? 2 * nap : 16;
+    }
+}
+
+__device__ __forceinline__ int64_t uni64(int64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Online serving: claim the next published, unclaimed ticket.  A wave claims
+// only a ticket the host has already published (compare-and-swap on the
+// device head below the host tail), so a wave that stops waiting holds
+// nothing and the host can relaunch the class later without losing a
+// request.  Returns false once the queue is closed and drained, or after
+// idle_ticks of the constant-rate wall clock without a claim (the wave then
+// exits; the host relaunches the class when requests are pending and every
+// wave has gone).
+__device__ __forceinline__ bool serve_claim(unsigned long long *head_p, const int64_t *tail_p,
+                                            const int32_t *closed_p, uint64_t idle_ticks,
+                                            int64_t &ticket)
+{
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    int nap = 1;
+    for (;;) {
+        // closed before tail: the host publishes its last tail, then closes;
+        // once closed is seen, the acquire orders the tail load after it
+        const int closed = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(closed_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        if (closed) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        const int64_t tail =
+            uni64(__hip_atomic_load(tail_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        int64_t head = uni64((int64_t)__hip_atomic_load(head_p, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT));
+        while (head < tail) {
+            unsigned long long got = 0ull;
+            if ((threadIdx.x & 63) == 0) got = atomicCAS(head_p, (unsigned long long)head,
+                                                         (unsigned long long)head + 1ull);
+            const int64_t g = uni64((int64_t)got);
+            if (g == head) {
+                ticket = head;
+                return true;
+            }
+            head = g;
+        }
+        if (closed) return false;
+        if ((uint64_t)wall_clock64() - t0 > idle_ticks) return false;
         for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(127);
         nap = nap < 16 ? 2 * nap : 16;
     }
@@ -337,6 +397,45 @@ __device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, in
     }
 }
 
+// Online serving walk: the runs go straight to the request's ring entry in
+// host memory.  Run x (counted from the alignment's end) belongs at
+// cig[cap - 1 - x], so the entry ends with the runs in forward order and no
+// reversal pass reads host memory back; runs are gathered 64 at a time (run x
+// in lane x & 63) and stored by the whole wavefront at once.  Runs past cap
+// are counted but not stored.  The outputs record follows as one store per
+// field lane.
+template <int C>
+__device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, int cap,
+                                               ServeRes *res, int i, int j, int h, int off,
+                                               int64_t qo, int64_t to, const uint32_t *tb)
+{
+    int lane = (int)(threadIdx.x & 63);
+    asm volatile("" : "+v"(lane));  // keep lane-derived addresses inside the request loop
+    int nr = 0;
+    int cnt[4] = {0, 0, 0, 0};
+    uint32_t buf = 0u;
+    auto ld = [&](const uint32_t *src, int64_t idx) {
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)src[idx]);
+    };
+    auto put = [&](int x, uint32_t vrun) {
+        if (lane == (x & 63)) buf = vrun;
+        if ((x & 63) == 63) {
+            const int y = cap - 1 - (x - 63 + lane);
+            if (y >= 0) cig[y] = buf;
+        }
+    };
+    const int score = h, qend = i, tend = j;
+    walk_steps<C>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    if (lane < (nr & 63)) {
+        const int y = cap - 1 - ((nr & ~63) + lane);
+        if (y >= 0) cig[y] = buf;
+    }
+    const int v = lane == 0 ? score : lane == 1 ? qend : lane == 2 ? tend : lane == 3 ? i
+                : lane == 4 ? j : lane == 5 ? nr : lane == 6 ? cnt[0] : lane == 7 ? cnt[1]
+                : lane == 8 ? cnt[2] : cnt[3];
+    if (lane < 10) reinterpret_cast<int32_t *>(res)[lane] = v;
+}
+
 struct TbKArgs {
     WalkCtx c;
     WalkOut o;
@@ -380,7 +479,8 @@ __global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
     a.o.counts[4 * p + 3] = cnt[3];
 }
 
-// SV: serving variant (persistent queue fed by the host); a separate
+// SV: serving variant (persistent queue fed by the host; staged requests of
+// gmfa_serve_run or the online ring of gmfa_serve_open), a separate
 // instantiation so the offline kernels carry none of its state.
 template <int C, int G, int KS, bool TB, bool NM, bool RK, bool SV = false>
 __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
@@ -405,27 +505,68 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
     const uint32_t mms = (uint32_t)(ma + mi);
     const int64_t t_hi = a.t_words - 1, q_hi = a.q_words - 1;
 
+    // online form: a runtime switch of the serving instantiation (a separate
+    // instantiation scheduled the DP with up to 90 more VGPRs)
+    bool online = false;
+    if constexpr (SV) online = a.ring_n > 0;
     for (;;) {
         int item = 0;
-        if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
-        item = __builtin_amdgcn_readfirstlane(item);
-        // serving: one pair per wavefront (G == 64), bounded by what the host
-        // has published
         int n_items = a.n_items;
-        if constexpr (SV) {
-            if (item >= a.n_pairs) break;  // past the class's last request: nothing to wait for
-            n_items = serve_wait(a.serve_tail, a.serve_closed, a.serve_gave_up, a.deadline_ticks, item);
+        // online serving: the request's fields come from its ring entry; its
+        // query sits in this wave's stage slot (the a.qseq stream of online
+        // launches is the stage array), so the forward pass is the staged one
+        int64_t ticket = 0;
+        int ring_r = 0;
+        int r_qlen = 0, r_tlen = 0, r_off = 0;
+        int64_t r_to = 0;
+        if (online) {
+            if (!serve_claim(a.head, a.ring_tail, a.serve_closed, a.deadline_ticks, ticket)) break;
+            // the host wrote the entry before publishing the ticket
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+            ring_r = (int)(ticket & (int64_t)(a.ring_n - 1));  // ring_n is a power of two
+            // lane index through an opaque copy: nothing derived from it is
+            // hoisted out of the request loop to stay live across the DP
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const uint32_t *dw = reinterpret_cast<const uint32_t *>(a.ring_desc + ring_r);
+            const uint32_t dv = ln < 6 ? dw[ln] : 0u;
+            r_to = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dv, 1) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane((int)dv, 0));
+            r_qlen = __builtin_amdgcn_readlane((int)dv, 2);
+            r_tlen = __builtin_amdgcn_readlane((int)dv, 3);
+            r_off = __builtin_amdgcn_readlane((int)dv, 4);
+            const int nw = __builtin_amdgcn_readlane((int)dv, 5);
+            // the query moves once into the stage slot; the forward pass and
+            // the walk then read it from device memory
+            const int64_t sw2 = 2 * (int64_t)a.stage_words;
+            uint32_t *stg = a.stage + (int64_t)blockIdx.x * sw2;
+            const uint32_t *src = a.ring_q + (int64_t)ring_r * sw2;
+            for (int x = ln; x < nw; x += 64) stg[x] = src[x];
+            if (NM)
+                for (int x = ln; x < nw; x += 64) stg[a.stage_words + x] = src[a.stage_words + x];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        } else {
+            if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
+            item = __builtin_amdgcn_readfirstlane(item);
+            // serving: one pair per wavefront (G == 64), bounded by what the host
+            // has published
+            if constexpr (SV) {
+                if (item >= a.n_pairs) break;  // past the class's last request: nothing to wait for
+                n_items = serve_wait(a.serve_tail, a.serve_closed, a.serve_gave_up, a.deadline_ticks, item);
+            }
+            if (item >= n_items) break;
         }
-        if (item >= n_items) break;
 
         const int pidx = item * (64 / G) + grp;
-        const bool has = pidx < (SV ? n_items : a.n_pairs);
-        const int p = has ? (a.order ? a.order[pidx] : pidx) : 0;
-        const int qlen = uni<G>(has ? a.qlen[p] : 0);
-        const int tlen = uni<G>(has ? a.tlen[p] : 0);
-        const int off = uni<G>(has ? a.boff[p] : 0);
-        const int64_t qo = has ? a.q_off[p] : 0;
-        const int64_t to = has ? a.t_off[p] : 0;
+        const bool has = online || pidx < (SV ? n_items : a.n_pairs);
+        const int p = (has && !online) ? (a.order ? a.order[pidx] : pidx) : 0;
+        const int qlen = online ? r_qlen : uni<G>(has ? a.qlen[p] : 0);
+        const int tlen = online ? r_tlen : uni<G>(has ? a.tlen[p] : 0);
+        const int off = online ? r_off : uni<G>(has ? a.boff[p] : 0);
+        const int64_t qo = online ? (int64_t)blockIdx.x * 32 * (int64_t)a.stage_words
+                                  : (has ? a.q_off[p] : 0);
+        const int64_t to = online ? r_to : (has ? a.t_off[p] : 0);
         const int maxq = (G == 64) ? qlen : wave_max(qlen);
 
         int Hp[C], Fp[C];
@@ -659,10 +800,24 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             res_i = (int)(0xFFFFFu - (uint32_t)((key >> 12) & 0xFFFFFu));
             res_j = res_i + off - hb + (int)(0xFFFu - (uint32_t)(key & 0xFFFu));
         }
-        if (gl == 0 && has) {
+        if (gl == 0 && has && !online) {
             a.score[p] = res_s;
             a.qend[p] = res_i;
             a.tend[p] = res_j;
+        }
+        if constexpr (TB && !RK && SV) {
+            if (online) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                walk_pair_ring<C>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap, a.cigar_cap,
+                                  a.ring_res + ring_r, res_i, res_j, res_s, off, qo, to,
+                                  a.tb + tb_base);
+                // every output of the request (stored by this wave) before its done word
+                if (lane == 0)
+                    __hip_atomic_store(a.ring_done + ring_r, ticket + 1, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+                continue;
+            }
         }
         if constexpr (TB && !RK) {
             if (a.walk) {
@@ -1079,13 +1234,17 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     gmfa_sw_tile tile;
     int rc = gmfa_sw_resolve_tile(P.half_band, P.cells_per_lane, P.lanes_per_pair, &tile);
     if (rc != GMFA_OK) return rc;
-    const int n = batch->n_pairs;
+    const bool online = serve && serve->ring_n > 0;
+    // online serving: requests come from the ring; the batch gives the
+    // reference (tseq / tmask / t_words) and the query length bound only
+    const int n = online ? 1 : batch->n_pairs;
     GMFA_REQUIRE(n >= 0, "negative n_pairs");
     if (n == 0) return GMFA_OK;
-    GMFA_REQUIRE(batch->qseq && batch->tseq && batch->q_off && batch->t_off && batch->qlen &&
-                     batch->tlen && batch->band_offset,
+    GMFA_REQUIRE(online || (batch->qseq && batch->tseq && batch->q_off && batch->t_off &&
+                            batch->qlen && batch->tlen && batch->band_offset),
                  "NULL batch pointer");
-    GMFA_REQUIRE(batch->q_words > 0 && batch->t_words > 0, "empty sequence streams");
+    GMFA_REQUIRE(batch->tseq, "NULL target stream");
+    GMFA_REQUIRE(batch->qseq && batch->q_words > 0 && batch->t_words > 0, "empty sequence streams");
     GMFA_REQUIRE(batch->max_qlen >= 0 && batch->max_qlen <= (1 << 20),
                  "query length bound %d outside [0, 2^20]", batch->max_qlen);
     GMFA_REQUIRE((int64_t)batch->max_qlen * (P.match + P.mismatch) < (1 << 24),
@@ -1094,7 +1253,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                  "per-row keys need query length x match below 2^19");
     const bool nm = P.flags & GMFA_SW_NMASK;
     GMFA_REQUIRE(!nm || (batch->qmask && batch->tmask), "GMFA_SW_NMASK needs qmask and tmask");
-    GMFA_REQUIRE(out->score && out->query_end && out->target_end && out->work_counter,
+    GMFA_REQUIRE(online || (out->score && out->query_end && out->target_end && out->work_counter),
                  "NULL output pointer");
     GMFA_REQUIRE(!(P.flags & GMFA_SW_TRACEBACK) || (out->tb && (out->tb_off || serve)),
                  "GMFA_SW_TRACEBACK needs tb and tb_off");
@@ -1124,8 +1283,8 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
         GMFA_REQUIRE(P.flags & GMFA_SW_TRACEBACK, "GMFA_SW_CIGAR needs GMFA_SW_TRACEBACK");
         GMFA_REQUIRE(!(P.flags & GMFA_SW_ROWKEYS),
                      "GMFA_SW_CIGAR walks the unadjusted best cell; x-drop batches use gmfa_sw_traceback");
-        GMFA_REQUIRE(out->cigar_off && out->cigar && out->n_cigar && out->read_start &&
-                         out->ref_start && out->counts,
+        GMFA_REQUIRE(online || (out->cigar_off && out->cigar && out->n_cigar &&
+                                out->read_start && out->ref_start && out->counts),
                      "GMFA_SW_CIGAR needs the CIGAR outputs");
         a.walk = 1;
         a.wo.cig_off = out->cigar_off; a.wo.cigar = out->cigar; a.wo.ncig = out->n_cigar;
@@ -1134,6 +1293,9 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
 
     a.serve_tail = nullptr; a.serve_closed = nullptr; a.serve_done = nullptr;
     a.serve_gave_up = nullptr; a.slab_words = 0; a.deadline_ticks = 0;
+    a.ring_n = 0; a.stage_words = 0; a.cigar_cap = 0;
+    a.ring_tail = nullptr; a.head = nullptr; a.ring_desc = nullptr; a.ring_q = nullptr;
+    a.ring_res = nullptr; a.ring_cigar = nullptr; a.ring_done = nullptr; a.stage = nullptr;
     int waves = 1;
     if (!serve) {
         waves = std::max(1, std::min(a.n_items, resident_waves(k)));
@@ -1153,9 +1315,31 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
         a.serve_gave_up = serve->gave_up; a.slab_words = serve->slab_words;
         a.deadline_ticks = serve->deadline_ticks;
         waves = serve->n_slots;
+        if (online) {
+            GMFA_REQUIRE(serve->ring_tail && serve->head && serve->ring_desc && serve->ring_q &&
+                             serve->ring_res && serve->ring_cigar && serve->ring_done &&
+                             serve->stage && serve->closed,
+                         "NULL online serving pointer");
+            GMFA_REQUIRE(serve->stage_words >= serve_stage_words(batch->max_qlen),
+                         "stage of %d words per slot is below the %d the longest query needs",
+                         serve->stage_words, serve_stage_words(batch->max_qlen));
+            GMFA_REQUIRE(serve->cigar_cap >= 1, "cigar_cap must be positive");
+            GMFA_REQUIRE((serve->ring_n & (serve->ring_n - 1)) == 0, "ring must be a power of two");
+            GMFA_REQUIRE(batch->qseq == serve->stage &&
+                             batch->q_words >= 2 * (int64_t)serve->stage_words * serve->n_slots &&
+                             (!nm || batch->qmask == serve->stage + serve->stage_words),
+                         "online batch: qseq must be the stage array (qmask one stage slot on)");
+            a.ring_n = serve->ring_n; a.stage_words = serve->stage_words;
+            a.cigar_cap = serve->cigar_cap; a.ring_tail = serve->ring_tail; a.head = serve->head;
+            a.ring_desc = serve->ring_desc; a.ring_q = serve->ring_q; a.ring_res = serve->ring_res;
+            a.ring_cigar = serve->ring_cigar; a.ring_done = serve->ring_done;
+            a.stage = serve->stage;
+        }
     }
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-    GMFA_CHECK_HIP(hipGetLastError());
+    if (!online) {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
+        GMFA_CHECK_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(k, dim3(waves), dim3(64), 0, s, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;

@@ -9,23 +9,60 @@
 
 namespace gmfa {
 
-// Persistent serving launch: the forward kernel claims request k only once
-// k < *tail, walks it into a CIGAR in the same launch, then sets done[k] = 1.
-// tail / closed / done / gave_up are device-visible pointers into
-// host-coherent memory.
+// One submitted request of an online session, written by the host into
+// host-coherent memory before the class's tail is advanced past it.
+struct alignas(32) ServeDesc {
+    int64_t t_off;        // first target base in the resident reference
+    int32_t qlen, tlen;
+    int32_t band_offset;
+    int32_t q_words;      // 2-bit words of the query in its ring entry
+    int32_t pad[2];
+};
+
+// The outputs of one request of an online session (host-coherent memory),
+// complete once the entry's done word holds ticket + 1.
+struct alignas(64) ServeRes {
+    int32_t score, query_end, target_end, read_start, ref_start, n_cigar;
+    int32_t counts[4];
+    int32_t pad[6];
+};
+
+// Persistent serving launch.  Staged form (ring_n == 0): the forward kernel
+// claims request k only once k < *tail, walks it into a CIGAR in the same
+// launch, then sets done[k] = 1.  Online form (ring_n > 0): a wavefront claims
+// the next published ticket from *head, copies that ring entry's query into
+// its slot of `stage`, aligns it and writes its outputs into the host ring.
+// Pointers named "host-coherent" are device-visible pointers into pinned
+// memory that the host thread reads and writes while the kernel runs.
 struct ServeLaunch {
-    const int32_t *tail;
-    const int32_t *closed;
-    int32_t *done;
-    int32_t *gave_up;
+    const int32_t *tail;      // staged: requests published (host-coherent)
+    const int32_t *closed;    // host-coherent: nothing more will be published
+    int32_t *done;            // staged: [n_pairs] completion flags (host-coherent)
+    int32_t *gave_up;         // host-coherent: a wave's idle wait hit its bound
     int64_t slab_words;       // traceback plane words per wave slot (out->tb holds n_slots slabs)
     uint64_t deadline_ticks;  // bound of one idle wait of a wave, wall-clock ticks
     int32_t n_slots;          // persistent wavefronts
+    // online form
+    int32_t ring_n;                  // ring entries (0: staged form)
+    const int64_t *ring_tail;        // host-coherent: tickets published
+    unsigned long long *head;        // device: next ticket to claim
+    const ServeDesc *ring_desc;      // host-coherent [ring_n]
+    const uint32_t *ring_q;          // host-coherent [ring_n][2 * stage_words]
+    ServeRes *ring_res;              // host-coherent [ring_n]
+    uint32_t *ring_cigar;            // host-coherent [ring_n][cigar_cap]
+    int64_t *ring_done;              // host-coherent [ring_n]: ticket + 1 when complete
+    uint32_t *stage;                 // device [n_slots][2 * stage_words]
+    int32_t stage_words;             // query words per slot (mask words follow)
+    int32_t cigar_cap;               // CIGAR runs kept per request
 };
 
 // Validates and launches the forward kernel (work-counter reset + the kernel)
-// on stream s; serve == nullptr is the offline batch of gmfa_sw_forward.
+// on stream s; serve == nullptr is the offline batch of gmfa_sw_forward.  The
+// online form does not reset its head (relaunches continue the ticket count).
 int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gmfa_sw_out *out,
               const ServeLaunch *serve, hipStream_t s);
+
+// Query words a stage slot holds for queries up to max_qlen bases.
+inline int32_t serve_stage_words(int32_t max_qlen) { return (max_qlen + 15) / 16 + 2; }
 
 }  // namespace gmfa

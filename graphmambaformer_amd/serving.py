@@ -18,6 +18,7 @@ for the duration of a run.
 from __future__ import annotations
 
 import ctypes
+import time
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -203,3 +204,186 @@ def latency_summary(arrival_s: np.ndarray, complete_s: np.ndarray, skip: int = 0
     return {"n": int(lat.size), "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_ms": round(float(lat.mean()), 3), "max_ms": round(float(lat.max()), 3)}
+
+
+# ---------------------------------------------------------------------------
+# Online sessions: requests submitted one at a time while the kernels run
+# ---------------------------------------------------------------------------
+
+@dataclass
+class OnlineClass:
+    """One band class of an online session.  cigar_cap=None keeps every run
+    (a banded local alignment has at most 2*max_qlen + 2*half_band + 2);
+    ring=None holds the next power of two >= 2 * n_slots requests in flight."""
+
+    half_band: int
+    max_qlen: int
+    n_slots: int
+    ring: Optional[int] = None
+    cigar_cap: Optional[int] = None
+    tile: Optional[SwTile] = None
+
+
+@dataclass
+class ServedAlignment:
+    cls: int
+    ticket: int
+    score: int
+    query_end: int
+    target_end: int
+    read_start: int
+    ref_start: int
+    cigar: list
+    counts: tuple
+    truncated: bool
+
+
+def _pow2_at_least(n: int) -> int:
+    p = 1
+    while p < n:
+        p *= 2
+    return p
+
+
+class ServeSession:
+    """Online serving of banded SW + CIGAR requests (gmfa_serve_open /
+    submit / poll / close) over a resident reference.  One host thread per
+    session.  The device memory of every class (traceback slabs, query stage
+    slots, ticket head) is held here as torch tensors; the library holds the
+    request rings in host-coherent memory.
+
+        with ServeSession(ref, [OnlineClass(32, 200, 512)], (2, 4, 4, 2)) as s:
+            t = s.submit(0, query_codes, t_start, t_len, band_offset)
+            done = s.poll()
+    """
+
+    def __init__(self, reference: ResidentReference, classes: Sequence[OnlineClass],
+                 scoring: tuple, idle_timeout_s: float = 1.0, poll_results: int = 4096):
+        L = lib()
+        self.reference = reference
+        dev = reference.words.device
+        ma, mi, go, ge = (int(x) for x in scoring)
+        self.classes = list(classes)
+        self._keep = []
+        cfgs = (_lib.ServeClassCfg * len(self.classes))()
+        max_cap = 1
+        for k, oc in enumerate(self.classes):
+            tile = oc.tile or serve_tile(oc.half_band)
+            if tile.lanes_per_pair != 64:
+                raise ValueError("serving needs a 64-lane tile (one pair per wavefront)")
+            ring = int(oc.ring) if oc.ring is not None else _pow2_at_least(2 * int(oc.n_slots))
+            cap = (int(oc.cigar_cap) if oc.cigar_cap is not None
+                   else 2 * int(oc.max_qlen) + 2 * int(oc.half_band) + 2)
+            max_cap = max(max_cap, cap)
+            slab = max(int(kernels.tb_bytes(np.array([int(oc.max_qlen)]), tile)[0]) // 4, 1)
+            sw = int(L.gmfa_serve_stage_words(int(oc.max_qlen)))
+            tb = torch.empty(int(oc.n_slots) * slab, dtype=torch.int32, device=dev)
+            stage = torch.zeros(int(oc.n_slots) * 2 * sw, dtype=torch.int32, device=dev)
+            head = torch.zeros(1, dtype=torch.int64, device=dev)
+            self._keep += [tb, stage, head]
+            c = cfgs[k]
+            c.params = kernels.make_params(ma, mi, go, ge, int(oc.half_band),
+                                           _lib.SW_TRACEBACK | _lib.SW_CIGAR, tile)
+            c.max_qlen, c.ring, c.n_slots, c.cigar_cap = int(oc.max_qlen), ring, int(oc.n_slots), cap
+            c.tb, c.slab_words, c.stage, c.head = ptr(tb), slab, ptr(stage), ptr(head)
+        self._h = None
+        self._results = (_lib.ServeResultRec * int(poll_results))()
+        self._cigar = np.zeros(max(max_cap, 1 << 16), np.uint32)
+        handle = ctypes.c_void_p()
+        torch.cuda.synchronize(dev)
+        check(L.gmfa_serve_open(ptr(reference.words), None, int(reference.n_bases), cfgs,
+                                len(self.classes), float(idle_timeout_s), ctypes.byref(handle)),
+              "gmfa_serve_open")
+        self._h = handle
+        self._pending = []
+
+    # -- requests ----------------------------------------------------------
+    def submit(self, cls: int, query: np.ndarray, t_start: int, t_len: int, band_offset: int,
+               two_bit: bool = True, wait_s: float = 30.0) -> int:
+        """Publish one request; returns its ticket (per class).  query: codes
+        0..3 (two_bit) or A,C,G,T = 1..4.  When the class's ring is full the
+        call polls (keeping the results for the next poll()) until an entry
+        frees, for at most wait_s."""
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        tk = ctypes.c_int64()
+        deadline = time.monotonic() + float(wait_s)
+        while True:
+            rc = lib().gmfa_serve_submit(self._handle(), int(cls), q.ctypes.data, len(q),
+                                         1 if two_bit else 0, int(t_start), int(t_len),
+                                         int(band_offset), ctypes.byref(tk))
+            if rc != _lib.GMFA_E_AGAIN:
+                check(rc, "gmfa_serve_submit")
+                return int(tk.value)
+            self._pending.extend(self._poll_raw(len(self._results)))
+            if time.monotonic() > deadline:
+                raise RuntimeError(f"class {cls}: ring still full after {wait_s} s")
+
+    def poll(self, max_results: Optional[int] = None) -> list:
+        """Completed requests (never blocks)."""
+        out, self._pending = self._pending, []
+        n = len(self._results) if max_results is None else int(max_results)
+        return out + self._poll_raw(max(0, n - len(out)))
+
+    def wait(self, tickets: int, timeout_s: float = 30.0) -> list:
+        """Poll until `tickets` results have come back (or the timeout)."""
+        got = []
+        end = time.monotonic() + timeout_s
+        while len(got) < tickets:
+            got += self.poll()
+            if time.monotonic() > end:
+                raise RuntimeError(f"{len(got)} of {tickets} requests completed in {timeout_s} s")
+        return got
+
+    def stats(self) -> dict:
+        st = _lib.ServeStats()
+        check(lib().gmfa_serve_session_stats(self._handle(), ctypes.byref(st)), "stats")
+        return {"submitted": st.submitted, "returned": st.returned,
+                "in_flight": st.in_flight, "launches": st.launches}
+
+    def close(self) -> ServeResult:
+        if self._h is None:
+            return ServeResult(0.0, 0, False, False)
+        rep = ServeReport()
+        h, self._h = self._h, None
+        check(lib().gmfa_serve_close(h, ctypes.byref(rep)), "gmfa_serve_close")
+        return ServeResult(wall_s=0.0, completed=int(rep.completed), timed_out=False,
+                           waves_gave_up=bool(rep.waves_gave_up))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                self.close()
+        except Exception:
+            pass
+
+    # -- internals ---------------------------------------------------------
+    def _handle(self):
+        if self._h is None:
+            raise RuntimeError("serving session is closed")
+        return self._h
+
+    def _poll_raw(self, n: int) -> list:
+        if n <= 0:
+            return []
+        got = ctypes.c_int32()
+        check(lib().gmfa_serve_poll(self._handle(), self._results, min(n, len(self._results)),
+                                    self._cigar.ctypes.data, len(self._cigar),
+                                    ctypes.byref(got)), "gmfa_serve_poll")
+        out = []
+        for k in range(got.value):
+            r = self._results[k]
+            trunc = r.status == _lib.GMFA_SERVE_TRUNCATED
+            runs = self._cigar[r.cigar_off:r.cigar_off + (0 if trunc else r.n_cigar)]
+            cig = [(_lib.OP_CHAR[int(v) & 15], int(v) >> 4) for v in runs]
+            out.append(ServedAlignment(cls=int(r.cls), ticket=int(r.ticket), score=int(r.score),
+                                       query_end=int(r.query_end), target_end=int(r.target_end),
+                                       read_start=int(r.read_start), ref_start=int(r.ref_start),
+                                       cigar=cig, counts=tuple(int(x) for x in r.counts),
+                                       truncated=trunc))
+        return out

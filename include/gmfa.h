@@ -42,6 +42,9 @@
  *                      (ExtensionEngine.extend_chains, extension.py:511-641,
  *                      one read after another) as a continuously batched
  *                      server: requests are aligned as they arrive
+ *   gmfa_serve_open / gmfa_serve_submit / gmfa_serve_poll / gmfa_serve_close
+ *                      the same per-read calls as an online session fed by a
+ *                      front end one request at a time
  */
 #ifndef GMFA_H
 #define GMFA_H
@@ -58,6 +61,7 @@ extern "C" {
 #define GMFA_E_INVALID -1   /* bad argument: maps to ValueError */
 #define GMFA_E_RUNTIME -2   /* HIP runtime / launch failure: maps to RuntimeError */
 #define GMFA_E_UNSUPPORTED -3 /* shape outside the compiled variants: RuntimeError */
+#define GMFA_E_AGAIN -4       /* serving ring full: poll, then retry */
 
 /* flags for gmfa_sw_params.flags */
 #define GMFA_SW_TRACEBACK 1u  /* write the 4-bit traceback planes */
@@ -395,6 +399,99 @@ typedef struct gmfa_serve_report {
 
 int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes, double timeout_s,
                    gmfa_serve_report *report);
+
+/* ---------------------------------------------------------------------------
+ * Online serving session: a front end submits requests one at a time while
+ * the kernels run, and polls for completed ones (the gmfa_serve_submit /
+ * gmfa_serve_poll pair).  The same per-band-class persistent kernels as
+ * gmfa_serve_run, fed from a ring per class in host-coherent memory:
+ * gmfa_serve_submit packs the query into the next free ring entry, writes the
+ * request (reference window, band offset) and publishes it; a wavefront claims
+ * it, copies the query into its device stage slot, aligns it, walks its CIGAR
+ * and writes the outputs straight into the entry; gmfa_serve_poll returns the
+ * completed entries and frees them.  A wavefront claims only published
+ * requests, so waves that idle for idle_timeout_s exit (freeing the CUs) and
+ * the class is relaunched by the next submit or poll that finds requests
+ * pending and no wave left.  One host thread per session; sessions on one
+ * device share its CUs.
+ *
+ * Device memory per class is the caller's: n_slots traceback slabs (tb,
+ * slab_words >= gmfa_sw_tb_bytes(max_qlen) / 4 each), n_slots query stage
+ * slots (stage, 2 * gmfa_serve_stage_words(max_qlen) words each) and the
+ * 8-byte ticket head (head, zeroed by gmfa_serve_open).  The library allocates
+ * the rings and control words in host-coherent memory for the session's life.
+ * ------------------------------------------------------------------------- */
+typedef struct gmfa_serve_session gmfa_serve_session;
+
+typedef struct gmfa_serve_class_cfg {
+    gmfa_sw_params params;  /* GMFA_SW_TRACEBACK | GMFA_SW_CIGAR (+ GMFA_SW_NMASK),
+                               a tile with 64 lanes per pair */
+    int32_t max_qlen;       /* longest query the class accepts, <= 2^20 */
+    int32_t ring;           /* requests in flight (submitted, not yet polled): a power of two <= 2^20 */
+    int32_t n_slots;        /* persistent wavefronts */
+    int32_t cigar_cap;      /* CIGAR runs kept per request; a longer CIGAR is
+                               reported with status GMFA_SERVE_TRUNCATED */
+    uint32_t *tb;           /* device: n_slots slabs of slab_words */
+    int64_t slab_words;
+    uint32_t *stage;        /* device: n_slots * 2 * gmfa_serve_stage_words(max_qlen) */
+    uint64_t *head;         /* device: one uint64 */
+} gmfa_serve_class_cfg;
+
+#define GMFA_SERVE_TRUNCATED 1  /* n_cigar > cigar_cap: no runs returned */
+
+typedef struct gmfa_serve_result {
+    int64_t ticket;         /* as returned by gmfa_serve_submit (per class, from 0) */
+    int32_t cls;
+    int32_t status;         /* 0 or GMFA_SERVE_TRUNCATED */
+    int32_t score;
+    int32_t query_end;      /* 1-based end row / column of the best cell */
+    int32_t target_end;
+    int32_t read_start;     /* 0-based start of the local alignment */
+    int32_t ref_start;      /* within the request's reference window */
+    int32_t n_cigar;        /* runs ((length << 4) | op, GMFA_OP_*) */
+    int32_t counts[4];      /* bases under =, X, I, D */
+    int64_t cigar_off;      /* first run in the caller's cigar buffer */
+} gmfa_serve_result;
+
+typedef struct gmfa_serve_stats {
+    int64_t submitted;      /* requests published over all classes */
+    int64_t returned;       /* requests returned by gmfa_serve_poll */
+    int64_t in_flight;      /* submitted - returned */
+    int64_t launches;       /* class kernel launches (1 per class + relaunches) */
+} gmfa_serve_stats;
+
+/* Query words of one stage slot for queries up to max_qlen bases. */
+int32_t gmfa_serve_stage_words(int32_t max_qlen);
+
+/* ref_seq / ref_mask: the resident reference, 2-bit stream of ref_bases bases
+ * (device; ref_mask needed by classes with GMFA_SW_NMASK).  idle_timeout_s in
+ * (0, 3600]: how long a wavefront waits for a request before it exits. */
+int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask, int64_t ref_bases,
+                    const gmfa_serve_class_cfg *classes, int32_t n_classes,
+                    double idle_timeout_s, gmfa_serve_session **session);
+
+/* Submit one request to class cls: the query as qlen host codes (two_bit: 0..3;
+ * else A,C,G,T = 1..4 and N = 5, N only for GMFA_SW_NMASK classes), aligned
+ * against reference bases [t_off, t_off + tlen) with the band centred on
+ * diagonal band_offset.  *ticket receives the request's ticket.  Returns
+ * GMFA_E_AGAIN when the class's ring is full (poll, then submit again). */
+int gmfa_serve_submit(gmfa_serve_session *session, int32_t cls, const uint8_t *codes,
+                      int32_t qlen, int32_t two_bit, int64_t t_off, int32_t tlen,
+                      int32_t band_offset, int64_t *ticket);
+
+/* Completed requests, at most max_results, their CIGAR runs packed into the
+ * host buffer cigar (cigar_cap runs); each returned request frees its ring
+ * entry.  Never blocks.  *n_results receives the count. */
+int gmfa_serve_poll(gmfa_serve_session *session, gmfa_serve_result *results,
+                    int32_t max_results, uint32_t *cigar, int64_t cigar_cap,
+                    int32_t *n_results);
+
+int gmfa_serve_session_stats(const gmfa_serve_session *session, gmfa_serve_stats *stats);
+
+/* Close the queues, wait for every submitted request to finish (results not
+ * yet polled are dropped) and free the session.  report->completed counts the
+ * requests completed during the session. */
+int gmfa_serve_close(gmfa_serve_session *session, gmfa_serve_report *report);
 
 #ifdef __cplusplus
 }

@@ -148,3 +148,121 @@ def test_serving_rejects_bad_inputs(mix):
     with pytest.raises(ValueError):
         S.run([c], timeout_s=5.0)
     torch.cuda.synchronize()
+
+
+def _online_classes(pools, slots, rings, cap=None):
+    from graphmambaformer_amd import serving as S
+
+    return [S.OnlineClass(half_band=hb, max_qlen=max(len(q) for q in qs), n_slots=n, ring=r,
+                          cigar_cap=cap)
+            for (_, hb, qs, _, _, _), n, r in zip(pools, slots, rings)]
+
+
+def _check_online(results, tickets, exp, label):
+    for r in results:
+        k = tickets[(r.cls, r.ticket)]
+        e = exp[r.cls][k]
+        ctx = (label, r.cls, r.ticket, k)
+        assert not r.truncated, ctx
+        assert r.score == e["score"], ctx
+        assert r.query_end == e["read_end"] and r.target_end == e["ref_end"], ctx
+        assert r.cigar == e["cigar"], ctx
+        assert r.read_start == e["read_start"] and r.ref_start == e["ref_start"], ctx
+
+
+def test_online_session_matches_oracle(mix):
+    """Requests submitted one at a time into small rings (ring-full waits and
+    wrap-around), a pause longer than the idle timeout (every wave exits; the
+    next submit relaunches the class), and every result bit-identical to the
+    oracle's."""
+    import time
+
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    rng = np.random.default_rng(5)
+    classes = _online_classes(pools, slots=(8, 8, 4), rings=(16, 8, 4))
+    _say("case: online session, 3 classes, rings 16/8/4, idle 0.05 s")
+    tickets, results = {}, []
+    sess = S.ServeSession(ref_dev, classes, SC, idle_timeout_s=0.05)
+    try:
+        for phase, n in enumerate((150, 40)):
+            order = rng.integers(0, 3, size=n)
+            for c in order:
+                _, hb, qs, pos, lens, offs = pools[c]
+                k = int(rng.integers(0, len(qs)))
+                t = sess.submit(int(c), qs[k], int(pos[k]), int(lens[k]), int(offs[k]))
+                tickets[(int(c), t)] = k
+                results += sess.poll()
+            results += sess.wait(len(tickets) - len(results), timeout_s=60.0)
+            if phase == 0:
+                time.sleep(0.3)  # every wave idles out
+        st = sess.stats()
+        _say("online stats", st)
+        assert st["submitted"] == len(tickets) == st["returned"] and st["in_flight"] == 0
+        assert st["launches"] > len(classes)  # relaunched after the pause
+    finally:
+        rep = sess.close()
+    assert rep.completed == len(tickets)
+    assert len(results) == len(tickets)
+    assert len({(r.cls, r.ticket) for r in results}) == len(tickets)
+    _check_online(results, tickets, exp, "online")
+
+
+def test_online_session_truncation_and_close_drain(mix):
+    """A CIGAR longer than cigar_cap comes back flagged, without runs, and the
+    rest of the record intact; close() drains requests never polled."""
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    _say("case: online session, cigar_cap 3, close with unpolled requests")
+    classes = _online_classes(pools[:1], slots=(4,), rings=(64,), cap=3)
+    sess = S.ServeSession(ref_dev, classes, SC, idle_timeout_s=1.0)
+    _, hb, qs, pos, lens, offs = pools[0]
+    tickets = {}
+    try:
+        for k in range(len(qs)):
+            tickets[(0, sess.submit(0, qs[k], int(pos[k]), int(lens[k]), int(offs[k])))] = k
+        got = sess.wait(len(qs), timeout_s=60.0)
+        for r in got:
+            e = exp[0][tickets[(0, r.ticket)]]
+            assert r.truncated == (len(e["cigar"]) > 3), (r.ticket, e["cigar"])
+            assert r.score == e["score"] and r.read_start == e["read_start"]
+            if not r.truncated:
+                assert r.cigar == e["cigar"]
+            else:
+                assert r.cigar == []
+        for k in range(8):
+            sess.submit(0, qs[k], int(pos[k]), int(lens[k]), int(offs[k]))
+    finally:
+        rep = sess.close()
+    assert rep.completed == len(qs) + 8
+
+
+def test_online_session_rejects_bad_requests(mix):
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    _say("case: online session invalid requests")
+    classes = _online_classes(pools[:1], slots=(2,), rings=(4,))
+    with S.ServeSession(ref_dev, classes, SC, idle_timeout_s=1.0) as sess:
+        _, hb, qs, pos, lens, offs = pools[0]
+        q = qs[0]
+        with pytest.raises(ValueError):  # longer than the class's max_qlen
+            sess.submit(0, np.zeros(classes[0].max_qlen + 1, np.uint8), 0, 10, 0)
+        with pytest.raises(ValueError):  # window leaves the reference
+            sess.submit(0, q, ref_dev.n_bases - 5, 10, 0)
+        with pytest.raises(ValueError):  # code outside 0..3
+            sess.submit(0, np.full(10, 7, np.uint8), 0, 10, 0)
+        with pytest.raises(ValueError):  # an N without N masks
+            sess.submit(0, np.full(10, 5, np.uint8), 0, 10, 0, two_bit=False)
+        with pytest.raises(ValueError):
+            sess.submit(3, q, 0, 10, 0)  # no such class
+        t = sess.submit(0, q, int(pos[0]), int(lens[0]), int(offs[0]))
+        r = sess.wait(1)
+        assert r[0].ticket == t and r[0].score == exp[0][0]["score"]
+    with pytest.raises(RuntimeError):
+        sess.submit(0, q, 0, 10, 0)  # closed

@@ -147,3 +147,37 @@ def test_entry_points_reject_bad_arguments_without_a_device():
     assert L.gmfa_sw_xdrop(None, None, None, None, None, 1, 4, 0, 5000, None, None, None, None,
                            None) == E
     assert b"half_band" in L.gmfa_last_error()
+
+
+def test_serve_session_validates_before_touching_the_device():
+    """gmfa_serve_open / submit / poll / close reject bad arguments (no
+    session, bad class configuration) before any HIP call."""
+    import ctypes
+
+    from graphmambaformer_amd import _lib
+
+    L = _lib.lib()
+    E = _lib.GMFA_E_INVALID
+    h = ctypes.c_void_p()
+    cfg = (_lib.ServeClassCfg * 1)()
+    assert L.gmfa_serve_open(None, None, 100, cfg, 1, 1.0, ctypes.byref(h)) == E
+    ref = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails first
+    assert L.gmfa_serve_open(ref, None, 0, cfg, 1, 1.0, ctypes.byref(h)) == E
+    assert L.gmfa_serve_open(ref, None, 100, cfg, 0, 1.0, ctypes.byref(h)) == E
+    assert L.gmfa_serve_open(ref, None, 100, cfg, 1, 0.0, ctypes.byref(h)) == E
+    c = cfg[0]
+    c.max_qlen, c.ring, c.n_slots, c.cigar_cap = 100, 6, 4, 100  # ring not a power of two
+    assert L.gmfa_serve_open(ref, None, 100, cfg, 1, 1.0, ctypes.byref(h)) == E
+    assert b"power of two" in L.gmfa_last_error()
+    c.ring = 8  # tb / stage / head missing
+    assert L.gmfa_serve_open(ref, None, 100, cfg, 1, 1.0, ctypes.byref(h)) == E
+    c.tb = c.stage = c.head = 0x1000
+    c.params.flags = _lib.SW_NMASK  # N masks without a reference mask
+    assert L.gmfa_serve_open(ref, None, 100, cfg, 1, 1.0, ctypes.byref(h)) == E
+    assert h.value is None
+    tk = ctypes.c_int64()
+    assert L.gmfa_serve_submit(None, 0, None, 0, 1, 0, 0, 0, ctypes.byref(tk)) == E
+    n = ctypes.c_int32()
+    assert L.gmfa_serve_poll(None, None, 0, None, 0, ctypes.byref(n)) == E
+    assert L.gmfa_serve_close(None, None) == E
+    assert L.gmfa_serve_stage_words(100) == (100 + 15) // 16 + 2
