@@ -93,12 +93,12 @@ class ServeResultRec(ctypes.Structure):
     _fields_ = [("ticket", _i64), ("cls", _i32), ("status", _i32), ("score", _i32),
                 ("query_end", _i32), ("target_end", _i32), ("read_start", _i32),
                 ("ref_start", _i32), ("n_cigar", _i32), ("counts", _i32 * 4),
-                ("cigar_off", _i64)]
+                ("cigar_off", _i64), ("claim_tick", _i64), ("done_tick", _i64)]
 
 
 class ServeStats(ctypes.Structure):
     _fields_ = [("submitted", _i64), ("returned", _i64), ("in_flight", _i64),
-                ("launches", _i64)]
+                ("launches", _i64), ("ticks_per_s", ctypes.c_double)]
 
 
 # Every exported symbol with its ctypes signature (must match include/gmfa.h).
@@ -142,6 +142,8 @@ SIGNATURES = {
                                        ctypes.c_double, ctypes.POINTER(_P)]),
     "gmfa_serve_submit": (ctypes.c_int, [_P, _i32, _P, _i32, _i32, _i64, _i32, _i32,
                                          ctypes.POINTER(_i64)]),
+    "gmfa_serve_submit_many": (ctypes.c_int, [_P, _i32, _P, _P, _P, _P, _i32, _P, _P, _P, _P,
+                                              ctypes.POINTER(_i32)]),
     "gmfa_serve_poll": (ctypes.c_int, [_P, ctypes.POINTER(ServeResultRec), _i32, _P, _i64,
                                        ctypes.POINTER(_i32)]),
     "gmfa_serve_session_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServeStats)]),

@@ -11,11 +11,13 @@
 // request (continuous batching), and classes run side by side on the CUs.
 //
 // Control words live in host-coherent pinned memory: the queue tails and
-// "closed" flags are written by this host thread and read by the device with
-// system-scope acquire loads; completion flags are written by the device
-// with system-scope release stores and polled here.  Every device wait is
-// bounded (deadline in wall-clock ticks), and the host closes every queue
-// before it returns, so every wavefront exits.
+// "closed" flags are written by this host thread; on the device, idle
+// wavefronts poll device-memory mirrors of them, which one wavefront at a time
+// (at most once per microsecond) refreshes with relaxed system-scope loads.
+// Completion flags (and, for online sessions, completion-queue records) are
+// written by the device with system-scope release stores and polled here.
+// Every device wait is bounded (deadline in wall-clock ticks), and the host
+// closes every queue before it returns, so every wavefront exits.
 #include "common.h"
 #include "sw_internal.h"
 
@@ -33,9 +35,18 @@ struct alignas(128) CtlWord {
     int32_t pad[31];
 };
 
-struct ClassCtl {
-    CtlWord tail, closed, gave_up;
+struct alignas(128) CtlWord64 {
+    int64_t v;
+    int64_t pad[15];
 };
+
+struct ClassCtl {
+    CtlWord64 tail;
+    CtlWord closed, gave_up;
+};
+
+// least interval between two reads of a class's host words by its wavefronts
+constexpr double kPollSeconds = 1e-6;
 
 }  // namespace
 }  // namespace gmfa
@@ -126,6 +137,8 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
         sl.gave_up = &ctl_d[c].gave_up.v;
         sl.slab_words = k.slab_words;
         sl.deadline_ticks = (uint64_t)((timeout_s + 5.0) * (double)khz * 1000.0);
+        sl.poll_ticks = (uint64_t)(kPollSeconds * (double)khz * 1000.0) + 1;
+        sl.dctl = reinterpret_cast<ServeDevCtl *>(k.out.work_counter);
         sl.n_slots = k.n_slots;
         const int rc = sw_launch(&k.batch, &k.params, &k.out, &sl, streams[c]);
         if (rc != GMFA_OK) return finish(rc);
@@ -150,7 +163,7 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
             while (x < k.batch.n_pairs && k.arrival_s[x] <= now) ++x;
             if (x != next[c]) {
                 next[c] = x;
-                __atomic_store_n(&ctl[c].tail.v, (int32_t)x, __ATOMIC_RELEASE);
+                __atomic_store_n(&ctl[c].tail.v, x, __ATOMIC_RELEASE);
             }
         }
         for (int c = 0; c < n_classes; ++c) {
@@ -185,11 +198,6 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
 namespace gmfa {
 namespace {
 
-struct alignas(128) CtlWord64 {
-    int64_t v;
-    int64_t pad[15];
-};
-
 struct OnlineCtl {
     CtlWord64 tail;
     CtlWord closed, gave_up;
@@ -207,6 +215,8 @@ struct OnlineClass {
     ServeRes *res = nullptr;
     uint32_t *cig = nullptr;
     int64_t *done = nullptr;
+    uint64_t *cq = nullptr;
+    int64_t cq_head = 0;             // next completion-queue position to read
     ServeLaunch sl{};
     gmfa_sw_batch batch{};
     gmfa_sw_out out{};  // only tb: the outputs go to the ring
@@ -232,12 +242,17 @@ struct gmfa_serve_session {
     uint64_t idle_ticks = 0;
     int device = 0;
     int64_t submitted = 0, returned = 0, completed = 0, launches = 0;
+    double ticks_per_s = 0.0;
 };
 
 namespace gmfa {
 namespace {
 
-__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
+__global__ void zero_ctl_kernel(ServeDevCtl *d)
+{
+    unsigned long long *w = reinterpret_cast<unsigned long long *>(d);
+    if (threadIdx.x < sizeof(ServeDevCtl) / 8) w[threadIdx.x] = 0ull;
+}
 
 int launch_class(gmfa_serve_session *s, OnlineClass &c)
 {
@@ -336,6 +351,7 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
     s->ref_mask = ref_mask;
     s->ref_bases = ref_bases;
     s->idle_ticks = (uint64_t)(idle_timeout_s * (double)khz * 1000.0);
+    s->ticks_per_s = (double)khz * 1000.0;
     s->device = dev;
     s->cls.resize(n_classes);
     for (int ci = 0; ci < n_classes; ++ci) {
@@ -352,7 +368,8 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
         const size_t b_res = sizeof(ServeRes) * ring;
         const size_t b_cig = (sizeof(uint32_t) * (size_t)k.cigar_cap * ring + 127) & ~(size_t)127;
         const size_t b_done = sizeof(int64_t) * ring;
-        const size_t bytes = b_ctl + b_desc + b_q + b_res + b_cig + b_done;
+        const size_t b_cq = sizeof(uint64_t) * ring;
+        const size_t bytes = b_ctl + b_desc + b_q + b_res + b_cig + b_done + b_cq;
         void *host = nullptr, *devp = nullptr;
         if (hipHostMalloc(&host, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
             gmfa::free_session(s);
@@ -373,6 +390,7 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
         c.res = (ServeRes *)(h + b_ctl + b_desc + b_q);
         c.cig = (uint32_t *)(h + b_ctl + b_desc + b_q + b_res);
         c.done = (int64_t *)(h + b_ctl + b_desc + b_q + b_res + b_cig);
+        c.cq = (uint64_t *)(h + b_ctl + b_desc + b_q + b_res + b_cig + b_done);
 
         c.batch = gmfa_sw_batch{};
         c.batch.max_qlen = k.max_qlen;
@@ -393,15 +411,17 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
         sl.gave_up = &dev_ptr(host, devp, c.ctl)->gave_up.v;
         sl.slab_words = k.slab_words;
         sl.deadline_ticks = s->idle_ticks;
+        sl.poll_ticks = (uint64_t)(kPollSeconds * (double)khz * 1000.0) + 1;
+        sl.dctl = reinterpret_cast<ServeDevCtl *>(k.head);
         sl.n_slots = k.n_slots;
         sl.ring_n = k.ring;
         sl.ring_tail = &dev_ptr(host, devp, c.ctl)->tail.v;
-        sl.head = (unsigned long long *)k.head;
         sl.ring_desc = dev_ptr(host, devp, c.desc);
         sl.ring_q = dev_ptr(host, devp, c.q);
         sl.ring_res = dev_ptr(host, devp, c.res);
         sl.ring_cigar = dev_ptr(host, devp, c.cig);
         sl.ring_done = dev_ptr(host, devp, c.done);
+        sl.ring_cq = dev_ptr(host, devp, c.cq);
         sl.stage = k.stage;
         sl.stage_words = c.stage_words;
         sl.cigar_cap = k.cigar_cap;
@@ -412,10 +432,9 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
             set_error("hipStreamCreateWithFlags failed");
             return GMFA_E_RUNTIME;
         }
-        // the head starts at ticket 0 (a kernel, not a memset: no memset nodes
-        // on a stream a caller might capture)
-        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, c.stream,
-                           (unsigned long long *)k.head);
+        // the head starts at ticket 0, the mirrors at 0 (a kernel, not a
+        // memset: no memset nodes on a stream a caller might capture)
+        hipLaunchKernelGGL(zero_ctl_kernel, dim3(1), dim3(64), 0, c.stream, sl.dctl);
         if (hipGetLastError() != hipSuccess) {
             gmfa::free_session(s);
             set_error("head reset launch failed");
@@ -433,11 +452,35 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
     return GMFA_OK;
 }
 
-extern "C" int gmfa_serve_submit(gmfa_serve_session *s, int32_t cls, const uint8_t *codes,
-                                 int32_t qlen, int32_t two_bit, int64_t t_off, int32_t tlen,
-                                 int32_t band_offset, int64_t *ticket)
+namespace gmfa {
+namespace {
+
+// 16 codes 0..3 (one per byte) -> one 2-bit word, base k at bits 2k..2k+1,
+// eight bytes at a time; bad != 0 if a byte is above 3.
+inline uint32_t pack16_2bit(const uint8_t *p, uint32_t &bad)
 {
-    GMFA_REQUIRE(s && ticket, "NULL argument");
+    uint64_t w[2];
+    std::memcpy(w, p, 16);
+    uint32_t out = 0u;
+    uint64_t hi = 0u;
+    for (int h = 0; h < 2; ++h) {
+        uint64_t x = w[h];
+        hi |= x & 0xFCFCFCFCFCFCFCFCull;
+        x = (x | (x >> 6)) & 0x000F000F000F000Full;   // byte pairs -> 4-bit nibbles
+        x = (x | (x >> 12)) & 0x000000FF000000FFull;  // nibble pairs -> bytes
+        x = (x | (x >> 24)) & 0xFFFFull;              // two bytes -> 16 bits
+        out |= (uint32_t)x << (16 * h);
+    }
+    bad = hi != 0u;
+    return out;
+}
+
+// Write one request into the next ring entry of its class (not yet published:
+// the caller advances the class's tail afterwards).
+int stage_request(gmfa_serve_session *s, int32_t cls, const uint8_t *codes, int32_t qlen,
+                  int32_t two_bit, int64_t t_off, int32_t tlen, int32_t band_offset,
+                  int64_t *ticket)
+{
     GMFA_REQUIRE(cls >= 0 && cls < (int)s->cls.size(), "class %d outside [0, %d)", cls,
                  (int)s->cls.size());
     OnlineClass &c = s->cls[cls];
@@ -450,8 +493,7 @@ extern "C" int gmfa_serve_submit(gmfa_serve_session *s, int32_t cls, const uint8
     GMFA_REQUIRE(band_offset >= -(1 << 30) && band_offset <= (1 << 30),
                  "band_offset %d outside [-2^30, 2^30]", band_offset);
     const int64_t t = c.next;
-    const bool was_empty = c.lo == c.next;
-    const int r = (int)(t % c.cfg.ring);
+    const int r = (int)(t & (c.cfg.ring - 1));
     if (c.occupant[r] >= 0) {
         set_error("class %d: ring of %d requests is full (poll, then submit again)", cls,
                   c.cfg.ring);
@@ -462,24 +504,31 @@ extern "C" int gmfa_serve_submit(gmfa_serve_session *s, int32_t cls, const uint8
     uint32_t *qw = c.q + (size_t)r * 2 * c.stage_words;
     uint32_t *mw = qw + c.stage_words;
     for (int w = 0; w < nw; ++w) {
-        uint32_t v = 0u, m = 0u;
+        uint32_t v = 0u, m = 0u, bad = 0u;
         const int b0 = 16 * w, b1 = std::min(qlen, b0 + 16);
-        for (int b = b0; b < b1; ++b) {
-            uint32_t x = codes[b];
-            uint32_t n = 0u;
-            if (two_bit) {
-                GMFA_REQUIRE(x <= 3u, "code %u at %d outside 0..3", x, b);
-            } else {
+        if (two_bit && b1 - b0 == 16) {
+            v = pack16_2bit(codes + b0, bad);
+            GMFA_REQUIRE(bad == 0u, "a code outside 0..3 in bases %d..%d", b0, b1 - 1);
+        } else if (two_bit) {
+            for (int b = b0; b < b1; ++b) {
+                const uint32_t x = codes[b];
+                bad |= x & ~3u;
+                v |= (x & 3u) << (2 * (b - b0));
+            }
+            GMFA_REQUIRE(bad == 0u, "a code outside 0..3 in bases %d..%d", b0, b1 - 1);
+        } else {
+            for (int b = b0; b < b1; ++b) {
+                uint32_t x = codes[b];
                 GMFA_REQUIRE(x >= 1u && x <= 5u, "code %u at %d outside 1..5", x, b);
+                uint32_t n = 0u;
                 if (x == 5u) {
                     GMFA_REQUIRE(c.nm, "an N (code 5 at %d) needs a GMFA_SW_NMASK class", b);
                     x = 1u;
                     n = 1u;
                 }
-                x -= 1u;
+                v |= (x - 1u) << (2 * (b - b0));
+                m |= n << (2 * (b - b0));
             }
-            v |= x << (2 * (b - b0));
-            m |= n << (2 * (b - b0));
         }
         qw[w] = v;
         if (c.nm) mw[w] = m;
@@ -492,10 +541,69 @@ extern "C" int gmfa_serve_submit(gmfa_serve_session *s, int32_t cls, const uint8
     d.q_words = nw;
     c.occupant[r] = t;
     c.next = t + 1;
-    __atomic_store_n(&c.ctl->tail.v, c.next, __ATOMIC_RELEASE);
     ++s->submitted;
     *ticket = t;
+    return GMFA_OK;
+}
+
+int publish(gmfa_serve_session *s, OnlineClass &c, bool was_empty)
+{
+    __atomic_store_n(&c.ctl->tail.v, c.next, __ATOMIC_RELEASE);
     return ensure_running(s, c, was_empty);
+}
+
+}  // namespace
+}  // namespace gmfa
+
+extern "C" int gmfa_serve_submit(gmfa_serve_session *s, int32_t cls, const uint8_t *codes,
+                                 int32_t qlen, int32_t two_bit, int64_t t_off, int32_t tlen,
+                                 int32_t band_offset, int64_t *ticket)
+{
+    GMFA_REQUIRE(s && ticket, "NULL argument");
+    GMFA_REQUIRE(cls >= 0 && cls < (int)s->cls.size(), "class %d outside [0, %d)", cls,
+                 (int)s->cls.size());
+    OnlineClass &c = s->cls[cls];
+    const bool was_empty = c.lo == c.next;
+    const int rc = stage_request(s, cls, codes, qlen, two_bit, t_off, tlen, band_offset, ticket);
+    if (rc != GMFA_OK) return rc;
+    return publish(s, c, was_empty);
+}
+
+extern "C" int gmfa_serve_submit_many(gmfa_serve_session *s, int32_t n, const int32_t *cls,
+                                      const uint8_t *codes, const int64_t *code_off,
+                                      const int32_t *qlen, int32_t two_bit, const int64_t *t_off,
+                                      const int32_t *tlen, const int32_t *band_offset,
+                                      int64_t *tickets, int32_t *n_submitted)
+{
+    GMFA_REQUIRE(s && n_submitted, "NULL argument");
+    *n_submitted = 0;
+    GMFA_REQUIRE(n >= 0, "negative n");
+    if (n == 0) return GMFA_OK;
+    GMFA_REQUIRE(cls && code_off && qlen && t_off && tlen && band_offset && tickets,
+                 "NULL request array");
+    const int nc = (int)s->cls.size();
+    std::vector<char> touched(nc, 0), was_empty(nc, 0);
+    for (int c = 0; c < nc; ++c) was_empty[c] = s->cls[c].lo == s->cls[c].next;
+    int rc = GMFA_OK, k = 0;
+    for (; k < n; ++k) {
+        if (code_off[k] < 0) {
+            set_error("request %d: negative code offset", k);
+            rc = GMFA_E_INVALID;
+            break;
+        }
+        rc = stage_request(s, cls[k], codes ? codes + code_off[k] : nullptr, qlen[k], two_bit,
+                           t_off[k], tlen[k], band_offset[k], &tickets[k]);
+        if (rc != GMFA_OK) break;
+        touched[cls[k]] = 1;
+    }
+    *n_submitted = k;
+    // one tail store per class for the whole burst
+    for (int c = 0; c < nc; ++c) {
+        if (!touched[c]) continue;
+        const int prc = publish(s, s->cls[c], was_empty[c] != 0);
+        if (prc != GMFA_OK && rc == GMFA_OK) rc = prc;
+    }
+    return rc;
 }
 
 extern "C" int gmfa_serve_poll(gmfa_serve_session *s, gmfa_serve_result *results,
@@ -511,15 +619,24 @@ extern "C" int gmfa_serve_poll(gmfa_serve_session *s, gmfa_serve_result *results
     for (int ci = 0; ci < (int)s->cls.size(); ++ci) {
         OnlineClass &c = s->cls[ci];
         const int32_t ring = c.cfg.ring, cap = c.cfg.cigar_cap;
-        for (int64_t t = c.lo; t < c.next && n < max_results; ++t) {
-            const int r = (int)(t % ring);
-            if (c.occupant[r] != t) continue;
-            if (__atomic_load_n(&c.done[r], __ATOMIC_ACQUIRE) != t + 1) continue;
+        // completions in the order the device recorded them; a record not yet
+        // written (its stamp is not the expected position) ends the scan
+        while (n < max_results) {
+            const uint64_t v = __atomic_load_n(&c.cq[c.cq_head & (ring - 1)], __ATOMIC_ACQUIRE);
+            if ((uint32_t)(v >> 32) != (uint32_t)(c.cq_head + 1)) break;
+            const uint32_t r = (uint32_t)v;
+            const int64_t t = r < (uint32_t)ring ? c.occupant[r] : -1;
+            if (t < 0) {
+                set_error("class %d: corrupt completion record %llx", ci, (unsigned long long)v);
+                *n_results = n;
+                return GMFA_E_RUNTIME;
+            }
             const ServeRes &e = c.res[r];
             const int nc = e.n_cigar;
             const bool trunc = nc > cap;
             const int keep = trunc ? 0 : nc;
             if (coff + keep > cigar_cap) break;
+            ++c.cq_head;
             gmfa_serve_result &o = results[n++];
             o.ticket = t;
             o.cls = ci;
@@ -532,6 +649,8 @@ extern "C" int gmfa_serve_poll(gmfa_serve_session *s, gmfa_serve_result *results
             o.n_cigar = nc;
             for (int k = 0; k < 4; ++k) o.counts[k] = e.counts[k];
             o.cigar_off = coff;
+            o.claim_tick = e.claim_tick;
+            o.done_tick = e.done_tick;
             if (keep)
                 std::memcpy(cigar + coff, c.cig + (size_t)r * cap + (cap - keep),
                             sizeof(uint32_t) * (size_t)keep);
@@ -558,6 +677,7 @@ extern "C" int gmfa_serve_session_stats(const gmfa_serve_session *s, gmfa_serve_
     st->returned = s->returned;
     st->in_flight = s->submitted - s->returned;
     st->launches = s->launches;
+    st->ticks_per_s = s->ticks_per_s;
     return GMFA_OK;
 }
 

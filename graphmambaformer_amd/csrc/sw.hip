@@ -84,59 +84,26 @@ struct SwKArgs {
     // continuous serving (serve_tail == nullptr: one offline batch).  Request
     // k may be claimed once the host has published k < *serve_tail; the
     // flags live in host-coherent memory.
-    const int32_t *serve_tail;    // requests published so far
+    const int64_t *serve_tail;    // requests published so far
     const int32_t *serve_closed;  // nonzero: nothing more will be published
     int32_t *serve_done;          // [n_pairs] set to 1 when request k is complete
     int32_t *serve_gave_up;       // set to 1 if a wave's wait hit deadline_ticks
     int64_t slab_words;           // traceback plane words per wave slot
     uint64_t deadline_ticks;      // bound of one idle wait (wall-clock ticks)
+    uint64_t poll_ticks;          // least interval between two reads of the host words
+    ServeDevCtl *dctl;            // device mirrors of the host words (serving)
     // online serving (ring_n > 0): requests arrive in a ring in host-coherent
     // memory while the kernel runs (fields as in ServeLaunch)
     int32_t ring_n, stage_words, cigar_cap;
     const int64_t *ring_tail;
-    unsigned long long *head;
     const ServeDesc *ring_desc;
     const uint32_t *ring_q;
     ServeRes *ring_res;
     uint32_t *ring_cigar;
     int64_t *ring_done;
+    uint64_t *ring_cq;
     uint32_t *stage;
 };
-
-// Wait until request `item` is published or the host closes the queue; returns
-// the number of published requests.  Every wait ends: at the latest after
-// deadline_ticks of the constant-rate wall clock (s_sleep 127 = 8128 cycles).
-// The loads are relaxed (system scope, so they reach host memory, but with no
-// acquire): the request table and sequences are written before the launch,
-// only the tail value itself travels while the kernel runs, and an acquire
-// would invalidate the L2 under every working wavefront at each poll.  (Scalars by value: taking
-// the kernel argument block by reference would move it to scratch memory.)
-__device__ __forceinline__ int serve_wait(const int32_t *tail_p, const int32_t *closed_p,
-                                          int32_t *gave_up, uint64_t deadline_ticks, int item)
-{
-    const uint64_t t0 = (uint64_t)wall_clock64();
-    // polls cross the host link: back off to one poll per ~50 us while idle
-    int nap = 1;
-    for (;;) {
-        const int tail = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(tail_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        if (item < tail) return tail;
-        const int closed = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(closed_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        if (closed) {
-            // the host publishes the final tail before closing
-            return __builtin_amdgcn_readfirstlane(
-                __hip_atomic_load(tail_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        }
-        if ((uint64_t)wall_clock64() - t0 > deadline_ticks) {
-            if (threadIdx.x == 0)
-                __hip_atomic_store(gave_up, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            return tail;
-        }
-        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(127);
-        nap = nap < 16 ? 2 * nap : 16;
-    }
-}
 
 __device__ __forceinline__ int64_t uni64(int64_t v)
 {
@@ -145,33 +112,96 @@ __device__ __forceinline__ int64_t uni64(int64_t v)
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Online serving: claim the next published, unclaimed ticket.  A wave claims
-// only a ticket the host has already published (compare-and-swap on the
-// device head below the host tail), so a wave that stops waiting holds
-// nothing and the host can relaunch the class later without losing a
-// request.  Returns false once the queue is closed and drained, or after
-// idle_ticks of the constant-rate wall clock without a claim (the wave then
-// exits; the host relaunches the class when requests are pending and every
-// wave has gone).
-__device__ __forceinline__ bool serve_claim(unsigned long long *head_p, const int64_t *tail_p,
-                                            const int32_t *closed_p, uint64_t idle_ticks,
-                                            int64_t &ticket)
+// The freshest known tail of a serving queue, and whether the host closed it.
+// Reads the device mirrors; when they are older than poll_ticks, the one
+// wavefront that wins the compare-and-swap on last_poll reads the host words
+// (relaxed, system scope: only the values themselves travel; the requests
+// were written before the tail) and raises the mirrors for everyone.  The
+// host publishes its final tail before closing, and the mirror tail is raised
+// before the mirror closed flag (release / acquire at device scope), so a
+// wavefront that sees closed also sees the final tail.
+__device__ __forceinline__ int64_t doorbell(ServeDevCtl *d, const int64_t *host_tail,
+                                            const int32_t *host_closed, uint64_t poll_ticks,
+                                            int &closed)
+{
+    const bool l0 = (threadIdx.x & 63) == 0;
+    closed = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(&d->closed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (closed) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    int64_t tail = uni64((int64_t)__hip_atomic_load(&d->tail, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT));
+    if (closed) return tail;
+    const uint64_t now = (uint64_t)wall_clock64();
+    const uint64_t last = (uint64_t)uni64((int64_t)__hip_atomic_load(
+        &d->last_poll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (now - last < poll_ticks) return tail;
+    unsigned long long won = 0ull;
+    if (l0) won = atomicCAS(&d->last_poll, (unsigned long long)last, (unsigned long long)now);
+    if ((uint64_t)uni64((int64_t)won) != last) return tail;  // another wavefront polls this time
+    const int hc = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(host_closed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+    if (hc) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the final tail is older
+    const int64_t ht =
+        uni64(__hip_atomic_load(host_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+    if (ht > tail) {
+        if (l0) atomicMax(&d->tail, (unsigned long long)ht);
+        tail = ht;
+    }
+    if (hc) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (l0) __hip_atomic_store(&d->closed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        closed = 1;
+    }
+    return tail;
+}
+
+// Staged serving: wait until request `item` is published or the queue is
+// closed; returns the number of published requests.  Every wait ends: at the
+// latest after deadline_ticks of the constant-rate wall clock (s_sleep 127 =
+// 8128 cycles).  (Scalars by value: taking the kernel argument block by
+// reference would move it to scratch memory.)
+__device__ __forceinline__ int serve_wait(ServeDevCtl *d, const int64_t *tail_p,
+                                          const int32_t *closed_p, int32_t *gave_up,
+                                          uint64_t deadline_ticks, uint64_t poll_ticks, int item)
 {
     const uint64_t t0 = (uint64_t)wall_clock64();
     int nap = 1;
     for (;;) {
-        // closed before tail: the host publishes its last tail, then closes;
-        // once closed is seen, the acquire orders the tail load after it
-        const int closed = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(closed_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        if (closed) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
-        const int64_t tail =
-            uni64(__hip_atomic_load(tail_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        int64_t head = uni64((int64_t)__hip_atomic_load(head_p, __ATOMIC_RELAXED,
+        int closed = 0;
+        const int64_t tail = doorbell(d, tail_p, closed_p, poll_ticks, closed);
+        const int n = (int)(tail < (int64_t)INT_MAX ? tail : (int64_t)INT_MAX);
+        if (item < n || closed) return n;
+        if ((uint64_t)wall_clock64() - t0 > deadline_ticks) {
+            if (threadIdx.x == 0)
+                __hip_atomic_store(gave_up, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            return n;
+        }
+        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(127);
+        nap = nap < 16 ? 2 * nap : 16;
+    }
+}
+
+// Online serving: claim the next published, unclaimed ticket.  A wave claims
+// only a ticket the host has already published (compare-and-swap on the
+// device head below the tail), so a wave that stops waiting holds nothing and
+// the host can relaunch the class later without losing a request.  Returns
+// false once the queue is closed and drained, or after idle_ticks of the
+// constant-rate wall clock without a claim (the wave then exits; the host
+// relaunches the class when requests are pending and every wave has gone).
+__device__ __forceinline__ bool serve_claim(ServeDevCtl *d, const int64_t *tail_p,
+                                            const int32_t *closed_p, uint64_t idle_ticks,
+                                            uint64_t poll_ticks, int64_t &ticket)
+{
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    int nap = 1;
+    for (;;) {
+        int closed = 0;
+        const int64_t tail = doorbell(d, tail_p, closed_p, poll_ticks, closed);
+        int64_t head = uni64((int64_t)__hip_atomic_load(&d->head, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT));
         while (head < tail) {
             unsigned long long got = 0ull;
-            if ((threadIdx.x & 63) == 0) got = atomicCAS(head_p, (unsigned long long)head,
+            if ((threadIdx.x & 63) == 0) got = atomicCAS(&d->head, (unsigned long long)head,
                                                          (unsigned long long)head + 1ull);
             const int64_t g = uni64((int64_t)got);
             if (g == head) {
@@ -407,7 +437,8 @@ __device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, in
 template <int C>
 __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, int cap,
                                                ServeRes *res, int i, int j, int h, int off,
-                                               int64_t qo, int64_t to, const uint32_t *tb)
+                                               int64_t qo, int64_t to, const uint32_t *tb,
+                                               uint64_t claim_tick)
 {
     int lane = (int)(threadIdx.x & 63);
     asm volatile("" : "+v"(lane));  // keep lane-derived addresses inside the request loop
@@ -430,10 +461,13 @@ __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, 
         const int y = cap - 1 - ((nr & ~63) + lane);
         if (y >= 0) cig[y] = buf;
     }
+    const uint64_t done_tick = (uint64_t)wall_clock64();
     const int v = lane == 0 ? score : lane == 1 ? qend : lane == 2 ? tend : lane == 3 ? i
                 : lane == 4 ? j : lane == 5 ? nr : lane == 6 ? cnt[0] : lane == 7 ? cnt[1]
-                : lane == 8 ? cnt[2] : cnt[3];
-    if (lane < 10) reinterpret_cast<int32_t *>(res)[lane] = v;
+                : lane == 8 ? cnt[2] : lane == 9 ? cnt[3]
+                : lane == 10 ? (int)(uint32_t)claim_tick : lane == 11 ? (int)(claim_tick >> 32)
+                : lane == 12 ? (int)(uint32_t)done_tick : (int)(done_tick >> 32);
+    if (lane < 14) reinterpret_cast<int32_t *>(res)[lane] = v;
 }
 
 struct TbKArgs {
@@ -519,17 +553,25 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         int ring_r = 0;
         int r_qlen = 0, r_tlen = 0, r_off = 0;
         int64_t r_to = 0;
+        uint64_t claim_tick = 0;
         if (online) {
-            if (!serve_claim(a.head, a.ring_tail, a.serve_closed, a.deadline_ticks, ticket)) break;
-            // the host wrote the entry before publishing the ticket
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+            if (!serve_claim(a.dctl, a.ring_tail, a.serve_closed, a.deadline_ticks, a.poll_ticks,
+                             ticket))
+                break;
+            claim_tick = (uint64_t)wall_clock64();
+            // The host wrote the entry before publishing the ticket (release).
+            // The entry is read with system-scope loads issued after the
+            // claim's tail load has returned: an acquire would add an L2
+            // invalidation per request under every working wavefront, which
+            // only non-coherent loads need.
             ring_r = (int)(ticket & (int64_t)(a.ring_n - 1));  // ring_n is a power of two
             // lane index through an opaque copy: nothing derived from it is
             // hoisted out of the request loop to stay live across the DP
             int ln = lane;
             asm volatile("" : "+v"(ln));
             const uint32_t *dw = reinterpret_cast<const uint32_t *>(a.ring_desc + ring_r);
-            const uint32_t dv = ln < 6 ? dw[ln] : 0u;
+            const uint32_t dv =
+                ln < 6 ? __hip_atomic_load(dw + ln, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0u;
             r_to = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dv, 1) << 32) |
                              (uint32_t)__builtin_amdgcn_readlane((int)dv, 0));
             r_qlen = __builtin_amdgcn_readlane((int)dv, 2);
@@ -541,9 +583,13 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             const int64_t sw2 = 2 * (int64_t)a.stage_words;
             uint32_t *stg = a.stage + (int64_t)blockIdx.x * sw2;
             const uint32_t *src = a.ring_q + (int64_t)ring_r * sw2;
-            for (int x = ln; x < nw; x += 64) stg[x] = src[x];
+            for (int x = ln; x < nw; x += 64)
+                stg[x] = __hip_atomic_load(src + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             if (NM)
-                for (int x = ln; x < nw; x += 64) stg[a.stage_words + x] = src[a.stage_words + x];
+                for (int x = ln; x < nw; x += 64)
+                    stg[a.stage_words + x] = __hip_atomic_load(src + a.stage_words + x,
+                                                               __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_SYSTEM);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         } else {
@@ -553,7 +599,8 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             // has published
             if constexpr (SV) {
                 if (item >= a.n_pairs) break;  // past the class's last request: nothing to wait for
-                n_items = serve_wait(a.serve_tail, a.serve_closed, a.serve_gave_up, a.deadline_ticks, item);
+                n_items = serve_wait(a.dctl, a.serve_tail, a.serve_closed, a.serve_gave_up,
+                                     a.deadline_ticks, a.poll_ticks, item);
             }
             if (item >= n_items) break;
         }
@@ -811,11 +858,18 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 walk_pair_ring<C>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap, a.cigar_cap,
                                   a.ring_res + ring_r, res_i, res_j, res_s, off, qo, to,
-                                  a.tb + tb_base);
-                // every output of the request (stored by this wave) before its done word
-                if (lane == 0)
+                                  a.tb + tb_base, claim_tick);
+                // every output of the request (stored by the wave's lanes) before
+                // its done word and its completion-queue record
+                __syncthreads();
+                if (lane == 0) {
                     __hip_atomic_store(a.ring_done + ring_r, ticket + 1, __ATOMIC_RELEASE,
                                        __HIP_MEMORY_SCOPE_SYSTEM);
+                    const unsigned long long pos = atomicAdd(&a.dctl->cq_pos, 1ull);
+                    __hip_atomic_store(a.ring_cq + (pos & (unsigned long long)(a.ring_n - 1)),
+                                       ((uint64_t)(pos + 1) << 32) | (uint32_t)ring_r,
+                                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
                 continue;
             }
         }
@@ -841,6 +895,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             }
         }
         if constexpr (SV) {
+            __syncthreads();  // the walk's stores by every lane, then the flag
             if (has && lane == 0) {
                 // every output of this pair (stored by this wave) before the flag
                 __hip_atomic_store(a.serve_done + p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -850,6 +905,14 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 }
 
 __global__ void reset_counter_kernel(uint32_t *c) { *c = 0u; }
+
+// zero a serving class's device control block (a kernel, not a memset: no
+// memset nodes on a stream a caller might capture)
+__global__ void reset_serve_ctl_kernel(ServeDevCtl *d)
+{
+    unsigned long long *w = reinterpret_cast<unsigned long long *>(d);
+    if (threadIdx.x < sizeof(ServeDevCtl) / 8) w[threadIdx.x] = 0ull;
+}
 
 // The same x-drop for groups of any size: one 256-thread workgroup per group,
 // each thread owning pairs p0 + tid, p0 + tid + 256, ...; per-pair state
@@ -1293,9 +1356,11 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
 
     a.serve_tail = nullptr; a.serve_closed = nullptr; a.serve_done = nullptr;
     a.serve_gave_up = nullptr; a.slab_words = 0; a.deadline_ticks = 0;
+    a.poll_ticks = 0; a.dctl = nullptr;
     a.ring_n = 0; a.stage_words = 0; a.cigar_cap = 0;
-    a.ring_tail = nullptr; a.head = nullptr; a.ring_desc = nullptr; a.ring_q = nullptr;
+    a.ring_tail = nullptr; a.ring_desc = nullptr; a.ring_q = nullptr;
     a.ring_res = nullptr; a.ring_cigar = nullptr; a.ring_done = nullptr; a.stage = nullptr;
+    a.ring_cq = nullptr;
     int waves = 1;
     if (!serve) {
         waves = std::max(1, std::min(a.n_items, resident_waves(k)));
@@ -1314,10 +1379,15 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
         a.serve_tail = serve->tail; a.serve_closed = serve->closed; a.serve_done = serve->done;
         a.serve_gave_up = serve->gave_up; a.slab_words = serve->slab_words;
         a.deadline_ticks = serve->deadline_ticks;
+        a.poll_ticks = serve->poll_ticks;
+        GMFA_REQUIRE(serve->dctl && ((uintptr_t)serve->dctl & 63) == 0,
+                     "serving needs 64 bytes of 64-byte aligned device control words");
+        a.dctl = serve->dctl;
         waves = serve->n_slots;
         if (online) {
-            GMFA_REQUIRE(serve->ring_tail && serve->head && serve->ring_desc && serve->ring_q &&
+            GMFA_REQUIRE(serve->ring_tail && serve->ring_desc && serve->ring_q &&
                              serve->ring_res && serve->ring_cigar && serve->ring_done &&
+                             serve->ring_cq &&
                              serve->stage && serve->closed,
                          "NULL online serving pointer");
             GMFA_REQUIRE(serve->stage_words >= serve_stage_words(batch->max_qlen),
@@ -1330,13 +1400,19 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                              (!nm || batch->qmask == serve->stage + serve->stage_words),
                          "online batch: qseq must be the stage array (qmask one stage slot on)");
             a.ring_n = serve->ring_n; a.stage_words = serve->stage_words;
-            a.cigar_cap = serve->cigar_cap; a.ring_tail = serve->ring_tail; a.head = serve->head;
+            a.cigar_cap = serve->cigar_cap; a.ring_tail = serve->ring_tail;
             a.ring_desc = serve->ring_desc; a.ring_q = serve->ring_q; a.ring_res = serve->ring_res;
             a.ring_cigar = serve->ring_cigar; a.ring_done = serve->ring_done;
+            a.ring_cq = serve->ring_cq;
             a.stage = serve->stage;
         }
     }
-    if (!online) {
+    if (serve && !online) {
+        // staged: the work counter is the low word of the control block
+        a.counter = reinterpret_cast<uint32_t *>(a.dctl);
+        hipLaunchKernelGGL(reset_serve_ctl_kernel, dim3(1), dim3(64), 0, s, a.dctl);
+        GMFA_CHECK_HIP(hipGetLastError());
+    } else if (!serve) {
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
         GMFA_CHECK_HIP(hipGetLastError());
     }

@@ -24,8 +24,27 @@ struct alignas(32) ServeDesc {
 struct alignas(64) ServeRes {
     int32_t score, query_end, target_end, read_start, ref_start, n_cigar;
     int32_t counts[4];
-    int32_t pad[6];
+    int64_t claim_tick, done_tick;  // device wall clock at the claim / the record
+    int32_t pad[2];
 };
+static_assert(sizeof(ServeRes) == 64 && sizeof(ServeDesc) == 32, "ring entry layouts");
+
+// Device-memory control words of one serving class (64 bytes of the caller's
+// device memory: gmfa_serve_class.out.work_counter / gmfa_serve_class_cfg.head).
+// Idle wavefronts poll these mirrors in device memory; at most one wavefront
+// per poll interval reads the host-coherent words over the host link and
+// raises the mirrors (a read of host memory by every idle wavefront clogs the
+// memory queues that the working wavefronts share).
+struct alignas(64) ServeDevCtl {
+    unsigned long long head;       // online: next ticket; staged: uint32 work counter (low word)
+    unsigned long long tail;       // mirror of the host tail
+    int32_t closed;                // mirror of the host closed flag
+    int32_t pad0;
+    unsigned long long last_poll;  // wall-clock tick of the last read of the host words
+    unsigned long long cq_pos;     // online: completions so far (completion queue position)
+    unsigned long long pad[3];
+};
+static_assert(sizeof(ServeDevCtl) == 64, "serving device control block");
 
 // Persistent serving launch.  Staged form (ring_n == 0): the forward kernel
 // claims request k only once k < *tail, walks it into a CIGAR in the same
@@ -35,22 +54,25 @@ struct alignas(64) ServeRes {
 // Pointers named "host-coherent" are device-visible pointers into pinned
 // memory that the host thread reads and writes while the kernel runs.
 struct ServeLaunch {
-    const int32_t *tail;      // staged: requests published (host-coherent)
+    const int64_t *tail;      // staged: requests published (host-coherent)
     const int32_t *closed;    // host-coherent: nothing more will be published
     int32_t *done;            // staged: [n_pairs] completion flags (host-coherent)
     int32_t *gave_up;         // host-coherent: a wave's idle wait hit its bound
     int64_t slab_words;       // traceback plane words per wave slot (out->tb holds n_slots slabs)
     uint64_t deadline_ticks;  // bound of one idle wait of a wave, wall-clock ticks
+    uint64_t poll_ticks;      // least interval between two reads of the host words
     int32_t n_slots;          // persistent wavefronts
     // online form
     int32_t ring_n;                  // ring entries (0: staged form)
     const int64_t *ring_tail;        // host-coherent: tickets published
-    unsigned long long *head;        // device: next ticket to claim
+    ServeDevCtl *dctl;               // device control words (staged: out->work_counter)
     const ServeDesc *ring_desc;      // host-coherent [ring_n]
     const uint32_t *ring_q;          // host-coherent [ring_n][2 * stage_words]
     ServeRes *ring_res;              // host-coherent [ring_n]
     uint32_t *ring_cigar;            // host-coherent [ring_n][cigar_cap]
     int64_t *ring_done;              // host-coherent [ring_n]: ticket + 1 when complete
+    uint64_t *ring_cq;               // host-coherent [ring_n] completion queue: completion k
+                                     // writes (k + 1) << 32 | entry at k mod ring_n
     uint32_t *stage;                 // device [n_slots][2 * stage_words]
     int32_t stage_words;             // query words per slot (mask words follow)
     int32_t cigar_cap;               // CIGAR runs kept per request

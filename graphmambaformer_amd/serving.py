@@ -169,7 +169,7 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
         qend=torch.zeros(n, dtype=torch.int32, device=dev),
         tend=torch.zeros(n, dtype=torch.int32, device=dev),
         tb=torch.empty(int(n_slots) * slab, dtype=torch.int32, device=dev),
-        counter=torch.zeros(1, dtype=torch.int32, device=dev), walk=walk, picks=picks)
+        counter=torch.zeros(16, dtype=torch.int32, device=dev), walk=walk, picks=picks)
 
 
 @dataclass
@@ -279,7 +279,7 @@ class ServeSession:
             sw = int(L.gmfa_serve_stage_words(int(oc.max_qlen)))
             tb = torch.empty(int(oc.n_slots) * slab, dtype=torch.int32, device=dev)
             stage = torch.zeros(int(oc.n_slots) * 2 * sw, dtype=torch.int32, device=dev)
-            head = torch.zeros(1, dtype=torch.int64, device=dev)
+            head = torch.zeros(8, dtype=torch.int64, device=dev)  # 64-byte control block
             self._keep += [tb, stage, head]
             c = cfgs[k]
             c.params = kernels.make_params(ma, mi, go, ge, int(oc.half_band),
@@ -318,6 +318,45 @@ class ServeSession:
             if time.monotonic() > deadline:
                 raise RuntimeError(f"class {cls}: ring still full after {wait_s} s")
 
+    def submit_many(self, cls, codes: np.ndarray, code_off, qlen, t_start, t_len, band_offset,
+                    two_bit: bool = True):
+        """Publish a burst of requests (gmfa_serve_submit_many); stops early
+        at a full ring.  Returns the tickets of the requests published (a
+        prefix of the burst)."""
+        n = len(qlen)
+        arrs = [np.ascontiguousarray(cls, np.int32), np.ascontiguousarray(code_off, np.int64),
+                np.ascontiguousarray(qlen, np.int32), np.ascontiguousarray(t_start, np.int64),
+                np.ascontiguousarray(t_len, np.int32), np.ascontiguousarray(band_offset, np.int32)]
+        codes = np.ascontiguousarray(codes, np.uint8)
+        tickets = np.empty(n, np.int64)
+        got = ctypes.c_int32()
+        c, co, ql, to, tl, bo = (a.ctypes.data for a in arrs)
+        rc = lib().gmfa_serve_submit_many(self._handle(), n, c, codes.ctypes.data, co, ql,
+                                          1 if two_bit else 0, to, tl, bo, tickets.ctypes.data,
+                                          ctypes.byref(got))
+        if rc != _lib.GMFA_E_AGAIN:
+            check(rc, "gmfa_serve_submit_many")
+        return tickets[:got.value]
+
+    RESULT_DTYPE = np.dtype([("ticket", np.int64), ("cls", np.int32), ("status", np.int32),
+                             ("score", np.int32), ("query_end", np.int32),
+                             ("target_end", np.int32), ("read_start", np.int32),
+                             ("ref_start", np.int32), ("n_cigar", np.int32),
+                             ("counts", np.int32, (4,)), ("cigar_off", np.int64),
+                             ("claim_tick", np.int64), ("done_tick", np.int64)])
+
+    def poll_arrays(self):
+        """Completed requests as a structured array (RESULT_DTYPE) plus a copy
+        of their CIGAR runs ((length << 4) | op; cigar_off indexes it)."""
+        got = ctypes.c_int32()
+        check(lib().gmfa_serve_poll(self._handle(), self._results, len(self._results),
+                                    self._cigar.ctypes.data, len(self._cigar),
+                                    ctypes.byref(got)), "gmfa_serve_poll")
+        n = got.value
+        rec = np.frombuffer(self._results, dtype=self.RESULT_DTYPE, count=n).copy()
+        end = int((rec["cigar_off"] + np.where(rec["status"] == 0, rec["n_cigar"], 0)).max()) if n else 0
+        return rec, self._cigar[:end].copy()
+
     def poll(self, max_results: Optional[int] = None) -> list:
         """Completed requests (never blocks)."""
         out, self._pending = self._pending, []
@@ -338,7 +377,8 @@ class ServeSession:
         st = _lib.ServeStats()
         check(lib().gmfa_serve_session_stats(self._handle(), ctypes.byref(st)), "stats")
         return {"submitted": st.submitted, "returned": st.returned,
-                "in_flight": st.in_flight, "launches": st.launches}
+                "in_flight": st.in_flight, "launches": st.launches,
+                "ticks_per_s": st.ticks_per_s}
 
     def close(self) -> ServeResult:
         if self._h is None:

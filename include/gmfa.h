@@ -373,7 +373,10 @@ int gmfa_kmer_emit(const gmfa_kmer_index *index, const gmfa_kmer_reads *reads,
  * out:    per-request outputs as for gmfa_sw_forward with GMFA_SW_CIGAR;
  *         out.tb holds n_slots plane slabs of slab_words uint32 each
  *         (slab_words >= gmfa_sw_tb_bytes(batch.max_qlen) / 4); out.tb_off
- *         is unused; out.work_counter is the class's queue head.
+ *         is unused; out.work_counter points to 64 bytes of 64-byte aligned
+ *         device memory (the class's queue head and device-side mirrors of
+ *         the control words, which idle wavefronts poll instead of host
+ *         memory).
  * A run ends when every request completed or after timeout_s; a wavefront
  * never waits longer than timeout_s + 5 s for a request.  The control words
  * (queue tails and completion flags) are allocated in host-coherent memory
@@ -418,7 +421,7 @@ int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes, double ti
  * Device memory per class is the caller's: n_slots traceback slabs (tb,
  * slab_words >= gmfa_sw_tb_bytes(max_qlen) / 4 each), n_slots query stage
  * slots (stage, 2 * gmfa_serve_stage_words(max_qlen) words each) and the
- * 8-byte ticket head (head, zeroed by gmfa_serve_open).  The library allocates
+ * 64-byte control block (head, zeroed by gmfa_serve_open).  The library allocates
  * the rings and control words in host-coherent memory for the session's life.
  * ------------------------------------------------------------------------- */
 typedef struct gmfa_serve_session gmfa_serve_session;
@@ -434,7 +437,8 @@ typedef struct gmfa_serve_class_cfg {
     uint32_t *tb;           /* device: n_slots slabs of slab_words */
     int64_t slab_words;
     uint32_t *stage;        /* device: n_slots * 2 * gmfa_serve_stage_words(max_qlen) */
-    uint64_t *head;         /* device: one uint64 */
+    uint64_t *head;         /* device: 64 bytes, 64-byte aligned (ticket head and
+                               device-side mirrors of the control words) */
 } gmfa_serve_class_cfg;
 
 #define GMFA_SERVE_TRUNCATED 1  /* n_cigar > cigar_cap: no runs returned */
@@ -451,6 +455,8 @@ typedef struct gmfa_serve_result {
     int32_t n_cigar;        /* runs ((length << 4) | op, GMFA_OP_*) */
     int32_t counts[4];      /* bases under =, X, I, D */
     int64_t cigar_off;      /* first run in the caller's cigar buffer */
+    int64_t claim_tick;     /* device wall clock when a wavefront took the request */
+    int64_t done_tick;      /* ... and when its outputs were complete */
 } gmfa_serve_result;
 
 typedef struct gmfa_serve_stats {
@@ -458,6 +464,7 @@ typedef struct gmfa_serve_stats {
     int64_t returned;       /* requests returned by gmfa_serve_poll */
     int64_t in_flight;      /* submitted - returned */
     int64_t launches;       /* class kernel launches (1 per class + relaunches) */
+    double ticks_per_s;     /* rate of the device wall clock of claim_tick / done_tick */
 } gmfa_serve_stats;
 
 /* Query words of one stage slot for queries up to max_qlen bases. */
@@ -478,6 +485,17 @@ int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask, int64_t r
 int gmfa_serve_submit(gmfa_serve_session *session, int32_t cls, const uint8_t *codes,
                       int32_t qlen, int32_t two_bit, int64_t t_off, int32_t tlen,
                       int32_t band_offset, int64_t *ticket);
+
+/* Submit n requests in order (request k: class cls[k], query codes[code_off[k]
+ * .. code_off[k] + qlen[k]), window t_off[k] / tlen[k], band_offset[k]), with
+ * one publish per class for the whole burst.  Stops at the first request
+ * that is invalid (GMFA_E_INVALID) or meets a full ring (GMFA_E_AGAIN); the
+ * requests before it are published.  *n_submitted receives their count and
+ * tickets[0 .. *n_submitted) their tickets. */
+int gmfa_serve_submit_many(gmfa_serve_session *session, int32_t n, const int32_t *cls,
+                           const uint8_t *codes, const int64_t *code_off, const int32_t *qlen,
+                           int32_t two_bit, const int64_t *t_off, const int32_t *tlen,
+                           const int32_t *band_offset, int64_t *tickets, int32_t *n_submitted);
 
 /* Completed requests, at most max_results, their CIGAR runs packed into the
  * host buffer cigar (cigar_cap runs); each returned request frees its ring

@@ -266,3 +266,72 @@ def test_online_session_rejects_bad_requests(mix):
         assert r[0].ticket == t and r[0].score == exp[0][0]["score"]
     with pytest.raises(RuntimeError):
         sess.submit(0, q, 0, 10, 0)  # closed
+
+
+def test_online_submit_many_bursts_match_oracle(mix):
+    """Bursts through gmfa_serve_submit_many (partial at a full ring, resumed
+    after polls) and results read back as arrays; every request bit-identical
+    to the oracle's."""
+    from graphmambaformer_amd import _lib
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    _say("case: online submit_many bursts, rings 8/4/4")
+    classes = _online_classes(pools, slots=(4, 4, 2), rings=(8, 4, 4))
+    flat, base = [], 0
+    reqs = []  # (class, pool index, code offset)
+    for c, (_, hb, qs, pos, lens, offs) in enumerate(pools):
+        for k, q in enumerate(qs):
+            reqs.append((c, k, base))
+            flat.append(np.asarray(q, np.uint8))
+            base += len(q)
+    codes = np.concatenate(flat)
+    rng = np.random.default_rng(9)
+    order = rng.permutation(len(reqs))
+    cl = np.array([reqs[i][0] for i in order], np.int32)
+    pk = np.array([reqs[i][1] for i in order])
+    co = np.array([reqs[i][2] for i in order], np.int64)
+    ql = np.array([len(pools[c][2][k]) for c, k in zip(cl, pk)], np.int32)
+    to = np.array([pools[c][3][k] for c, k in zip(cl, pk)], np.int64)
+    tl = np.array([pools[c][4][k] for c, k in zip(cl, pk)], np.int32)
+    bo = np.array([pools[c][5][k] for c, k in zip(cl, pk)], np.int32)
+    tickets = {}
+    recs, cigs = [], []
+    with S.ServeSession(ref_dev, classes, SC, idle_timeout_s=1.0) as sess:
+        i, partial = 0, 0
+        while i < len(order):
+            tk = sess.submit_many(cl[i:], codes, co[i:], ql[i:], to[i:], tl[i:], bo[i:])
+            for m, t in enumerate(tk):
+                tickets[(int(cl[i + m]), int(t))] = int(pk[i + m])
+            partial += len(tk) < len(order) - i
+            i += len(tk)
+            r, cg = sess.poll_arrays()
+            recs.append(r)
+            cigs.append(cg)
+        while sum(len(r) for r in recs) < len(order):
+            r, cg = sess.poll_arrays()
+            recs.append(r)
+            cigs.append(cg)
+    assert partial > 0  # the small rings cut some bursts short
+    n = 0
+    for r, cg in zip(recs, cigs):
+        for x in r:
+            e = exp[int(x["cls"])][tickets[(int(x["cls"]), int(x["ticket"]))]]
+            assert x["status"] == 0
+            runs = cg[x["cigar_off"]:x["cigar_off"] + x["n_cigar"]]
+            cig = [(_lib.OP_CHAR[int(v) & 15], int(v) >> 4) for v in runs]
+            ctx = (int(x["cls"]), int(x["ticket"]))
+            assert x["score"] == e["score"] and cig == e["cigar"], ctx
+            assert x["query_end"] == e["read_end"] and x["target_end"] == e["ref_end"], ctx
+            assert x["read_start"] == e["read_start"] and x["ref_start"] == e["ref_start"], ctx
+            n += 1
+    assert n == len(order)
+    # an invalid request in a burst stops it there; the ones before are published
+    with S.ServeSession(ref_dev, classes, SC, idle_timeout_s=1.0) as sess:
+        bad_ql = ql[:3].copy()
+        bad_ql[2] = classes[int(cl[2])].max_qlen + 1
+        with pytest.raises(ValueError):
+            sess.submit_many(cl[:3], codes, co[:3], bad_ql, to[:3], tl[:3], bo[:3])
+        assert sess.stats()["submitted"] == 2
+        assert len(sess.wait(2)) == 2
