@@ -583,13 +583,24 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             const int64_t sw2 = 2 * (int64_t)a.stage_words;
             uint32_t *stg = a.stage + (int64_t)blockIdx.x * sw2;
             const uint32_t *src = a.ring_q + (int64_t)ring_r * sw2;
-            for (int x = ln; x < nw; x += 64)
-                stg[x] = __hip_atomic_load(src + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            if (NM)
-                for (int x = ln; x < nw; x += 64)
-                    stg[a.stage_words + x] = __hip_atomic_load(src + a.stage_words + x,
-                                                               __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_SYSTEM);
+            // four loads in flight per lane before a wait: each is a round
+            // trip over the host link
+            auto copy = [&](uint32_t *dst, const uint32_t *from) __attribute__((always_inline)) {
+                for (int x = ln; x < nw; x += 256) {
+                    uint32_t v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        v[u] = x + 64 * u < nw ? __hip_atomic_load(from + x + 64 * u,
+                                                                   __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_SYSTEM)
+                                               : 0u;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (x + 64 * u < nw) dst[x + 64 * u] = v[u];
+                }
+            };
+            copy(stg, src);
+            if (NM) copy(stg + a.stage_words, src + a.stage_words);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         } else {
@@ -863,7 +874,9 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 // its done word and its completion-queue record
                 __syncthreads();
                 if (lane == 0) {
-                    __hip_atomic_store(a.ring_done + ring_r, ticket + 1, __ATOMIC_RELEASE,
+                    // the done word needs no release of its own: the record's
+                    // release below orders it and the outputs before the record
+                    __hip_atomic_store(a.ring_done + ring_r, ticket + 1, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_SYSTEM);
                     const unsigned long long pos = atomicAdd(&a.dctl->cq_pos, 1ull);
                     __hip_atomic_store(a.ring_cq + (pos & (unsigned long long)(a.ring_n - 1)),
