@@ -633,6 +633,32 @@ def main():
     log(f"[bench] host-inclusive: {result['host_inclusive']}")
     del inp_h, fwd_h, tb_h, runs_h
 
+    # ------------------------------------- x-drop extension path (drop-in form)
+    # what ExtensionEngine / banded_affine_sw_batch run with the reference's
+    # x-drop on: forward with planes and per-row keys, the batch-wide x-drop of
+    # each group of 8 pairs (one read's chains), the walk from the x-drop ends
+    gofs = np.arange(0, w.n_pairs + 1, 8)
+    xd = {}
+
+    def xdrop_step():
+        f = K.sw_forward(run.inp, *scoring(w), w.half_band, traceback=True, rowkeys=True,
+                         order=run.order, tile=tile, workspace=run.ws)
+        sc_, qe_, te_ = K.sw_xdrop(f, run.inp, gofs, 600.0, w.half_band)
+        xd["tb"] = K.sw_traceback(run.inp, f, sc_, qe_, te_, workspace=run.ws)
+        return f
+
+    xdrop_step()
+    xms = time_events(torch, xdrop_step, 3)
+    result["sw_long_xdrop"] = {
+        "gcups": round(total_cells / (xms * 1e-3) / 1e9, 2),
+        "ms": round(xms, 3),
+        "workload": "sw_long, forward with traceback planes + per-row keys (longest first), "
+                    "batch x-drop 600 over groups of 8 pairs, standalone walk from the x-drop "
+                    "ends; GCUPS over the full band cells",
+    }
+    log(f"[bench] x-drop path: {result['sw_long_xdrop']}")
+    del xd
+
     # ------------------------------------------------------------- per-pair form
     sub = np.arange(0, w.n_pairs, 256)[:64]
     pp = per_pair_gcups(torch, K, w, sub, dev)

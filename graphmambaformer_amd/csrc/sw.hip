@@ -534,6 +534,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
     const bool dead_hi = gl >= lstar;                     // registers k > kstar dead
     const bool guard = (kstar == C - 1) && (gl == lstar); // its d+1 is the band guard
     const bool lane_live = gl <= lstar;
+    const uint32_t live_mask = 0u - (uint32_t)lane_live;
     const int lCe = gl * C * ge;
     const int cH = go - mi, cF = ge - mi;
     const uint32_t mms = (uint32_t)(ma + mi);
@@ -810,9 +811,11 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     // the group's last lane ends up holding it
                     const int htrue = (rk >> 5) - i * mi;
                     const int d = gl * C + 31 - (rk & 31);
-                    const int key = lane_live ? (int)(((uint32_t)htrue << 11) | (uint32_t)(2047 - d))
-                                              : INT_MIN;
-                    rkeys[r] = seg_scan_max<G>(key);
+                    // live keys are >= 1 (H >= 0, 2047 - d >= 1), so a dead
+                    // lane's key 0 never wins; an AND, not a select: the select
+                    // form made the scheduler keep every row live (256 VGPRs +
+                    // 256 AGPRs + scratch with traceback planes)
+                    rkeys[r] = (int)((((uint32_t)htrue << 11) | (uint32_t)(2047 - d)) & live_mask);
                 }
             };
             // Hp[k] is overwritten in pass 2 only after pass 1 consumed it, and
@@ -826,10 +829,16 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 #pragma unroll
                 for (int r = 0; r < 8; ++r) row(std::true_type{}, r);
             }
-            if (RK && has && gl == G - 1) {
+            if constexpr (RK) {
+                // group max of each row's lane key, after the block: eight
+                // independent scans off the rows' H dependency chain
 #pragma unroll
-                for (int r = 0; r < 8; ++r)
-                    if (i0 + r <= qlen) a.rowkey[rk_base + (int64_t)(i0 + r - 1)] = rkeys[r];
+                for (int r = 0; r < 8; ++r) rkeys[r] = seg_scan_max<G>(rkeys[r]);
+                if (has && gl == G - 1) {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r)
+                        if (i0 + r <= qlen) a.rowkey[rk_base + (int64_t)(i0 + r - 1)] = rkeys[r];
+                }
             }
             if (TB && has && lane_live && b < nblk_mine) {
                 uint32_t *dst = a.tb + tb_base + ((int64_t)b * (lstar + 1) + gl) * C;
