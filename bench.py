@@ -691,11 +691,17 @@ def main():
     rs = BatchRun(torch, K, ws_, dev, True)
     rs.step()
     ms = time_events(torch, rs.step, 10)
+    # the same step captured in a HIP graph (no host work per launch)
+    gr = K.SwGraph(rs.inp, scoring(ws_), ws_.half_band, order=rs.order, tile=rs.tile,
+                   workspace=rs.ws, cigar=True)
+    gms = time_events(torch, gr.replay, 20)
     pps = per_pair_gcups(torch, K, ws_, np.arange(ws_.n_pairs), dev)
     cgs, css = cpu_gcups(ws_, np.arange(ws_.n_pairs), threads)
     result["sw_short"] = {
         "batched_gcups": round(rs.total_cells / (ms / 1e3) / 1e9, 3),
         "batched_ms": round(ms, 4),
+        "graph_gcups": round(rs.total_cells / (gms / 1e3) / 1e9, 3),
+        "graph_ms": round(gms, 4),
         "per_pair_gcups": round(pps, 4),
         "cpu_gcups": round(cgs, 4),
         "cpu_threads": threads,

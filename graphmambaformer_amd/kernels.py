@@ -358,6 +358,44 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
                      rowkey=rk, rowkey_off_host=rk_off_host, tile=tile, params=params, walk=walk)
 
 
+class SwGraph:
+    """One batch step on a resident batch -- the work-counter reset and the
+    persistent forward kernel, with the CIGAR walk fused when ``cigar`` --
+    captured in a HIP graph: ``replay()`` relaunches it with no host work per
+    launch.  Inputs, workspace and outputs keep their device addresses, so new
+    pairs with the same lengths and offsets can be written into ``inp``'s
+    tensors between replays.  The library's own launches hold no memset
+    nodes (the counter is reset by a kernel), and every buffer is allocated
+    before or during capture, none by a replay."""
+
+    def __init__(self, inp: SwInputs, scoring: tuple, half_band: int,
+                 order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
+                 workspace: Optional[dict] = None, cigar: bool = True):
+        dev = inp.device
+        self.tile = tile or resolve_tile(half_band)
+        self.ws = workspace if workspace is not None else make_workspace(inp, self.tile,
+                                                                        cigar=cigar)
+
+        def step():
+            return sw_forward(inp, *scoring, half_band, traceback=True, order=order,
+                              tile=self.tile, workspace=self.ws, cigar=cigar)
+
+        # one eager pass on a side stream first (module load, allocator warm-up)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            step()
+        torch.cuda.current_stream(dev).wait_stream(side)
+        torch.cuda.synchronize(dev)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = step()
+
+    def replay(self) -> SwForward:
+        self.graph.replay()
+        return self.out
+
+
 @dataclass
 class SwTraceback:
     cigar: torch.Tensor      # flat BAM runs (len << 4 | op), int32 view
