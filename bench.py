@@ -24,6 +24,13 @@ import time
 
 import numpy as np
 
+# The serving legs run one persistent kernel per band class side by side.
+# HIP maps streams onto a per-process pool of hardware queues (4 by default)
+# and a kernel waits behind a running one on a shared queue, so give the
+# process more queues than streams before the HIP runtime starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 METRIC = "banded affine-SW GCUPS batched vs per-pair; p50/p99 latency at fixed QPS"
 HBM_PEAK_GBS = 8000.0
 INT32_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 39.3 T int32 VALU lane-ops/s
@@ -143,7 +150,7 @@ def cpu_gcups(w, idx, threads):
 SERVE_SLOTS = (512, 1024, 2048)  # persistent wavefronts of the short / long / noisy classes
 
 
-def serving_extra(torch, WL, dev, n_req, load):
+def serving_extra(torch, WL, dev, n_req, load, online=True):
     """p50 / p99 latency of a Poisson request stream at a fixed rate.
 
     A resident 46 Mbp reference; requests 60% short (150 bp, half_band 32), 30%
@@ -225,7 +232,8 @@ def serving_extra(torch, WL, dev, n_req, load):
         f"{per_class}")
     del pools
     torch.cuda.empty_cache()
-    res["online"] = online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc)
+    if online:
+        res["online"] = online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc)
     return res
 
 
@@ -273,8 +281,17 @@ def online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc):
         try:
             i = done = 0
             t0 = time.perf_counter()
+            last_done, last_t, last_log = 0, 0.0, 0.0
             while done < n_req:
                 now = time.perf_counter() - t0
+                if done != last_done:
+                    last_done, last_t = done, now
+                elif now - last_t > 60.0:
+                    raise RuntimeError(f"online serving made no progress for 60 s "
+                                       f"({done} of {n_req} done, {i} submitted)")
+                if now - last_log > 5.0:
+                    last_log = now
+                    log(f"[bench] online: {done} of {n_req} done, {i} submitted, {now:.1f} s")
                 j = int(np.searchsorted(arrivals, now, side="right"))
                 host["iterations"] += 1
                 if j > i:
@@ -465,6 +482,9 @@ def main():
     ap.add_argument("--cpu-sample-stride", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=200_000,
                     help="requests of the serving stream (0: skip the serving measurement)")
+    ap.add_argument("--no-online", action="store_true",
+                    help="skip the online-session serving leg (for profiled runs: under a "
+                         "kernel tracer the persistent serving kernels run far slower)")
     ap.add_argument("--serve-load", type=float, default=0.65,
                     help="offered load of the paced serving run, as a fraction of the "
                          "saturation capacity")
@@ -771,7 +791,8 @@ def main():
 
     # ----------------------------------------------------------------- serving
     if args.serve_requests > 0:
-        result["serving"] = serving_extra(torch, WL, dev, args.serve_requests, args.serve_load)
+        result["serving"] = serving_extra(torch, WL, dev, args.serve_requests, args.serve_load,
+                                          online=not args.no_online)
         torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ chains

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace gmfa {
@@ -42,11 +43,41 @@ struct alignas(128) CtlWord64 {
 
 struct ClassCtl {
     CtlWord64 tail;
-    CtlWord closed, gave_up;
+    CtlWord closed, gave_up, started;
 };
 
 // least interval between two reads of a class's host words by its wavefronts
 constexpr double kPollSeconds = 1e-6;
+
+// The class kernels are persistent and must run side by side.  HIP maps
+// streams onto a per-process pool of hardware queues (GPU_MAX_HW_QUEUES, 4
+// by default) and dispatches in order within a queue, so a class kernel whose
+// stream shares a queue with another class's running kernel waits until that
+// one exits.  Wait (bounded) until every class kernel has signalled that it
+// runs; report the queue sharing instead of serving at a fraction of the
+// device.
+constexpr double kStartSeconds = 5.0;
+
+int wait_started(const std::vector<int32_t *> &flags)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        int missing = -1;
+        for (size_t c = 0; c < flags.size(); ++c)
+            if (flags[c] && !__atomic_load_n(flags[c], __ATOMIC_ACQUIRE)) missing = (int)c;
+        if (missing < 0) return GMFA_OK;
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
+            kStartSeconds) {
+            set_error("serving class %d's kernel did not start within %.0f s while the other "
+                      "classes run: its stream shares a HIP hardware queue with another class's "
+                      "persistent kernel (raise GPU_MAX_HW_QUEUES above the number of streams "
+                      "the process uses)",
+                      missing, kStartSeconds);
+            return GMFA_E_RUNTIME;
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+}
 
 }  // namespace
 }  // namespace gmfa
@@ -135,6 +166,7 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
         sl.closed = &ctl_d[c].closed.v;
         sl.done = done_d + flag_off[c];
         sl.gave_up = &ctl_d[c].gave_up.v;
+        sl.started = &ctl_d[c].started.v;
         sl.slab_words = k.slab_words;
         sl.deadline_ticks = (uint64_t)((timeout_s + 5.0) * (double)khz * 1000.0);
         sl.poll_ticks = (uint64_t)(kPollSeconds * (double)khz * 1000.0) + 1;
@@ -143,6 +175,13 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
         const int rc = sw_launch(&k.batch, &k.params, &k.out, &sl, streams[c]);
         if (rc != GMFA_OK) return finish(rc);
         launched[c] = 1;
+    }
+    {
+        std::vector<int32_t *> flags(n_classes, nullptr);
+        for (int c = 0; c < n_classes; ++c)
+            if (launched[c]) flags[c] = &ctl[c].started.v;
+        const int rc = wait_started(flags);
+        if (rc != GMFA_OK) return finish(rc);
     }
 
     using clock = std::chrono::steady_clock;
@@ -200,7 +239,7 @@ namespace {
 
 struct OnlineCtl {
     CtlWord64 tail;
-    CtlWord closed, gave_up;
+    CtlWord closed, gave_up, started;
 };
 
 struct OnlineClass {
@@ -409,6 +448,7 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
         sl.closed = &dev_ptr(host, devp, c.ctl)->closed.v;
         sl.done = nullptr;
         sl.gave_up = &dev_ptr(host, devp, c.ctl)->gave_up.v;
+        sl.started = &dev_ptr(host, devp, c.ctl)->started.v;
         sl.slab_words = k.slab_words;
         sl.deadline_ticks = s->idle_ticks;
         sl.poll_ticks = (uint64_t)(kPollSeconds * (double)khz * 1000.0) + 1;
@@ -447,6 +487,15 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
             set_error("%s", msg.c_str());
             return rc;
         }
+    }
+    std::vector<int32_t *> flags;
+    for (auto &c : s->cls) flags.push_back(&c.ctl->started.v);
+    const int src = wait_started(flags);
+    if (src != GMFA_OK) {
+        const std::string msg = gmfa_last_error();
+        gmfa::free_session(s);
+        set_error("%s", msg.c_str());
+        return src;
     }
     *session = s;
     return GMFA_OK;

@@ -88,6 +88,7 @@ struct SwKArgs {
     const int32_t *serve_closed;  // nonzero: nothing more will be published
     int32_t *serve_done;          // [n_pairs] set to 1 when request k is complete
     int32_t *serve_gave_up;       // set to 1 if a wave's wait hit deadline_ticks
+    int32_t *serve_started;       // set to 1 by the first wavefront of the launch
     int64_t slab_words;           // traceback plane words per wave slot
     uint64_t deadline_ticks;      // bound of one idle wait (wall-clock ticks)
     uint64_t poll_ticks;          // least interval between two reads of the host words
@@ -543,7 +544,13 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
     // online form: a runtime switch of the serving instantiation (a separate
     // instantiation scheduled the DP with up to 90 more VGPRs)
     bool online = false;
-    if constexpr (SV) online = a.ring_n > 0;
+    if constexpr (SV) {
+        online = a.ring_n > 0;
+        // tell the host this class's kernel is running (the host checks that
+        // every class runs side by side)
+        if (blockIdx.x == 0 && threadIdx.x == 0 && a.serve_started)
+            __hip_atomic_store(a.serve_started, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     for (;;) {
         int item = 0;
         int n_items = a.n_items;
@@ -1378,6 +1385,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
 
     a.serve_tail = nullptr; a.serve_closed = nullptr; a.serve_done = nullptr;
     a.serve_gave_up = nullptr; a.slab_words = 0; a.deadline_ticks = 0;
+    a.serve_started = nullptr;
     a.poll_ticks = 0; a.dctl = nullptr;
     a.ring_n = 0; a.stage_words = 0; a.cigar_cap = 0;
     a.ring_tail = nullptr; a.ring_desc = nullptr; a.ring_q = nullptr;
@@ -1400,6 +1408,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                      (long long)serve->slab_words, (long long)need);
         a.serve_tail = serve->tail; a.serve_closed = serve->closed; a.serve_done = serve->done;
         a.serve_gave_up = serve->gave_up; a.slab_words = serve->slab_words;
+        a.serve_started = serve->started;
         a.deadline_ticks = serve->deadline_ticks;
         a.poll_ticks = serve->poll_ticks;
         GMFA_REQUIRE(serve->dctl && ((uintptr_t)serve->dctl & 63) == 0,

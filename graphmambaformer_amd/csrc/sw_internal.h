@@ -58,6 +58,7 @@ struct ServeLaunch {
     const int32_t *closed;    // host-coherent: nothing more will be published
     int32_t *done;            // staged: [n_pairs] completion flags (host-coherent)
     int32_t *gave_up;         // host-coherent: a wave's idle wait hit its bound
+    int32_t *started;         // host-coherent: set by the launch's first wavefront
     int64_t slab_words;       // traceback plane words per wave slot (out->tb holds n_slots slabs)
     uint64_t deadline_ticks;  // bound of one idle wait of a wave, wall-clock ticks
     uint64_t poll_ticks;      // least interval between two reads of the host words

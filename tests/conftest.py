@@ -4,6 +4,13 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# Serving tests run one persistent kernel per band class side by side; HIP
+# maps streams onto a per-process pool of hardware queues (4 by default) and a
+# kernel waits behind a running one on a shared queue.  Set before the HIP
+# runtime starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
