@@ -483,8 +483,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200_000,
                     help="requests of the serving stream (0: skip the serving measurement)")
     ap.add_argument("--no-online", action="store_true",
-                    help="skip the online-session serving leg (for profiled runs: under a "
-                         "kernel tracer the persistent serving kernels run far slower)")
+                    help="skip the online-session serving leg (keeps a profiled run short: "
+                         "its Python front end makes ~10^5 library calls)")
     ap.add_argument("--serve-load", type=float, default=0.65,
                     help="offered load of the paced serving run, as a fraction of the "
                          "saturation capacity")
