@@ -148,6 +148,10 @@ def cpu_gcups(w, idx, threads):
 
 
 SERVE_SLOTS = (512, 1024, 2048)  # persistent wavefronts of the short / long / noisy classes
+# traceback-plane budgets of the serving classes (paged planes: a request is
+# admitted when its planes' pages are free); per-slot slabs for each class's
+# longest request would take 0.02 / 6.6 / 156 GB
+SERVE_PLANE_BUDGET = (256 << 20, 8 << 30, 32 << 30)
 
 
 def serving_extra(torch, WL, dev, n_req, load, online=True):
@@ -180,7 +184,7 @@ def serving_extra(torch, WL, dev, n_req, load, online=True):
         for c, p in enumerate(mix.pools):
             sel = np.flatnonzero(cls == c)
             out.append(S.make_class(p.name, ref, pools[c], idx[sel], arrivals[sel], sc,
-                                    SERVE_SLOTS[c]))
+                                    SERVE_SLOTS[c], plane_budget_bytes=SERVE_PLANE_BUDGET[c]))
         return out
 
     def gather(classes):
@@ -190,6 +194,11 @@ def serving_extra(torch, WL, dev, n_req, load, online=True):
         return comp
 
     sat = build(np.zeros(n_req))
+    slab_gb = lambda k: round(k.n_slots * k.slab_words * 4 / 2**30, 2)  # noqa: E731
+    pages = {p.name: ({"page_blocks": k.pages.page_blocks, "pages": k.pages.n_pages,
+                       "pool_gb": round(k.pages.pool_bytes / 2**30, 2),
+                       "slab_equivalent_gb": slab_gb(k)} if k.pages else {"slab_gb": slab_gb(k)})
+             for p, k in zip(mix.pools, sat)}
     r_sat = S.run(sat, timeout_s=300.0)
     capacity = n_req / r_sat.wall_s
     del sat
@@ -222,11 +231,13 @@ def serving_extra(torch, WL, dev, n_req, load, online=True):
         "completed": r.completed,
         "timed_out": r.timed_out,
         "slots": dict(zip([p.name for p in mix.pools], SERVE_SLOTS)),
+        "planes": pages,
         "workload": f"{n_req} Poisson requests, 60% short (150 bp, half_band 32) / 30% long "
                     "(lognormal 12 kb, half_band 256) / 10% noisy (lognormal 8 kb, 6% errors, "
                     "half_band 751) against a resident 46 Mbp 2-bit reference; pools of "
                     "2048/1024/512 distinct windows, seed 4; first 10% of arrivals are warm-up; "
-                    "queries pre-staged in HBM",
+                    "queries pre-staged in HBM; traceback planes paged from per-class budgets "
+                    "(admission by footprint)",
     }
     log(f"[bench] serving paced at {rate:.0f} req/s: {res['latency']} by class "
         f"{per_class}")
@@ -267,7 +278,8 @@ def online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc):
     tl = np.array([mix.pools[c].t_len[k] for c, k in zip(cls, idx)], np.int32)
     bo = np.array([mix.pools[c].band_offset[k] for c, k in zip(cls, idx)], np.int32)
     ocls = [S.OnlineClass(half_band=p.half_band, max_qlen=max(len(q) for q in p.queries),
-                          n_slots=SERVE_SLOTS[c], cigar_cap=ONLINE_CIGAR_CAP[c])
+                          n_slots=SERVE_SLOTS[c], cigar_cap=ONLINE_CIGAR_CAP[c],
+                          plane_budget_bytes=SERVE_PLANE_BUDGET[c])
             for c, p in enumerate(mix.pools)]
 
     def drive(arrivals):

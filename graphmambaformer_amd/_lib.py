@@ -22,6 +22,7 @@ GMFA_E_RUNTIME = -2
 GMFA_E_UNSUPPORTED = -3
 GMFA_E_AGAIN = -4
 GMFA_SERVE_TRUNCATED = 1
+GMFA_SERVE_NO_PAGES = 2
 
 SW_TRACEBACK = 1
 SW_NMASK = 2
@@ -75,7 +76,8 @@ class KmerReads(ctypes.Structure):
 class ServeClass(ctypes.Structure):
     _fields_ = [("batch", SwBatch), ("params", SwParams), ("out", SwOut),
                 ("slab_words", _i64), ("n_slots", _i32), ("reserved", _i32),
-                ("arrival_s", _P), ("complete_s", _P)]
+                ("arrival_s", _P), ("complete_s", _P),
+                ("page_blocks", _i32), ("n_pages", _i32), ("page_owner", _P)]
 
 
 class ServeReport(ctypes.Structure):
@@ -86,7 +88,7 @@ class ServeReport(ctypes.Structure):
 class ServeClassCfg(ctypes.Structure):
     _fields_ = [("params", SwParams), ("max_qlen", _i32), ("ring", _i32), ("n_slots", _i32),
                 ("cigar_cap", _i32), ("tb", _P), ("slab_words", _i64), ("stage", _P),
-                ("head", _P)]
+                ("head", _P), ("page_blocks", _i32), ("n_pages", _i32), ("page_owner", _P)]
 
 
 class ServeResultRec(ctypes.Structure):

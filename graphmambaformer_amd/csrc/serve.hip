@@ -168,6 +168,14 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
         sl.gave_up = &ctl_d[c].gave_up.v;
         sl.started = &ctl_d[c].started.v;
         sl.slab_words = k.slab_words;
+        sl.page_blocks = k.page_blocks;
+        sl.n_pages = k.n_pages;
+        sl.page_owner = k.page_owner;
+        sl.alloc_ticks = (uint64_t)((timeout_s + 5.0) * (double)khz * 1000.0);
+        if (k.page_blocks > 0) {
+            const int rc = serve_reset_pages(k.page_owner, k.n_pages, streams[c]);
+            if (rc != GMFA_OK) return finish(rc);
+        }
         sl.deadline_ticks = (uint64_t)((timeout_s + 5.0) * (double)khz * 1000.0);
         sl.poll_ticks = (uint64_t)(kPollSeconds * (double)khz * 1000.0) + 1;
         sl.dctl = reinterpret_cast<ServeDevCtl *>(k.out.work_counter);
@@ -465,6 +473,11 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
         sl.stage = k.stage;
         sl.stage_words = c.stage_words;
         sl.cigar_cap = k.cigar_cap;
+        sl.page_blocks = k.page_blocks;
+        sl.n_pages = k.n_pages;
+        sl.page_owner = k.page_owner;
+        // a wait for pages is bounded by the longer of the idle bound and 60 s
+        sl.alloc_ticks = (uint64_t)(std::max(idle_timeout_s, 60.0) * (double)khz * 1000.0);
 
         if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
             c.stream = nullptr;
@@ -475,6 +488,11 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
         // the head starts at ticket 0, the mirrors at 0 (a kernel, not a
         // memset: no memset nodes on a stream a caller might capture)
         hipLaunchKernelGGL(zero_ctl_kernel, dim3(1), dim3(64), 0, c.stream, sl.dctl);
+        if (k.page_blocks > 0 && serve_reset_pages(k.page_owner, k.n_pages, c.stream) != GMFA_OK) {
+            gmfa::free_session(s);
+            set_error("page-owner reset launch failed");
+            return GMFA_E_RUNTIME;
+        }
         if (hipGetLastError() != hipSuccess) {
             gmfa::free_session(s);
             set_error("head reset launch failed");
@@ -682,14 +700,15 @@ extern "C" int gmfa_serve_poll(gmfa_serve_session *s, gmfa_serve_result *results
             }
             const ServeRes &e = c.res[r];
             const int nc = e.n_cigar;
+            const bool nopages = nc < 0;
             const bool trunc = nc > cap;
-            const int keep = trunc ? 0 : nc;
+            const int keep = (trunc || nopages) ? 0 : nc;
             if (coff + keep > cigar_cap) break;
             ++c.cq_head;
             gmfa_serve_result &o = results[n++];
             o.ticket = t;
             o.cls = ci;
-            o.status = trunc ? GMFA_SERVE_TRUNCATED : 0;
+            o.status = nopages ? GMFA_SERVE_NO_PAGES : trunc ? GMFA_SERVE_TRUNCATED : 0;
             o.score = e.score;
             o.query_end = e.query_end;
             o.target_end = e.target_end;

@@ -104,6 +104,10 @@ struct SwKArgs {
     int64_t *ring_done;
     uint64_t *ring_cq;
     uint32_t *stage;
+    // paged traceback planes (serving; page_shift < 0: per-slot slabs)
+    int32_t page_shift, n_pages;
+    int32_t *page_owner;
+    uint64_t alloc_ticks;
 };
 
 __device__ __forceinline__ int64_t uni64(int64_t v)
@@ -216,6 +220,73 @@ __device__ __forceinline__ bool serve_claim(ServeDevCtl *d, const int64_t *tail_
         for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(127);
         nap = nap < 16 ? 2 * nap : 16;
     }
+}
+
+// Paged traceback planes (serving).  A request's planes live in pages of
+// 2^shift 8-row blocks taken from its class's pool, all or nothing (a short
+// pass gives every page back, so no two waves each hold part of what the
+// other needs).  A pass starts at a shared hint that moves by `need` per
+// request, so consecutive requests look at consecutive pages; it reads 64
+// owner words at a time and tries compare-and-swap only on as many of the
+// free-looking ones as it still needs (no page is won and handed back).  The
+// page list goes to `pl`, the wavefront's own LDS array (one wavefront per
+// workgroup).  False after `bound` ticks without enough free pages.
+__device__ __forceinline__ bool take_pages(int32_t *owner, int n_pages, int need,
+                                           unsigned long long *hint, int32_t *pl, uint64_t bound)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t t0 = (uint64_t)wall_clock64();
+    int nap = 1;
+    for (;;) {
+        unsigned long long h0 = 0ull;
+        if (lane == 0) h0 = atomicAdd(hint, (unsigned long long)need);
+        const int start = (int)((uint64_t)uni64((int64_t)h0) % (uint64_t)n_pages);
+        int got = 0;
+        for (int scanned = 0; scanned < n_pages && got < need;) {
+            const int pg = (int)((uint32_t)(start + scanned + lane) % (uint32_t)n_pages);
+            const bool in = scanned + lane < n_pages;
+            const bool looks_free =
+                in && __hip_atomic_load(&owner[pg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+            const uint64_t mf = __ballot(looks_free);
+            const int rf = (int)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(mf >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mf, 0u));
+            const bool tried = looks_free && rf < need - got;
+            const bool won = tried && atomicCAS(&owner[pg], 0, 1) == 0;
+            const uint64_t mw = __ballot(won);
+            if (won) {
+                const int rw = (int)__builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(mw >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mw, 0u));
+                pl[got + rw] = pg;
+            }
+            got += (int)__popcll(mw);
+            // past the last page tried (free-looking pages beyond it stay for
+            // the next pass), or past all 64 when fewer were tried than looked free
+            const uint64_t mt = __ballot(tried);
+            scanned += (int)__popcll(mf) > (int)__popcll(mt) ? 64 - (int)__clzll(mt) : 64;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (got >= need) return true;
+        for (int e = lane; e < got; e += 64) atomicExch(&owner[pl[e]], 0);
+        if ((uint64_t)wall_clock64() - t0 > bound) return false;
+        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(127);
+        nap = nap < 16 ? 2 * nap : 16;
+    }
+}
+
+// called by the whole wavefront
+__device__ __forceinline__ void free_pages(int32_t *owner, const int32_t *pl, int need)
+{
+    for (int e = (int)(threadIdx.x & 63); e < need; e += 64) atomicExch(&owner[pl[e]], 0);
+}
+
+// Physical page of plane page `page` (wave-uniform).  One LDS address for
+// every lane, so the value is right whichever lanes are active (a register
+// table read with readlane is not: a lane outside the EXEC mask, such as a
+// band lane past the stored ones during the plane store, holds no value).
+__device__ __forceinline__ uint32_t page_of(const int32_t *pl, int page)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane(pl[page]);
 }
 
 __device__ __forceinline__ int64_t clampi64(int64_t x, int64_t hi)
@@ -392,6 +463,92 @@ __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int
     if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
 }
 
+// The same walk reading the planes through a plane-load functor ldp(block,
+// diagonal): serving with paged planes.
+template <int C, typename Ld, typename Ldp, typename Put>
+__device__ __forceinline__ void walk_steps_paged(const WalkCtx &c, int &i, int &j, int h, int off,
+                                           int64_t qo, int64_t to, Ldp ldp, Ld ld,
+                                           Put put, int &nr, int (&cnt)[4])
+{
+    const bool econt_ok = c.go != c.ge;
+    const int blkstride = c.Gs * C;  // dwords between vertically adjacent blocks
+    int state = 0;  // 0 H, 1 F (I), 2 E (D)
+    int run_op = -1, run_len = 0;
+    int64_t cur = -1, pf_addr = -1;
+    uint32_t v = 0u, pf = 0u;
+    SeqWindow qwin, twin, qmwin, tmwin;
+    auto emit = [&](int op, int n) {
+        if (op == run_op) {
+            run_len += n;
+        } else {
+            if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
+            run_op = op;
+            run_len = n;
+        }
+    };
+    while (i > 0 && j > 0) {
+        const int d = j - i - off + c.hb;
+        if (d < 0 || d >= c.W) break;  // unreachable for planes this library wrote
+        // plane word of block (i-1)/8 = blk * blkstride + d
+        const int64_t addr = plane_word<C>(i, d, c.Gs);
+        if (addr != cur) {
+            const int blk = (i - 1) >> 3;
+            v = (addr == pf_addr) ? pf : ldp(blk, d);
+            cur = addr;
+            pf_addr = (i > 8) ? addr - blkstride : -1;
+            if (pf_addr >= 0) pf = ldp(blk - 1, d);
+        }
+        const int r0 = (i - 1) & 7;
+        const uint32_t nib = (v >> (28 - 4 * r0)) & 15u;
+        if (state == 0) {
+            if (h <= 0) break;
+            if (!(nib & 4u)) {
+                // rows r0, r0-1, ... of this dword while not_diag stays 0
+                const uint32_t y = (v & 0x44444444u) & (~0u << (30 - 4 * r0));
+                int m = y ? r0 - ((30 - (int)__builtin_ctz(y)) >> 2) : r0 + 1;
+                m = min(m, min(i, j));
+                if (h <= (m - 1) * c.ma) m = 1;
+                const uint32_t lenmask = (m >= 16) ? ~0u : ((1u << (2 * m)) - 1u);
+                const int64_t eq = qo + i - m, et = to + j - m;
+                uint32_t x = seq_bits(c.qseq, c.q_words, eq, qwin, ld) ^
+                             seq_bits(c.tseq, c.t_words, et, twin, ld);
+                x = (x | (x >> 1)) & 0x55555555u;
+                if (c.nm)
+                    x |= (seq_bits(c.qmask, c.q_words, eq, qmwin, ld) |
+                          seq_bits(c.tmask, c.t_words, et, tmwin, ld)) & 0x55555555u;
+                x &= lenmask;  // bit 2t: base i-m+t mismatches
+                const int nmis = __builtin_popcount(x);
+                h -= c.ma * (m - nmis) - c.mi * nmis;
+                cnt[0] += m - nmis;
+                cnt[1] += nmis;
+                if (x == 0u) {
+                    emit(GMFA_OP_EQ, m);
+                } else {
+                    for (int t = m - 1; t >= 0; --t)
+                        emit(((x >> (2 * t)) & 1u) ? GMFA_OP_X : GMFA_OP_EQ, 1);
+                }
+                i -= m;
+                j -= m;
+            } else {
+                state = (nib & 2u) ? 2 : 1;
+            }
+        } else if (state == 1) {
+            emit(GMFA_OP_I, 1);
+            cnt[2]++;
+            --i;
+            if (nib & 8u) h += c.ge;
+            else { h += c.go; state = 0; }
+        } else {
+            emit(GMFA_OP_D, 1);
+            cnt[3]++;
+            --j;
+            if (econt_ok && (nib & 1u)) h += c.ge;
+            else { h += c.go; state = 0; }
+        }
+    }
+    if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
+}
+
 // Fused-path walk of one pair by a whole wavefront (state wave-uniform).
 template <int C>
 __device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, int p, int i, int j,
@@ -409,6 +566,41 @@ __device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, in
         if (lane == 0) out[x] = vrun;
     };
     walk_steps<C>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int x = lane; x < nr / 2; x += 64) {
+        const int y = nr - 1 - x;
+        const uint32_t va = out[x], vb = out[y];
+        out[x] = vb;
+        out[y] = va;
+    }
+    if (lane == 0) {
+        o.ncig[p] = nr;
+        o.rstart[p] = i;
+        o.tstart[p] = j;
+        o.counts[4 * p + 0] = cnt[0];
+        o.counts[4 * p + 1] = cnt[1];
+        o.counts[4 * p + 2] = cnt[2];
+        o.counts[4 * p + 3] = cnt[3];
+    }
+}
+
+// walk_pair for paged planes.
+template <int C, typename Ldp>
+__device__ __forceinline__ void walk_pair_paged(const WalkCtx &c, const WalkOut &o, int p, int i, int j,
+                                          int h, int off, int64_t qo, int64_t to, Ldp ldp)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    uint32_t *out = o.cigar + o.cig_off[p];
+    int nr = 0;
+    int cnt[4] = {0, 0, 0, 0};
+    auto ld = [&](const uint32_t *src, int64_t idx) {
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)src[idx]);
+    };
+    auto put = [&](int x, uint32_t vrun) {
+        if (lane == 0) out[x] = vrun;
+    };
+    walk_steps_paged<C>(c, i, j, h, off, qo, to, ldp, ld, put, nr, cnt);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     for (int x = lane; x < nr / 2; x += 64) {
@@ -458,6 +650,43 @@ __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, 
     };
     const int score = h, qend = i, tend = j;
     walk_steps<C>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    if (lane < (nr & 63)) {
+        const int y = cap - 1 - ((nr & ~63) + lane);
+        if (y >= 0) cig[y] = buf;
+    }
+    const uint64_t done_tick = (uint64_t)wall_clock64();
+    const int v = lane == 0 ? score : lane == 1 ? qend : lane == 2 ? tend : lane == 3 ? i
+                : lane == 4 ? j : lane == 5 ? nr : lane == 6 ? cnt[0] : lane == 7 ? cnt[1]
+                : lane == 8 ? cnt[2] : lane == 9 ? cnt[3]
+                : lane == 10 ? (int)(uint32_t)claim_tick : lane == 11 ? (int)(claim_tick >> 32)
+                : lane == 12 ? (int)(uint32_t)done_tick : (int)(done_tick >> 32);
+    if (lane < 14) reinterpret_cast<int32_t *>(res)[lane] = v;
+}
+
+// walk_pair_ring for paged planes.
+template <int C, typename Ldp>
+__device__ __forceinline__ void walk_pair_ring_paged(const WalkCtx &c, uint32_t *cig, int cap,
+                                               ServeRes *res, int i, int j, int h, int off,
+                                               int64_t qo, int64_t to, Ldp ldp,
+                                               uint64_t claim_tick)
+{
+    int lane = (int)(threadIdx.x & 63);
+    asm volatile("" : "+v"(lane));  // keep lane-derived addresses inside the request loop
+    int nr = 0;
+    int cnt[4] = {0, 0, 0, 0};
+    uint32_t buf = 0u;
+    auto ld = [&](const uint32_t *src, int64_t idx) {
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)src[idx]);
+    };
+    auto put = [&](int x, uint32_t vrun) {
+        if (lane == (x & 63)) buf = vrun;
+        if ((x & 63) == 63) {
+            const int y = cap - 1 - (x - 63 + lane);
+            if (y >= 0) cig[y] = buf;
+        }
+    };
+    const int score = h, qend = i, tend = j;
+    walk_steps_paged<C>(c, i, j, h, off, qo, to, ldp, ld, put, nr, cnt);
     if (lane < (nr & 63)) {
         const int y = cap - 1 - ((nr & ~63) + lane);
         if (y >= 0) cig[y] = buf;
@@ -634,6 +863,49 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                                   : (has ? a.q_off[p] : 0);
         const int64_t to = online ? r_to : (has ? a.t_off[p] : 0);
         const int maxq = (G == 64) ? qlen : wave_max(qlen);
+
+        // serving with paged planes: the request's pages before its forward
+        // pass (admission by the planes' HBM footprint)
+        int32_t *pl = nullptr;
+        int need_pages = 0;
+        bool paged = false;
+        if constexpr (SV && TB) {
+            __shared__ int32_t page_list[kMaxPages];
+            pl = page_list;
+            paged = a.page_shift >= 0;
+            if (paged && has) {
+                need_pages = (((qlen + 7) >> 3) + (1 << a.page_shift) - 1) >> a.page_shift;
+                if (!take_pages(a.page_owner, a.n_pages, need_pages, &a.dctl->page_hint, pl,
+                                a.alloc_ticks)) {
+                    // no pages within the bound: the request completes without
+                    // outputs (n_cigar = -1) and the host is told
+                    if (lane == 0)
+                        __hip_atomic_store(a.serve_gave_up, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                    if (online) {
+                        if (lane == 0) {
+                            ServeRes *res = a.ring_res + ring_r;
+                            res->score = 0; res->query_end = 0; res->target_end = 0;
+                            res->read_start = 0; res->ref_start = 0; res->n_cigar = -1;
+                            __hip_atomic_store(a.ring_done + ring_r, ticket + 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_SYSTEM);
+                            const unsigned long long pos = atomicAdd(&a.dctl->cq_pos, 1ull);
+                            __hip_atomic_store(
+                                a.ring_cq + (pos & (unsigned long long)(a.ring_n - 1)),
+                                ((uint64_t)(pos + 1) << 32) | (uint32_t)ring_r, __ATOMIC_RELEASE,
+                                __HIP_MEMORY_SCOPE_SYSTEM);
+                        }
+                    } else if (lane == 0) {
+                        a.wo.ncig[p] = -1;
+                        __hip_atomic_store(a.serve_done + p, 1, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                    }
+                    continue;
+                }
+            }
+        }
+        const int64_t page_words = paged ? ((int64_t)(lstar + 1) * C) << a.page_shift : 0;
+        const int32_t page_mask = paged ? (1 << a.page_shift) - 1 : 0;
 
         int Hp[C], Fp[C];
         uint32_t acc[C];
@@ -849,6 +1121,11 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             }
             if (TB && has && lane_live && b < nblk_mine) {
                 uint32_t *dst = a.tb + tb_base + ((int64_t)b * (lstar + 1) + gl) * C;
+                if constexpr (SV) {
+                    if (paged)
+                        dst = a.tb + (int64_t)page_of(pl, b >> a.page_shift) * page_words +
+                              ((int64_t)(b & page_mask) * (lstar + 1) + gl) * C;
+                }
 #pragma unroll
                 for (int k = 0; k < C; ++k) dst[k] = acc[k];
             }
@@ -883,9 +1160,21 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             if (online) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                walk_pair_ring<C>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap, a.cigar_cap,
-                                  a.ring_res + ring_r, res_i, res_j, res_s, off, qo, to,
-                                  a.tb + tb_base, claim_tick);
+                if (paged) {
+                    const int64_t blkstride = (int64_t)(lstar + 1) * C;
+                    auto ldp = [&](int blk, int d) {
+                        const uint32_t pgp = page_of(pl, blk >> a.page_shift);
+                        return (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tb[
+                            (int64_t)pgp * page_words + (int64_t)(blk & page_mask) * blkstride + d]);
+                    };
+                    walk_pair_ring_paged<C>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap,
+                                            a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s,
+                                            off, qo, to, ldp, claim_tick);
+                } else {
+                    walk_pair_ring<C>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap,
+                                      a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s, off,
+                                      qo, to, a.tb + tb_base, claim_tick);
+                }
                 // every output of the request (stored by the wave's lanes) before
                 // its done word and its completion-queue record
                 __syncthreads();
@@ -899,6 +1188,9 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                                        ((uint64_t)(pos + 1) << 32) | (uint32_t)ring_r,
                                        __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
                 }
+                // after the release above (which wrote this L2's dirty plane
+                // lines back), the pages may go to a wave on another XCD
+                if (paged) free_pages(a.page_owner, pl, need_pages);
                 continue;
             }
         }
@@ -917,9 +1209,24 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     const int offg = __builtin_amdgcn_readlane(off, L0);
                     const int64_t qog = readlane64(qo, L0), tog = readlane64(to, L0);
                     const int64_t tbg = readlane64(tb_base, L0);
-                    walk_pair<C>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
-                                 __builtin_amdgcn_readlane(res_j, L0),
-                                 __builtin_amdgcn_readlane(res_s, L0), offg, qog, tog, a.tb + tbg);
+                    if (paged) {
+                        const int64_t blkstride = (int64_t)wc.Gs * C;
+                        auto ldp = [&](int blk, int d) {
+                            const uint32_t pgp = page_of(pl, blk >> a.page_shift);
+                            return (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tb[
+                                (int64_t)pgp * page_words + (int64_t)(blk & page_mask) * blkstride +
+                                d]);
+                        };
+                        walk_pair_paged<C>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
+                                           __builtin_amdgcn_readlane(res_j, L0),
+                                           __builtin_amdgcn_readlane(res_s, L0), offg, qog, tog,
+                                           ldp);
+                    } else {
+                        walk_pair<C>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
+                                     __builtin_amdgcn_readlane(res_j, L0),
+                                     __builtin_amdgcn_readlane(res_s, L0), offg, qog, tog,
+                                     a.tb + tbg);
+                    }
                 }
             }
         }
@@ -929,11 +1236,19 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 // every output of this pair (stored by this wave) before the flag
                 __hip_atomic_store(a.serve_done + p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
             }
+            // after that release (dirty plane lines written back), free the pages
+            if (paged) free_pages(a.page_owner, pl, need_pages);
         }
     }
 }
 
 __global__ void reset_counter_kernel(uint32_t *c) { *c = 0u; }
+
+__global__ void reset_pages_kernel(int32_t *owner, int32_t n)
+{
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += gridDim.x * blockDim.x)
+        owner[i] = 0;
+}
 
 // zero a serving class's device control block (a kernel, not a memset: no
 // memset nodes on a stream a caller might capture)
@@ -1312,6 +1627,15 @@ static WalkCtx make_walk_ctx(const gmfa_sw_batch *batch, const gmfa_sw_params *p
 
 namespace gmfa {
 
+int serve_reset_pages(int32_t *owner, int32_t n_pages, hipStream_t s)
+{
+    if (!owner || n_pages <= 0) return GMFA_OK;
+    const int blocks = std::min(1024, (n_pages + 255) / 256);
+    hipLaunchKernelGGL(reset_pages_kernel, dim3(blocks), dim3(256), 0, s, owner, n_pages);
+    GMFA_CHECK_HIP(hipGetLastError());
+    return GMFA_OK;
+}
+
 int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gmfa_sw_out *out,
               const ServeLaunch *serve, hipStream_t s)
 {
@@ -1391,6 +1715,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     a.ring_tail = nullptr; a.ring_desc = nullptr; a.ring_q = nullptr;
     a.ring_res = nullptr; a.ring_cigar = nullptr; a.ring_done = nullptr; a.stage = nullptr;
     a.ring_cq = nullptr;
+    a.page_shift = -1; a.n_pages = 0; a.page_owner = nullptr; a.alloc_ticks = 0;
     int waves = 1;
     if (!serve) {
         waves = std::max(1, std::min(a.n_items, resident_waves(k)));
@@ -1402,10 +1727,31 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                      "serving needs GMFA_SW_TRACEBACK | GMFA_SW_CIGAR and no row keys");
         GMFA_REQUIRE(batch->order == nullptr, "serving takes requests in arrival order (order must be NULL)");
         GMFA_REQUIRE(serve->n_slots >= 1, "n_slots must be positive");
-        const int64_t need = gmfa_sw_tb_bytes(batch->max_qlen, &tile) / 4;
-        GMFA_REQUIRE(serve->slab_words >= need,
-                     "slab of %lld plane words per slot is below the %lld the longest query needs",
-                     (long long)serve->slab_words, (long long)need);
+        if (serve->page_blocks > 0) {
+            // paged planes: pages of page_blocks 8-row blocks from one pool
+            const int pb = serve->page_blocks;
+            GMFA_REQUIRE((pb & (pb - 1)) == 0, "page_blocks %d is not a power of two", pb);
+            const int64_t nblk = (batch->max_qlen + 7) / 8;
+            const int64_t pages = (nblk + pb - 1) / pb;
+            GMFA_REQUIRE(pages <= kMaxPages,
+                         "the longest query needs %lld pages of %d blocks; at most %d (use larger "
+                         "pages)", (long long)pages, pb, kMaxPages);
+            GMFA_REQUIRE(serve->n_pages >= std::max<int64_t>(pages, 1),
+                         "a pool of %d pages cannot hold the longest query (%lld pages)",
+                         serve->n_pages, (long long)pages);
+            GMFA_REQUIRE(serve->page_owner, "paged planes need page_owner");
+            int sh = 0;
+            while ((1 << sh) < pb) ++sh;
+            a.page_shift = sh;
+            a.n_pages = serve->n_pages;
+            a.page_owner = serve->page_owner;
+            a.alloc_ticks = serve->alloc_ticks;
+        } else {
+            const int64_t need = gmfa_sw_tb_bytes(batch->max_qlen, &tile) / 4;
+            GMFA_REQUIRE(serve->slab_words >= need,
+                         "slab of %lld plane words per slot is below the %lld the longest query "
+                         "needs", (long long)serve->slab_words, (long long)need);
+        }
         a.serve_tail = serve->tail; a.serve_closed = serve->closed; a.serve_done = serve->done;
         a.serve_gave_up = serve->gave_up; a.slab_words = serve->slab_words;
         a.serve_started = serve->started;

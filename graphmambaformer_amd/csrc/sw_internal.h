@@ -42,7 +42,8 @@ struct alignas(64) ServeDevCtl {
     int32_t pad0;
     unsigned long long last_poll;  // wall-clock tick of the last read of the host words
     unsigned long long cq_pos;     // online: completions so far (completion queue position)
-    unsigned long long pad[3];
+    unsigned long long page_hint;  // paged planes: where the next page search starts
+    unsigned long long pad[2];
 };
 static_assert(sizeof(ServeDevCtl) == 64, "serving device control block");
 
@@ -77,7 +78,22 @@ struct ServeLaunch {
     uint32_t *stage;                 // device [n_slots][2 * stage_words]
     int32_t stage_words;             // query words per slot (mask words follow)
     int32_t cigar_cap;               // CIGAR runs kept per request
+    // paged traceback planes (page_blocks > 0): out->tb is a pool of n_pages
+    // pages of page_blocks 8-row plane blocks; a wavefront takes the pages of
+    // its request (all or nothing) before the forward pass and frees them after
+    // the walk, so requests are admitted by the planes' HBM footprint.
+    int32_t page_blocks;             // power of two
+    int32_t n_pages;
+    int32_t *page_owner;             // device [n_pages], 0 = free
+    uint64_t alloc_ticks;            // bound of one wait for pages
 };
+
+// pages one request may hold (the wavefront's page list is an LDS array)
+constexpr int kMaxPages = 256;
+
+// Zero a class's page-owner words (a kernel: no memset nodes on a stream a
+// caller might capture).
+int serve_reset_pages(int32_t *owner, int32_t n_pages, hipStream_t s);
 
 // Validates and launches the forward kernel (work-counter reset + the kernel)
 // on stream s; serve == nullptr is the offline batch of gmfa_sw_forward.  The

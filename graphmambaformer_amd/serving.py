@@ -40,6 +40,47 @@ def serve_tile(half_band: int) -> SwTile:
 
 
 @dataclass
+class PageGeometry:
+    """Paged traceback planes of one class: pages of `page_blocks` 8-row
+    plane blocks (a power of two) from a pool of `n_pages`."""
+
+    page_blocks: int
+    n_pages: int
+    max_pages: int
+    page_words: int
+
+    @property
+    def pool_bytes(self) -> int:
+        return self.n_pages * self.page_words * 4
+
+
+def page_geometry(tile: SwTile, max_qlen: int, budget_bytes: int,
+                  page_blocks: Optional[int] = None) -> PageGeometry:
+    """Pages for a plane budget: up to about 1 MB each (the longest query's
+    planes in one page where they fit), the longest query within 256 pages."""
+    blkw = int(tile.lanes_stored) * int(tile.cells_per_lane)  # words per 8-row block
+    max_blocks = max((int(max_qlen) + 7) // 8, 1)
+    if page_blocks is None:
+        pb = 1
+        while pb * 2 * blkw * 4 <= (1 << 20) and pb < max_blocks:
+            pb *= 2
+        while -(-max_blocks // pb) > 256:
+            pb *= 2
+    else:
+        pb = int(page_blocks)
+    page_words = pb * blkw
+    max_pages = -(-max_blocks // pb)
+    if max_pages > 256:
+        raise ValueError(f"the longest query needs {max_pages} pages of {pb} blocks; at most 256")
+    n_pages = int(budget_bytes) // (page_words * 4)
+    if n_pages < max_pages:
+        raise ValueError(f"a plane budget of {budget_bytes} bytes holds {n_pages} pages; the "
+                         f"longest query needs {max_pages}")
+    return PageGeometry(page_blocks=pb, n_pages=n_pages, max_pages=max_pages,
+                        page_words=page_words)
+
+
+@dataclass
 class ResidentReference:
     """A reference held on the device as one 2-bit stream (16 bases per word)."""
 
@@ -99,6 +140,8 @@ class ServeClassRun:
     counter: torch.Tensor
     walk: kernels.SwTraceback
     picks: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    pages: Optional[PageGeometry] = None
+    page_owner: Optional[torch.Tensor] = None
 
     @property
     def n(self) -> int:
@@ -120,6 +163,9 @@ class ServeClassRun:
         o.read_start, o.ref_start, o.counts = ptr(w.read_start), ptr(w.ref_start), ptr(w.counts)
         s.slab_words = int(self.slab_words)
         s.n_slots = int(self.n_slots)
+        if self.pages is not None:
+            s.page_blocks, s.n_pages = self.pages.page_blocks, self.pages.n_pages
+            s.page_owner = ptr(self.page_owner)
         s.arrival_s = self.arrival_s.ctypes.data
         s.complete_s = self.complete_s.ctypes.data
         return s
@@ -127,8 +173,13 @@ class ServeClassRun:
 
 def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks: np.ndarray,
                arrival_s: np.ndarray, scoring: tuple, n_slots: int,
-               tile: Optional[SwTile] = None) -> ServeClassRun:
-    """Request table for pool entries `picks` (in arrival order)."""
+               tile: Optional[SwTile] = None,
+               plane_budget_bytes: Optional[int] = None,
+               page_blocks: Optional[int] = None) -> ServeClassRun:
+    """Request table for pool entries `picks` (in arrival order).  With
+    `plane_budget_bytes`, the traceback planes come from a pool of pages of
+    that size and requests are admitted by their planes' footprint; without,
+    every slot holds a slab for the class's longest query."""
     dev = reference.words.device
     picks = np.asarray(picks, np.int64)
     n = len(picks)
@@ -152,6 +203,10 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
                            tlen_host=tl, boff_host=bo, has_n=False)
     max_q = int(ql.max()) if n else 0
     slab = max(int(kernels.tb_bytes(np.array([max_q]), tile)[0]) // 4, 1)
+    pages = owner = None
+    if plane_budget_bytes is not None:
+        pages = page_geometry(tile, max_q, plane_budget_bytes, page_blocks)
+        owner = torch.zeros(pages.n_pages, dtype=torch.int32, device=dev)
     cap = ql.astype(np.int64) + tl.astype(np.int64) + 1
     coff = np.zeros(n + 1, np.int64)
     np.cumsum(cap, out=coff[1:])
@@ -168,8 +223,10 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
         complete_s=np.full(n, -1.0), score=torch.zeros(n, dtype=torch.int32, device=dev),
         qend=torch.zeros(n, dtype=torch.int32, device=dev),
         tend=torch.zeros(n, dtype=torch.int32, device=dev),
-        tb=torch.empty(int(n_slots) * slab, dtype=torch.int32, device=dev),
-        counter=torch.zeros(16, dtype=torch.int32, device=dev), walk=walk, picks=picks)
+        tb=torch.empty(pages.n_pages * pages.page_words if pages else int(n_slots) * slab,
+                       dtype=torch.int32, device=dev),
+        counter=torch.zeros(16, dtype=torch.int32, device=dev), walk=walk, picks=picks,
+        pages=pages, page_owner=owner)
 
 
 @dataclass
@@ -222,6 +279,8 @@ class OnlineClass:
     ring: Optional[int] = None
     cigar_cap: Optional[int] = None
     tile: Optional[SwTile] = None
+    plane_budget_bytes: Optional[int] = None  # paged planes (admission by footprint)
+    page_blocks: Optional[int] = None  # 8-row blocks per page (default: about 1 MB pages)
 
 
 @dataclass
@@ -277,7 +336,15 @@ class ServeSession:
             max_cap = max(max_cap, cap)
             slab = max(int(kernels.tb_bytes(np.array([int(oc.max_qlen)]), tile)[0]) // 4, 1)
             sw = int(L.gmfa_serve_stage_words(int(oc.max_qlen)))
-            tb = torch.empty(int(oc.n_slots) * slab, dtype=torch.int32, device=dev)
+            pages = None
+            if oc.plane_budget_bytes is not None:
+                pages = page_geometry(tile, int(oc.max_qlen), oc.plane_budget_bytes,
+                                      oc.page_blocks)
+                owner = torch.zeros(pages.n_pages, dtype=torch.int32, device=dev)
+                self._keep.append(owner)
+                tb = torch.empty(pages.n_pages * pages.page_words, dtype=torch.int32, device=dev)
+            else:
+                tb = torch.empty(int(oc.n_slots) * slab, dtype=torch.int32, device=dev)
             stage = torch.zeros(int(oc.n_slots) * 2 * sw, dtype=torch.int32, device=dev)
             head = torch.zeros(8, dtype=torch.int64, device=dev)  # 64-byte control block
             self._keep += [tb, stage, head]
@@ -286,6 +353,9 @@ class ServeSession:
                                            _lib.SW_TRACEBACK | _lib.SW_CIGAR, tile)
             c.max_qlen, c.ring, c.n_slots, c.cigar_cap = int(oc.max_qlen), ring, int(oc.n_slots), cap
             c.tb, c.slab_words, c.stage, c.head = ptr(tb), slab, ptr(stage), ptr(head)
+            if pages is not None:
+                c.page_blocks, c.n_pages = pages.page_blocks, pages.n_pages
+                c.page_owner = ptr(owner)
         self._h = None
         self._results = (_lib.ServeResultRec * int(poll_results))()
         self._cigar = np.zeros(max(max_cap, 1 << 16), np.uint32)
@@ -419,6 +489,9 @@ class ServeSession:
         for k in range(got.value):
             r = self._results[k]
             trunc = r.status == _lib.GMFA_SERVE_TRUNCATED
+            if r.status == _lib.GMFA_SERVE_NO_PAGES:
+                raise RuntimeError(f"class {r.cls} ticket {r.ticket}: no traceback pages "
+                                   "within the wait bound (plane budget too small)")
             runs = self._cigar[r.cigar_off:r.cigar_off + (0 if trunc else r.n_cigar)]
             cig = [(_lib.OP_CHAR[int(v) & 15], int(v) >> 4) for v in runs]
             out.append(ServedAlignment(cls=int(r.cls), ticket=int(r.ticket), score=int(r.score),

@@ -391,6 +391,17 @@ typedef struct gmfa_serve_class {
     int32_t reserved;
     const double *arrival_s; /* host [batch.n_pairs], nondecreasing */
     double *complete_s;      /* host [batch.n_pairs], out (-1: not completed) */
+    /* Paged traceback planes (page_blocks > 0; else per-slot slabs as above):
+     * out.tb is a pool of n_pages pages of page_blocks (a power of two) 8-row
+     * plane blocks, page_blocks * gmfa_sw_tile.lanes_stored *
+     * gmfa_sw_tile.cells_per_lane words each.  A wavefront takes the pages of
+     * its request before the forward pass and frees them after the walk:
+     * requests are admitted by the planes' HBM footprint (waiting while the
+     * pool is short).  page_owner: device int32 [n_pages].  The longest query
+     * may need at most 256 pages. */
+    int32_t page_blocks;
+    int32_t n_pages;
+    int32_t *page_owner;
 } gmfa_serve_class;
 
 typedef struct gmfa_serve_report {
@@ -439,9 +450,15 @@ typedef struct gmfa_serve_class_cfg {
     uint32_t *stage;        /* device: n_slots * 2 * gmfa_serve_stage_words(max_qlen) */
     uint64_t *head;         /* device: 64 bytes, 64-byte aligned (ticket head and
                                device-side mirrors of the control words) */
+    /* paged traceback planes, as in gmfa_serve_class (page_blocks 0: tb holds
+     * n_slots slabs of slab_words) */
+    int32_t page_blocks;
+    int32_t n_pages;
+    int32_t *page_owner;
 } gmfa_serve_class_cfg;
 
 #define GMFA_SERVE_TRUNCATED 1  /* n_cigar > cigar_cap: no runs returned */
+#define GMFA_SERVE_NO_PAGES 2   /* no traceback pages within the wait bound: no outputs */
 
 typedef struct gmfa_serve_result {
     int64_t ticket;         /* as returned by gmfa_serve_submit (per class, from 0) */

@@ -335,3 +335,102 @@ def test_online_submit_many_bursts_match_oracle(mix):
             sess.submit_many(cl[:3], codes, co[:3], bad_ql, to[:3], tl[:3], bo[:3])
         assert sess.stats()["submitted"] == 2
         assert len(sess.wait(2)) == 2
+
+
+def _tight_budgets(pools, factor):
+    """Plane budgets holding about `factor` of each class's longest request."""
+    from graphmambaformer_amd import serving as S
+
+    out = []
+    for (_, hb, qs, _, _, _) in pools:
+        tile = S.serve_tile(hb)
+        mq = max(len(q) for q in qs)
+        blocks = (mq + 7) // 8
+        words = blocks * tile.lanes_stored * tile.cells_per_lane
+        out.append(int(words * 4 * factor) + 1)
+    return out
+
+
+# 8-row blocks per page: the default (pages of about 1 MB, a few per request)
+# and pages small enough that the long and noisy classes' longest requests
+# hold more pages than their tiles have stored band lanes (57 and 63), so
+# page numbers past every stored lane are looked up during the plane store
+PAGE_BLOCKS = {"default": (None, None, None), "small": (1, 2, 2)}
+
+
+def _check_many_pages(pages_by_class, tag):
+    from graphmambaformer_amd import serving as S
+
+    if tag != "small":
+        return
+    _, pools = _small_mix()
+    for c in (1, 2):
+        lanes = S.serve_tile(pools[c][1]).lanes_stored
+        assert pages_by_class[c] > lanes, (c, pages_by_class[c], lanes)
+
+
+@pytest.mark.parametrize("tag", sorted(PAGE_BLOCKS))
+def test_serving_paged_planes_tight_budget_matches_oracle(mix, tag):
+    """Staged serving with the traceback planes in a page pool that holds
+    about two of a class's longest requests: requests wait for pages
+    (admission by footprint) and every result equals the oracle's."""
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    budgets = _tight_budgets(pools, 2.5)
+    picks = [np.arange(len(e)) for e in exp]
+    arrivals = [np.zeros(len(p)) for p in picks]
+    classes = [S.make_class(f"paged{c}", ref_dev, pools_dev[c], picks[c], arrivals[c], SC, n,
+                            plane_budget_bytes=budgets[c], page_blocks=PAGE_BLOCKS[tag][c])
+               for c, n in enumerate((16, 16, 8))]
+    _check_many_pages([c.pages.max_pages for c in classes], tag)
+    _say(f"case: staged serving, paged planes ({tag} pages)",
+         [(c.pages.page_blocks, c.pages.n_pages, c.pages.max_pages) for c in classes])
+    res = S.run(classes, timeout_s=60.0)
+    assert res.completed == sum(c.n for c in classes) and not res.timed_out
+    assert not res.waves_gave_up
+    for c, e in zip(classes, exp):
+        _check(c, e, c.name)
+
+
+@pytest.mark.parametrize("tag", sorted(PAGE_BLOCKS))
+def test_online_paged_planes_tight_budget_matches_oracle(mix, tag):
+    """Online sessions with paged planes and tight budgets: every request
+    completes and equals the oracle's."""
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    budgets = _tight_budgets(pools, 2.5)
+    classes = _online_classes(pools, slots=(8, 8, 4), rings=(64, 32, 16))
+    for oc, b, pb in zip(classes, budgets, PAGE_BLOCKS[tag]):
+        oc.plane_budget_bytes, oc.page_blocks = b, pb
+    geo = [S.page_geometry(S.serve_tile(oc.half_band), oc.max_qlen, oc.plane_budget_bytes,
+                           oc.page_blocks) for oc in classes]
+    _check_many_pages([g.max_pages for g in geo], tag)
+    rng = np.random.default_rng(17)
+    _say(f"case: online session, paged planes ({tag} pages)",
+         [(g.page_blocks, g.n_pages, g.max_pages) for g in geo])
+    tickets, results = {}, []
+    with S.ServeSession(ref_dev, classes, SC, idle_timeout_s=1.0) as sess:
+        for c in rng.integers(0, 3, size=160):
+            _, hb, qs, pos, lens, offs = pools[c]
+            k = int(rng.integers(0, len(qs)))
+            t = sess.submit(int(c), qs[k], int(pos[k]), int(lens[k]), int(offs[k]))
+            tickets[(int(c), t)] = k
+            results += sess.poll()
+        results += sess.wait(len(tickets) - len(results), timeout_s=60.0)
+    assert len(results) == len(tickets)
+    _check_online(results, tickets, exp, "online-paged")
+
+
+def test_paged_budget_below_one_request_is_rejected(mix):
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    budgets = _tight_budgets(pools, 0.5)
+    with pytest.raises(ValueError):
+        S.make_class("x", ref_dev, pools_dev[1], np.arange(3), np.zeros(3), SC, 4,
+                     plane_budget_bytes=budgets[1])
