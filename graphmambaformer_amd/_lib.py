@@ -77,7 +77,8 @@ class ServeClass(ctypes.Structure):
     _fields_ = [("batch", SwBatch), ("params", SwParams), ("out", SwOut),
                 ("slab_words", _i64), ("n_slots", _i32), ("reserved", _i32),
                 ("arrival_s", _P), ("complete_s", _P),
-                ("page_blocks", _i32), ("n_pages", _i32), ("page_owner", _P)]
+                ("page_blocks", _i32), ("n_pages", _i32), ("page_owner", _P),
+                ("page_wait_s", ctypes.c_double)]
 
 
 class ServeReport(ctypes.Structure):
@@ -88,7 +89,8 @@ class ServeReport(ctypes.Structure):
 class ServeClassCfg(ctypes.Structure):
     _fields_ = [("params", SwParams), ("max_qlen", _i32), ("ring", _i32), ("n_slots", _i32),
                 ("cigar_cap", _i32), ("tb", _P), ("slab_words", _i64), ("stage", _P),
-                ("head", _P), ("page_blocks", _i32), ("n_pages", _i32), ("page_owner", _P)]
+                ("head", _P), ("page_blocks", _i32), ("n_pages", _i32), ("page_owner", _P),
+                ("page_wait_s", ctypes.c_double)]
 
 
 class ServeResultRec(ctypes.Structure):

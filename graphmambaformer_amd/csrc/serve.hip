@@ -98,6 +98,9 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
         GMFA_REQUIRE(np >= 0, "class %d: negative n_pairs", c);
         if (np == 0) continue;
         GMFA_REQUIRE(k.arrival_s && k.complete_s, "class %d: NULL arrival_s / complete_s", c);
+        GMFA_REQUIRE(std::isfinite(k.page_wait_s) && k.page_wait_s >= 0.0 &&
+                         k.page_wait_s <= 3600.0,
+                     "class %d: page_wait_s must be in [0, 3600]", c);
         for (int i = 0; i < np; ++i) {
             const double t = k.arrival_s[i];
             GMFA_REQUIRE(std::isfinite(t) && t >= 0.0 && (i == 0 || t >= k.arrival_s[i - 1]),
@@ -171,7 +174,8 @@ extern "C" int gmfa_serve_run(const gmfa_serve_class *classes, int32_t n_classes
         sl.page_blocks = k.page_blocks;
         sl.n_pages = k.n_pages;
         sl.page_owner = k.page_owner;
-        sl.alloc_ticks = (uint64_t)((timeout_s + 5.0) * (double)khz * 1000.0);
+        sl.alloc_ticks = (uint64_t)((k.page_wait_s > 0.0 ? k.page_wait_s : timeout_s + 5.0) *
+                                    (double)khz * 1000.0);
         if (k.page_blocks > 0) {
             const int rc = serve_reset_pages(k.page_owner, k.n_pages, streams[c]);
             if (rc != GMFA_OK) return finish(rc);
@@ -383,6 +387,9 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
         GMFA_REQUIRE(k.cigar_cap >= 1 && k.cigar_cap <= (1 << 22),
                      "class %d: cigar_cap %d outside [1, 2^22]", c, k.cigar_cap);
         GMFA_REQUIRE(k.tb && k.stage && k.head, "class %d: NULL tb / stage / head", c);
+        GMFA_REQUIRE(std::isfinite(k.page_wait_s) && k.page_wait_s >= 0.0 &&
+                         k.page_wait_s <= 3600.0,
+                     "class %d: page_wait_s must be in [0, 3600]", c);
         GMFA_REQUIRE(!(k.params.flags & GMFA_SW_NMASK) || ref_mask,
                      "class %d: GMFA_SW_NMASK needs ref_mask", c);
     }
@@ -476,8 +483,11 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
         sl.page_blocks = k.page_blocks;
         sl.n_pages = k.n_pages;
         sl.page_owner = k.page_owner;
-        // a wait for pages is bounded by the longer of the idle bound and 60 s
-        sl.alloc_ticks = (uint64_t)(std::max(idle_timeout_s, 60.0) * (double)khz * 1000.0);
+        // a wait for pages is bounded by page_wait_s, by default the longer of
+        // the idle bound and 60 s
+        sl.alloc_ticks = (uint64_t)((k.page_wait_s > 0.0 ? k.page_wait_s
+                                                         : std::max(idle_timeout_s, 60.0)) *
+                                    (double)khz * 1000.0);
 
         if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
             c.stream = nullptr;

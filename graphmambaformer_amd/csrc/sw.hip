@@ -868,7 +868,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         // pass (admission by the planes' HBM footprint)
         int32_t *pl = nullptr;
         int need_pages = 0;
-        bool paged = false;
+        bool paged = false, nopages = false;
         if constexpr (SV && TB) {
             __shared__ int32_t page_list[kMaxPages];
             pl = page_list;
@@ -877,30 +877,15 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 need_pages = (((qlen + 7) >> 3) + (1 << a.page_shift) - 1) >> a.page_shift;
                 if (!take_pages(a.page_owner, a.n_pages, need_pages, &a.dctl->page_hint, pl,
                                 a.alloc_ticks)) {
-                    // no pages within the bound: the request completes without
-                    // outputs (n_cigar = -1) and the host is told
+                    // no pages within the bound: the request skips the forward
+                    // pass (best cell at row 0, so the walk reads no plane) and
+                    // completes through the normal path with n_cigar = -1; the
+                    // host is told
                     if (lane == 0)
                         __hip_atomic_store(a.serve_gave_up, 1, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_SYSTEM);
-                    if (online) {
-                        if (lane == 0) {
-                            ServeRes *res = a.ring_res + ring_r;
-                            res->score = 0; res->query_end = 0; res->target_end = 0;
-                            res->read_start = 0; res->ref_start = 0; res->n_cigar = -1;
-                            __hip_atomic_store(a.ring_done + ring_r, ticket + 1, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_SYSTEM);
-                            const unsigned long long pos = atomicAdd(&a.dctl->cq_pos, 1ull);
-                            __hip_atomic_store(
-                                a.ring_cq + (pos & (unsigned long long)(a.ring_n - 1)),
-                                ((uint64_t)(pos + 1) << 32) | (uint32_t)ring_r, __ATOMIC_RELEASE,
-                                __HIP_MEMORY_SCOPE_SYSTEM);
-                        }
-                    } else if (lane == 0) {
-                        a.wo.ncig[p] = -1;
-                        __hip_atomic_store(a.serve_done + p, 1, __ATOMIC_RELEASE,
-                                           __HIP_MEMORY_SCOPE_SYSTEM);
-                    }
-                    continue;
+                    nopages = true;
+                    need_pages = 0;
                 }
             }
         }
@@ -913,7 +898,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         for (int k = 0; k < C; ++k) { Hp[k] = 0; Fp[k] = kNeg; acc[k] = 0u; }
         int bestK = 31, besti = 0;
 
-        const int nblk = (maxq + 7) >> 3;
+        const int nblk = nopages ? 0 : (maxq + 7) >> 3;
         const int nblk_mine = (qlen + 7) >> 3;
         // serving: each wave slot owns one plane slab, reused pair after pair
         const int64_t tb_base = (TB && has) ? (SV ? (int64_t)blockIdx.x * a.slab_words : a.tb_off[p])
@@ -1162,10 +1147,18 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 if (paged) {
                     const int64_t blkstride = (int64_t)(lstar + 1) * C;
+                    // the walk moves up a block at a time: look the page up
+                    // only when it changes
+                    int cur_pg = -1;
+                    const uint32_t *pbase = a.tb;
                     auto ldp = [&](int blk, int d) {
-                        const uint32_t pgp = page_of(pl, blk >> a.page_shift);
-                        return (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tb[
-                            (int64_t)pgp * page_words + (int64_t)(blk & page_mask) * blkstride + d]);
+                        const int pgi = blk >> a.page_shift;
+                        if (pgi != cur_pg) {
+                            cur_pg = pgi;
+                            pbase = a.tb + (int64_t)page_of(pl, pgi) * page_words;
+                        }
+                        return (uint32_t)__builtin_amdgcn_readfirstlane(
+                            (int)pbase[(int64_t)(blk & page_mask) * blkstride + d]);
                     };
                     walk_pair_ring_paged<C>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap,
                                             a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s,
@@ -1175,6 +1168,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                                       a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s, off,
                                       qo, to, a.tb + tb_base, claim_tick);
                 }
+                if (nopages && lane == 0) a.ring_res[ring_r].n_cigar = -1;
                 // every output of the request (stored by the wave's lanes) before
                 // its done word and its completion-queue record
                 __syncthreads();
@@ -1211,11 +1205,16 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     const int64_t tbg = readlane64(tb_base, L0);
                     if (paged) {
                         const int64_t blkstride = (int64_t)wc.Gs * C;
+                        int cur_pg = -1;
+                        const uint32_t *pbase = a.tb;
                         auto ldp = [&](int blk, int d) {
-                            const uint32_t pgp = page_of(pl, blk >> a.page_shift);
-                            return (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tb[
-                                (int64_t)pgp * page_words + (int64_t)(blk & page_mask) * blkstride +
-                                d]);
+                            const int pgi = blk >> a.page_shift;
+                            if (pgi != cur_pg) {
+                                cur_pg = pgi;
+                                pbase = a.tb + (int64_t)page_of(pl, pgi) * page_words;
+                            }
+                            return (uint32_t)__builtin_amdgcn_readfirstlane(
+                                (int)pbase[(int64_t)(blk & page_mask) * blkstride + d]);
                         };
                         walk_pair_paged<C>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
                                            __builtin_amdgcn_readlane(res_j, L0),
@@ -1231,6 +1230,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             }
         }
         if constexpr (SV) {
+            if (nopages && lane == 0) a.wo.ncig[p] = -1;
             __syncthreads();  // the walk's stores by every lane, then the flag
             if (has && lane == 0) {
                 // every output of this pair (stored by this wave) before the flag

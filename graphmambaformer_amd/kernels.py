@@ -429,9 +429,11 @@ def sw_traceback(inp: SwInputs, fwd: SwForward, score=None, query_end=None, targ
 
 def decode_cigars(tbk: SwTraceback) -> list[list[tuple[str, int]]]:
     """Device CIGAR runs -> list of [(op, n)] per pair (the runs are gathered
-    on the device, so only they cross to the host)."""
+    on the device, so only they cross to the host).  A pair without outputs
+    (n_cigar < 0: a served request that found no traceback pages) decodes as
+    an empty list."""
     off = tbk.cigar_off.to(torch.int64)
-    nc = tbk.n_cigar.to(torch.int64)
+    nc = tbk.n_cigar.to(torch.int64).clamp_min(0)
     n = int(nc.numel())
     if n == 0:
         return []

@@ -142,6 +142,7 @@ class ServeClassRun:
     picks: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
     pages: Optional[PageGeometry] = None
     page_owner: Optional[torch.Tensor] = None
+    page_wait_s: float = 0.0
 
     @property
     def n(self) -> int:
@@ -166,6 +167,7 @@ class ServeClassRun:
         if self.pages is not None:
             s.page_blocks, s.n_pages = self.pages.page_blocks, self.pages.n_pages
             s.page_owner = ptr(self.page_owner)
+            s.page_wait_s = float(self.page_wait_s)
         s.arrival_s = self.arrival_s.ctypes.data
         s.complete_s = self.complete_s.ctypes.data
         return s
@@ -175,7 +177,8 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
                arrival_s: np.ndarray, scoring: tuple, n_slots: int,
                tile: Optional[SwTile] = None,
                plane_budget_bytes: Optional[int] = None,
-               page_blocks: Optional[int] = None) -> ServeClassRun:
+               page_blocks: Optional[int] = None,
+               page_wait_s: float = 0.0) -> ServeClassRun:
     """Request table for pool entries `picks` (in arrival order).  With
     `plane_budget_bytes`, the traceback planes come from a pool of pages of
     that size and requests are admitted by their planes' footprint; without,
@@ -226,7 +229,7 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
         tb=torch.empty(pages.n_pages * pages.page_words if pages else int(n_slots) * slab,
                        dtype=torch.int32, device=dev),
         counter=torch.zeros(16, dtype=torch.int32, device=dev), walk=walk, picks=picks,
-        pages=pages, page_owner=owner)
+        pages=pages, page_owner=owner, page_wait_s=page_wait_s)
 
 
 @dataclass
@@ -281,6 +284,7 @@ class OnlineClass:
     tile: Optional[SwTile] = None
     plane_budget_bytes: Optional[int] = None  # paged planes (admission by footprint)
     page_blocks: Optional[int] = None  # 8-row blocks per page (default: about 1 MB pages)
+    page_wait_s: float = 0.0  # bound of one wait for pages (0: the library's default)
 
 
 @dataclass
@@ -295,6 +299,7 @@ class ServedAlignment:
     cigar: list
     counts: tuple
     truncated: bool
+    no_pages: bool = False  # no traceback pages within the wait bound: no outputs
 
 
 def _pow2_at_least(n: int) -> int:
@@ -356,6 +361,7 @@ class ServeSession:
             if pages is not None:
                 c.page_blocks, c.n_pages = pages.page_blocks, pages.n_pages
                 c.page_owner = ptr(owner)
+                c.page_wait_s = float(oc.page_wait_s)
         self._h = None
         self._results = (_lib.ServeResultRec * int(poll_results))()
         self._cigar = np.zeros(max(max_cap, 1 << 16), np.uint32)
@@ -489,14 +495,12 @@ class ServeSession:
         for k in range(got.value):
             r = self._results[k]
             trunc = r.status == _lib.GMFA_SERVE_TRUNCATED
-            if r.status == _lib.GMFA_SERVE_NO_PAGES:
-                raise RuntimeError(f"class {r.cls} ticket {r.ticket}: no traceback pages "
-                                   "within the wait bound (plane budget too small)")
-            runs = self._cigar[r.cigar_off:r.cigar_off + (0 if trunc else r.n_cigar)]
+            nopages = r.status == _lib.GMFA_SERVE_NO_PAGES
+            runs = self._cigar[r.cigar_off:r.cigar_off + (0 if trunc or nopages else r.n_cigar)]
             cig = [(_lib.OP_CHAR[int(v) & 15], int(v) >> 4) for v in runs]
             out.append(ServedAlignment(cls=int(r.cls), ticket=int(r.ticket), score=int(r.score),
                                        query_end=int(r.query_end), target_end=int(r.target_end),
                                        read_start=int(r.read_start), ref_start=int(r.ref_start),
                                        cigar=cig, counts=tuple(int(x) for x in r.counts),
-                                       truncated=trunc))
+                                       truncated=trunc, no_pages=nopages))
         return out

@@ -398,10 +398,13 @@ typedef struct gmfa_serve_class {
      * its request before the forward pass and frees them after the walk:
      * requests are admitted by the planes' HBM footprint (waiting while the
      * pool is short).  page_owner: device int32 [n_pages].  The longest query
-     * may need at most 256 pages. */
+     * may need at most 256 pages.  page_wait_s bounds one request's wait for
+     * pages (0: timeout_s + 5 s); past it the request completes with no
+     * outputs (out.n_cigar = -1) and the report's waves_gave_up is set. */
     int32_t page_blocks;
     int32_t n_pages;
     int32_t *page_owner;
+    double page_wait_s;
 } gmfa_serve_class;
 
 typedef struct gmfa_serve_report {
@@ -455,6 +458,8 @@ typedef struct gmfa_serve_class_cfg {
     int32_t page_blocks;
     int32_t n_pages;
     int32_t *page_owner;
+    double page_wait_s;  /* bound of one wait for pages (0: the longer of the
+                            idle bound and 60 s); past it: GMFA_SERVE_NO_PAGES */
 } gmfa_serve_class_cfg;
 
 #define GMFA_SERVE_TRUNCATED 1  /* n_cigar > cigar_cap: no runs returned */

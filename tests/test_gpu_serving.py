@@ -163,7 +163,7 @@ def _check_online(results, tickets, exp, label):
         k = tickets[(r.cls, r.ticket)]
         e = exp[r.cls][k]
         ctx = (label, r.cls, r.ticket, k)
-        assert not r.truncated, ctx
+        assert not r.truncated and not r.no_pages, ctx
         assert r.score == e["score"], ctx
         assert r.query_end == e["read_end"] and r.target_end == e["ref_end"], ctx
         assert r.cigar == e["cigar"], ctx
@@ -423,6 +423,75 @@ def test_online_paged_planes_tight_budget_matches_oracle(mix, tag):
         results += sess.wait(len(tickets) - len(results), timeout_s=60.0)
     assert len(results) == len(tickets)
     _check_online(results, tickets, exp, "online-paged")
+
+
+def _contended_noisy(pools):
+    """The noisy class's requests that each need more than half of a pool
+    holding exactly the longest one's pages, with that pool's budget: at
+    most one of them holds pages at a time."""
+    from graphmambaformer_amd import serving as S
+
+    _, hb, qs, _, _, _ = pools[2]
+    tile = S.serve_tile(hb)
+    mq = max(len(q) for q in qs)
+    geo = S.page_geometry(tile, mq, 1 << 40, page_blocks=2)
+    budget = geo.max_pages * geo.page_words * 4
+    need = np.array([((len(q) + 7) // 8 + 1) // 2 for q in qs])
+    big = np.flatnonzero(2 * need > geo.max_pages)
+    assert len(big) >= 3
+    return big, budget
+
+
+def test_serving_no_pages_within_bound_completes_without_outputs(mix):
+    """Staged serving with a page pool that holds one of the contending
+    requests at a time and a 20 us wait bound: the requests that find no
+    pages complete with n_cigar = -1 (and the run reports waves that gave
+    up); every other request equals the oracle."""
+    from graphmambaformer_amd import kernels as K
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    big, budget = _contended_noisy(pools)
+    cls = S.make_class("nopages", ref_dev, pools_dev[2], big, np.zeros(len(big)), SC, len(big),
+                       plane_budget_bytes=budget, page_blocks=2, page_wait_s=20e-6)
+    _say("case: staged serving, no pages within 20 us", (len(big), cls.pages.n_pages))
+    res = S.run([cls], timeout_s=60.0)
+    assert res.completed == len(big) and not res.timed_out
+    ncig = cls.walk.n_cigar.cpu().numpy()
+    failed = np.flatnonzero(ncig < 0)
+    assert len(failed) >= 1 and res.waves_gave_up
+    score = cls.score.cpu().numpy()
+    cig = K.decode_cigars(cls.walk)
+    for k, pidx in enumerate(cls.picks):
+        if ncig[k] < 0:
+            continue
+        e = exp[2][int(pidx)]
+        assert int(score[k]) == e["score"] and cig[k] == e["cigar"], (k, int(pidx))
+
+
+def test_online_no_pages_within_bound_is_reported(mix):
+    """Online: requests that find no pages within page_wait_s come back with
+    no_pages set and no CIGAR; the others equal the oracle."""
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    _, pools = _small_mix()
+    big, budget = _contended_noisy(pools)
+    _, hb, qs, pos, lens, offs = pools[2]
+    oc = S.OnlineClass(half_band=hb, max_qlen=max(len(q) for q in qs), n_slots=len(big),
+                       ring=16, plane_budget_bytes=budget, page_blocks=2, page_wait_s=20e-6)
+    _say("case: online session, no pages within 20 us", len(big))
+    tickets, results = {}, []
+    with S.ServeSession(ref_dev, [oc], SC, idle_timeout_s=1.0) as sess:
+        for k in big:
+            tickets[(0, sess.submit(0, qs[k], int(pos[k]), int(lens[k]), int(offs[k])))] = int(k)
+        results += sess.wait(len(big), timeout_s=60.0)
+    assert len(results) == len(big)
+    failed = [r for r in results if r.no_pages]
+    assert len(failed) >= 1
+    assert all(r.cigar == [] for r in failed)
+    _check_online([r for r in results if not r.no_pages], tickets, [exp[2]], "online-nopages")
 
 
 def test_paged_budget_below_one_request_is_rejected(mix):
