@@ -743,12 +743,91 @@ __global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
     a.o.counts[4 * p + 3] = cnt[3];
 }
 
+// One row of the scalar cell of KS == C tiles, on the scalar unit (the
+// vector unit bounds the kernel).  Inputs and state are wave-uniform:
+//   dg = H(i-1, d) + (match ? ma + mi : 0)      (row-shifted domain)
+//   f  = max(zprev - cH, kNeg - cF)               (no band cell at d+1)
+//   e  = g63 + max(d63, 0) - (go + (C-1)*ge)      (lane 63's prefix, one on)
+//   h  = max(dg, f, zrow, e); out of the matrix (xl < 0): h = zrow, f = kNeg
+// then the plane bits (f_cont 0: F(i-1, d+1) is out of band; not_diag,
+// not_f, e_cont = sign of lane 63's A - G) and the first strict maximum.
+// Written as one asm block: compiled from C, the compares and the max chain
+// went to the vector unit.
+__device__ __forceinline__ void scalar_cell(int &Hs, int &Fs, uint32_t &acc, int &bK, int &bi,
+                                            int tf, int qc, int mf, int mms, int zprev, int zrow,
+                                            int cH, int bv, int d63, int g63, int k1, int imi5,
+                                            int i, int xl, int neg)
+{
+    int t0, t1, dg, fs, hs;
+    // the asm takes SGPRs; values the compiler placed in VGPRs (the carried
+    // state also feeds the DPP "old" operand) move over here, the others
+    // cost nothing
+#define GMFA_UNI(x) x = __builtin_amdgcn_readfirstlane(x)
+    GMFA_UNI(Hs); GMFA_UNI(Fs); GMFA_UNI(bK); GMFA_UNI(bi);
+    acc = (uint32_t)__builtin_amdgcn_readfirstlane((int)acc);
+    GMFA_UNI(tf); GMFA_UNI(qc); GMFA_UNI(mf); GMFA_UNI(mms); GMFA_UNI(zprev); GMFA_UNI(zrow);
+    GMFA_UNI(cH); GMFA_UNI(bv); GMFA_UNI(d63); GMFA_UNI(g63); GMFA_UNI(k1); GMFA_UNI(imi5);
+    GMFA_UNI(i); GMFA_UNI(xl); GMFA_UNI(neg);
+#undef GMFA_UNI
+    asm("s_cmp_eq_u32 %[tf], %[qc]\n\t"
+        "s_cselect_b32 %[t0], %[mms], 0\n\t"
+        "s_cmp_eq_u32 %[mf], 0\n\t"
+        "s_cselect_b32 %[t0], %[t0], 0\n\t"
+        "s_add_i32 %[dg], %[Hs], %[t0]\n\t"
+        "s_sub_i32 %[fs], %[zprev], %[cH]\n\t"
+        "s_max_i32 %[fs], %[fs], %[bv]\n\t"
+        "s_max_i32 %[t0], %[d63], 0\n\t"
+        "s_add_i32 %[t0], %[t0], %[g63]\n\t"
+        "s_sub_i32 %[t0], %[t0], %[k1]\n\t"
+        "s_max_i32 %[hs], %[dg], %[fs]\n\t"
+        "s_max_i32 %[hs], %[hs], %[zrow]\n\t"
+        "s_max_i32 %[hs], %[hs], %[t0]\n\t"
+        "s_cmp_lt_i32 %[xl], 0\n\t"
+        "s_cselect_b32 %[hs], %[zrow], %[hs]\n\t"
+        "s_cselect_b32 %[fs], %[neg], %[fs]\n\t"
+        "s_sub_i32 %[t0], %[dg], %[hs]\n\t"
+        "s_lshr_b32 %[t0], %[t0], 31\n\t"
+        "s_lshl_b32 %[t0], %[t0], 2\n\t"
+        "s_sub_i32 %[t1], %[fs], %[hs]\n\t"
+        "s_lshr_b32 %[t1], %[t1], 31\n\t"
+        "s_lshl_b32 %[t1], %[t1], 1\n\t"
+        "s_or_b32 %[t0], %[t0], %[t1]\n\t"
+        "s_lshr_b32 %[t1], %[d63], 31\n\t"
+        "s_or_b32 %[t0], %[t0], %[t1]\n\t"
+        "s_lshl_b32 %[acc], %[acc], 4\n\t"
+        "s_or_b32 %[acc], %[acc], %[t0]\n\t"
+        "s_lshl_b32 %[t0], %[hs], 5\n\t"
+        "s_or_b32 %[t0], %[t0], 31\n\t"
+        "s_sub_i32 %[t0], %[t0], %[imi5]\n\t"
+        "s_or_b32 %[t1], %[bK], 31\n\t"
+        "s_cmp_gt_i32 %[t0], %[t1]\n\t"
+        "s_cselect_b32 %[bK], %[t0], %[bK]\n\t"
+        "s_cselect_b32 %[bi], %[i], %[bi]\n\t"
+        "s_mov_b32 %[Hs], %[hs]\n\t"
+        "s_mov_b32 %[Fs], %[fs]"
+        : [Hs] "+s"(Hs), [Fs] "+s"(Fs), [acc] "+s"(acc), [bK] "+s"(bK), [bi] "+s"(bi),
+          [t0] "=&s"(t0), [t1] "=&s"(t1), [dg] "=&s"(dg), [fs] "=&s"(fs), [hs] "=&s"(hs)
+        : [tf] "s"(tf), [qc] "s"(qc), [mf] "s"(mf), [mms] "s"(mms), [zprev] "s"(zprev),
+          [zrow] "s"(zrow), [cH] "s"(cH), [bv] "s"(bv), [d63] "s"(d63), [g63] "s"(g63),
+          [k1] "s"(k1), [imi5] "s"(imi5), [i] "s"(i), [xl] "s"(xl), [neg] "s"(neg)
+        : "scc");
+}
+
 // SV: serving variant (persistent queue fed by the host; staged requests of
 // gmfa_serve_run or the online ring of gmfa_serve_open), a separate
 // instantiation so the offline kernels carry none of its state.
+//
+// KS == C (XS): a band one diagonal wider than the lanes hold, W = C*64 + 1
+// (half_band = 32*C, e.g. 256 -> 8x64 + 1).  The 64 lanes own diagonals
+// 0 .. W-2 with no dead lane; diagonal W-1 is a scalar cell (lane 64,
+// register 0 of the plane layout) computed on the scalar unit from lane 63's
+// prefix and window, while the vector unit, the bound of this kernel, does
+// 8 cells per lane instead of 9 with 7 dead lanes.
 template <int C, int G, int KS, bool TB, bool NM, bool RK, bool SV = false>
 __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 {
+    constexpr bool XS = KS == C;
+    static_assert(!XS || (G == 64 && C + 8 <= 32), "scalar-cell tiles: 64 lanes, C <= 24");
     constexpr int NCW = (C + 15) / 16;  // target words feeding the cells
     constexpr int NA = NCW + 1;         // aligned window words
     constexpr int NWR = NA + 1;         // raw words loaded per block
@@ -897,6 +976,11 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 #pragma unroll
         for (int k = 0; k < C; ++k) { Hp[k] = 0; Fp[k] = kNeg; acc[k] = 0u; }
         int bestK = 31, besti = 0;
+        // XS: the scalar cell's H and F of the previous row, plane bits and
+        // first maximum (wave-uniform)
+        int Hs = 0, Fs = kNeg, bestKs = 31, bestis = 0;
+        uint32_t accs = 0u;
+        (void)Hs; (void)Fs; (void)bestKs; (void)bestis; (void)accs;
 
         const int nblk = nopages ? 0 : (maxq + 7) >> 3;
         const int nblk_mine = (qlen + 7) >> 3;
@@ -947,6 +1031,18 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     if (NM) na[x] = __builtin_amdgcn_alignbit(wn[x + 1], wn[x], sh);
                 }
             }
+            // XS: lane 63's window also holds the scalar cell's bases (its
+            // cells C .. C+7 over the block's rows)
+            uint64_t tws = 0, nws = 0;
+            int rks[8];
+            (void)tws; (void)nws; (void)rks;
+            if constexpr (XS) {
+                tws = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ta[1], G - 1) << 32) |
+                      (uint32_t)__builtin_amdgcn_readlane((int)ta[0], G - 1);
+                if (NM)
+                    nws = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)na[1], G - 1) << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane((int)na[0], G - 1);
+            }
             e_t += 8;
             if (b + 1 < nblk) {
                 load_t(e_t, wt, a.tseq);
@@ -996,8 +1092,9 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     lo = l0 - gl * C;
                     span = (uint32_t)max(h0 - l0 + 1, 0);
                 }
-                int hN = from_next_lane(zprev, Hp[0]);
-                int fN = from_next_lane(kNeg, Fp[0]);
+                // lane 63's d+1: the band guard, or the scalar cell (XS)
+                int hN = from_next_lane(XS ? Hs : zprev, Hp[0]);
+                int fN = from_next_lane(XS ? Fs : kNeg, Fp[0]);
                 hN = guard ? zprev : hN;
                 fN = guard ? kNeg : fN;
 
@@ -1033,7 +1130,29 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 #pragma unroll
                 for (int k = 1; k < C; ++k) Gv[k] = max(Gv[k - 1], Av[k - 1]);
                 const int Glast = Gv[C - 1];
-                const int dprev = from_prev_lane(Av[C - 1] - Glast);
+                const int dlast = Av[C - 1] - Glast;
+                const int dprev = from_prev_lane(dlast);
+                if constexpr (XS) {
+                    // the scalar cell d = W-1 as register C of lane 63: E from
+                    // that lane's prefix, no band cell at d+1 (H 0 and F -inf
+                    // there, as for a guard lane)
+                    const int d63 = __builtin_amdgcn_readlane(dlast, G - 1);
+                    const int g63 = __builtin_amdgcn_readlane(Glast, G - 1);
+                    const int tf = (int)((uint32_t)(tws >> (2 * (C + r))) & 3u);
+                    const int mf = NM ? (int)(((uint32_t)qn) | ((uint32_t)(nws >> (2 * (C + r))) & 1u))
+                                      : 0;
+                    // SLOW: the cell is out of the matrix unless every term is >= 0
+                    const int xl = SLOW ? (int)((uint32_t)(qlen - i) |
+                                                (uint32_t)(tlen + hb - off - i - (W - 1)) |
+                                                (uint32_t)((W - 1) - (hb + 1 - off - i)))
+                                        : 0;
+                    scalar_cell(Hs, Fs, accs, bestKs, bestis, tf, (int)qc, mf, (int)mms, zprev,
+                                zrow, cH, kNeg - cF, d63, g63, go + (C - 1) * ge,
+                                (int)((uint32_t)(i * mi) << 5), i, xl, kNeg);
+                    if constexpr (RK)
+                        rks[r] = (int)(((uint32_t)(Hs - i * mi) << 11) |
+                                       (uint32_t)(2047 - (W - 1)));
+                }
 
                 int rk = INT_MIN;
                 int Gkm1 = Xl;
@@ -1098,6 +1217,10 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 // independent scans off the rows' H dependency chain
 #pragma unroll
                 for (int r = 0; r < 8; ++r) rkeys[r] = seg_scan_max<G>(rkeys[r]);
+                if constexpr (XS) {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) rkeys[r] = max(rkeys[r], rks[r]);
+                }
                 if (has && gl == G - 1) {
 #pragma unroll
                     for (int r = 0; r < 8; ++r)
@@ -1113,6 +1236,9 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 }
 #pragma unroll
                 for (int k = 0; k < C; ++k) dst[k] = acc[k];
+                // XS: the scalar cell's plane dword, lane 64's register 0
+                if constexpr (XS)
+                    if (gl == G - 1) dst[C] = accs;
             }
         }
 
@@ -1123,6 +1249,16 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         uint64_t key = ((uint64_t)(uint32_t)bH << 40) |
                        ((uint64_t)(0xFFFFFu - (uint32_t)besti) << 12) | (uint64_t)(0xFFFu - (uint32_t)bd);
         if (bH == 0) key = 0;
+        if constexpr (XS) {
+            // the scalar cell's first maximum (diagonal W-1) joins lane 0's
+            const int bHs = bestKs >> 5;
+            if (bestis != 0 && bHs != 0 && gl == 0) {
+                const uint64_t ks = ((uint64_t)(uint32_t)bHs << 40) |
+                                    ((uint64_t)(0xFFFFFu - (uint32_t)bestis) << 12) |
+                                    (uint64_t)(0xFFFu - (uint32_t)(W - 1));
+                key = ks > key ? ks : key;
+            }
+        }
 #pragma unroll
         for (int o = G / 2; o >= 1; o >>= 1) {
             const uint32_t lo32 = (uint32_t)__shfl_xor((int)(uint32_t)key, o);
@@ -1446,7 +1582,7 @@ SwKernel pick_kernel(uint32_t flags)
 template <int C, int G, int KS>
 SwKernel pick_serve_kernel(uint32_t flags)
 {
-    if constexpr (G == 64) {
+    if constexpr (G == 64 && KS != C) {
         if (flags & GMFA_SW_NMASK) return sw_forward_kernel<C, G, KS, true, true, false, true>;
         return sw_forward_kernel<C, G, KS, true, false, false, true>;
     }
@@ -1465,9 +1601,9 @@ struct TileEntry {
 #define GMFA_TILE(C, G, KS) \
     {C, G, KS, &pick_kernel<C, G, KS>, &sw_traceback_kernel<C>, &pick_serve_kernel<C, G, KS>}
 static const TileEntry kTiles[] = {
-    // exact fits of the benchmark bands: W=65 (5x13), W=513 (9x57); W=1503 (24x63, 15 live
-    // registers in the last lane)
-    GMFA_TILE(5, 16, 4),   GMFA_TILE(9, 64, 8),  GMFA_TILE(24, 64, 14),
+    // exact fits of the benchmark bands: W=65 (5x13), W=513 (9x57, or 8x64 + the scalar
+    // cell); W=1503 (24x63, 15 live registers in the last lane)
+    GMFA_TILE(5, 16, 4),   GMFA_TILE(9, 64, 8),  GMFA_TILE(8, 64, 8), GMFA_TILE(24, 64, 14),
     // general tiles, any band up to C*G
     GMFA_TILE(2, 16, -1),  GMFA_TILE(5, 16, -1), GMFA_TILE(8, 16, -1),
     GMFA_TILE(4, 32, -1),  GMFA_TILE(8, 32, -1),
@@ -1476,8 +1612,12 @@ static const TileEntry kTiles[] = {
 };
 #undef GMFA_TILE
 
+// band width a tile holds: C*G, plus the scalar cell of KS == C tiles
+static int tile_capacity(const TileEntry &t) { return t.C * t.G + (t.KS == t.C ? 1 : 0); }
+
 static bool tile_serves(const TileEntry &t, int W)
 {
+    if (t.KS == t.C) return W == t.C * t.G + 1;  // scalar-cell tile: exactly that band
     if (t.C * t.G < W) return false;
     return t.KS < 0 || (W - 1) % t.C == t.KS;
 }
@@ -1503,7 +1643,7 @@ static const TileEntry *default_tile(int W)
     double best_eff = -1.0;
     for (const auto &t : kTiles) {
         if (!tile_serves(t, W)) continue;
-        const double eff = (double)W / (double)(t.G * t.C) + (t.KS >= 0 ? 1e-3 : 0.0);
+        const double eff = (double)W / (double)tile_capacity(t) + (t.KS >= 0 ? 1e-3 : 0.0);
         if (eff > best_eff + 1e-9) { best = &t; best_eff = eff; }
     }
     return best;
@@ -1527,7 +1667,7 @@ extern "C" int gmfa_sw_resolve_tile(int32_t half_band, int32_t cells_per_lane,
                       cells_per_lane, lanes_per_pair);
             return GMFA_E_UNSUPPORTED;
         }
-        if (t->C * t->G < W) {
+        if (tile_capacity(*t) < W) {
             set_error("tile %dx%d cannot hold a band of width %d", t->C, t->G, W);
             return GMFA_E_UNSUPPORTED;
         }

@@ -726,15 +726,16 @@ def cigar_bam(cigar: torch.Tensor, cigar_off: torch.Tensor, n_cigar: torch.Tenso
 def autotune_tile(inp: SwInputs, half_band: int, scoring: tuple, reps: int = 2,
                   order: Optional[torch.Tensor] = None):
     """Time every compiled tile that holds the band (the exact/specialised
-    variant where one exists) on ``inp`` with the fused walk; returns the
+    variant where one exists, e.g. 8x64 plus the scalar cell for W = 513) on ``inp`` with the fused walk; returns the
     fastest tile and {(cells_per_lane, lanes_per_pair): ms}."""
     W = 2 * int(half_band) + 1
     timings = {}
     best, best_ms = None, None
     for C, G in list_tiles():
-        if C * G < W:
+        try:  # the specialised variant for this band, or the general tile if it holds it
+            tile = resolve_tile(half_band, C, G)
+        except (RuntimeError, ValueError):
             continue
-        tile = resolve_tile(half_band, C, G)
         ws = make_workspace(inp, tile, cigar=True)
         sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
                    workspace=ws, cigar=True)

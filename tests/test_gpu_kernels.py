@@ -140,6 +140,37 @@ def test_sw_specialised_band_tiles(hb, fused):
     _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"spec{hb}")
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_sw_scalar_cell_tile_matches_oracle_and_9x64(fused):
+    """W = 513 as 8 x 64 lanes plus the scalar cell (diagonal 512 on the scalar
+    unit, plane lane 64): every output equals the oracle's and the 9 x 64
+    tile's, row keys included; pairs shorter than the band, N codes, band
+    offsets that push diagonal 512 in and out of the matrix."""
+    rng = np.random.default_rng(513)
+    k = K()
+    qs, ts, offs = [], [], []
+    for p in range(24):
+        n = int(rng.integers(1, 1400)) if p % 3 else int(rng.integers(1, 40))
+        q, t = _random_pair(rng, n, with_n=(p % 4 == 2))
+        qs.append(q)
+        ts.append(t)
+        offs.append(int(rng.integers(-300, 300)) if p % 2 else int(rng.integers(-5, 25)))
+    _say("case: scalar-cell tile 8x64+1, fused", fused)
+    t8 = k.resolve_tile(256, 8, 64)
+    assert (t8.cells_per_lane, t8.lanes_stored) == (8, 65)
+    got = _gpu_align(qs, ts, offs, 256, (2, 4, 4, 2), tile=(8, 64), fused=fused)
+    _check_against_oracle(qs, ts, offs, 256, (2, 4, 4, 2), got, "xs513")
+    inp = k.upload_pairs(qs, ts, offs, DEV, alphabet="ref")
+    keys = []
+    for C, G in ((8, 64), (9, 64)):
+        f = k.sw_forward(inp, 2, 4, 4, 2, 256, traceback=True, rowkeys=True,
+                         tile=k.resolve_tile(256, C, G))
+        keys.append((f.score.cpu().numpy(), f.rowkey.cpu().numpy(), f.rowkey_off_host))
+    assert np.array_equal(keys[0][0], keys[1][0])
+    assert np.array_equal(keys[0][2], keys[1][2])
+    assert np.array_equal(keys[0][1], keys[1][1])
+
+
 def test_sw_short_config_full():
     """1,024 x 150 bp pairs, hb 32 at offset 32, scoring 2/-4/-4/-2 (whole config)."""
     from graphmambaformer_amd import workloads

@@ -34,8 +34,12 @@ def test_library_exports_every_declared_symbol():
 def test_tile_resolution_and_tb_bytes():
     from graphmambaformer_amd import kernels as K
 
-    t = K.resolve_tile(256)
+    t = K.resolve_tile(256)  # W = 513 = 8 x 64 + the scalar cell (plane lane 64)
+    assert (t.cells_per_lane, t.lanes_per_pair, t.lanes_stored) == (8, 64, 65)
+    t = K.resolve_tile(256, 9, 64)
     assert (t.cells_per_lane, t.lanes_per_pair, t.lanes_stored) == (9, 64, 57)
+    t = K.resolve_tile(255, 8, 64)  # W = 511: the general 8x64 tile
+    assert (t.cells_per_lane, t.lanes_per_pair, t.lanes_stored) == (8, 64, 64)
     t = K.resolve_tile(32)
     assert t.cells_per_lane * t.lanes_per_pair >= 65
     for hb in (0, 1, 5, 64, 200, 511, 751, 1023):
