@@ -42,10 +42,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(cells: int, fused: bool):
+def measured_traffic(cells: int, fused: bool, tile=None):
     """HBM bytes per forward launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE runs (calibrated against a plain 1 GiB copy), if they were
-    taken on this exact batch and walk mode; else None."""
+    taken on this exact batch, tile and walk mode; else None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "sw_forward_sw_long_traffic.json")
     try:
@@ -54,6 +54,8 @@ def measured_traffic(cells: int, fused: bool):
     except (OSError, ValueError):
         return None
     if rec.get("cells") != cells or not fused:
+        return None
+    if tile is not None and rec.get("tile") != [tile.cells_per_lane, tile.lanes_per_pair]:
         return None
     return rec.get("traffic_bytes_per_launch")
 
@@ -531,8 +533,9 @@ def main():
     log(f"[bench] sw_long generated: {w.n_pairs} pairs, {w.cells().sum():.3e} cells, "
         f"{time.time() - t0:.1f}s")
     fused = args.walk == "fused"
-    # tile autotuning on every 16th pair (by cell count) of the batch
-    sub_t = WL.subsample(w, np.argsort(w.cells(), kind="stable")[::16])
+    # tile autotuning on every 4th pair (by cell count) of the batch: 4,096
+    # pairs keep about 4 wavefronts per SIMD resident, as the batch does
+    sub_t = WL.subsample(w, np.argsort(w.cells(), kind="stable")[::4])
     inp_t = K.upload_pairs(sub_t.queries, sub_t.targets, sub_t.band_offset, dev, alphabet="2bit")
     tuned, tune_ms = K.autotune_tile(inp_t, w.half_band, scoring(w),
                                      order=torch.from_numpy(lpt_order(sub_t.cells())).to(dev))
@@ -618,7 +621,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(hbm_gbs / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(total_cells, fused),
+            "traffic": measured_traffic(total_cells, fused, tile),
             "kernel": "sw_forward_kernel",
             "note": "the kernel is int32-VALU bound; see roofline_valu",
         },

@@ -743,22 +743,63 @@ __global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
     a.o.counts[4 * p + 3] = cnt[3];
 }
 
-// One row of the scalar cell of KS == C tiles, on the scalar unit (the
-// vector unit bounds the kernel).  Inputs and state are wave-uniform:
+// Scalar-unit operations for the scalar cell of KS == C tiles.  Compiled
+// from C, its compares and max chain went to the vector unit; these small
+// asm statements keep them on the scalar unit while the scheduler can still
+// interleave them with the rows' vector instructions.
+__device__ __forceinline__ int s_max(int a, int b)
+{
+    int r;
+    asm("s_max_i32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b) : "scc");
+    return r;
+}
+// c ? a : b for a nonzero test of c
+__device__ __forceinline__ int s_sel_nz(int c, int a, int b)
+{
+    int r;
+    asm("s_cmp_lg_u32 %1, 0\n\ts_cselect_b32 %0, %2, %3" : "=s"(r) : "s"(c), "s"(a), "s"(b) : "scc");
+    return r;
+}
+// x < 0 ? a : b
+__device__ __forceinline__ int s_sel_neg(int x, int a, int b)
+{
+    int r;
+    asm("s_cmp_lt_i32 %1, 0\n\ts_cselect_b32 %0, %2, %3" : "=s"(r) : "s"(x), "s"(a), "s"(b) : "scc");
+    return r;
+}
+// a == b ? x : 0
+__device__ __forceinline__ int s_sel_eq(int a, int b, int x)
+{
+    int r;
+    asm("s_cmp_eq_u32 %1, %2\n\ts_cselect_b32 %0, %3, 0" : "=s"(r) : "s"(a), "s"(b), "s"(x) : "scc");
+    return r;
+}
+// (t > (bK | 31)) ? (bK, bi) = (t, i)
+__device__ __forceinline__ void s_best(int t, int i, int &bK, int &bi)
+{
+    int u;
+    asm("s_or_b32 %[u], %[bK], 31\n\t"
+        "s_cmp_gt_i32 %[t], %[u]\n\t"
+        "s_cselect_b32 %[bK], %[t], %[bK]\n\t"
+        "s_cselect_b32 %[bi], %[i], %[bi]"
+        : [bK] "+s"(bK), [bi] "+s"(bi), [u] "=&s"(u)
+        : [t] "s"(t), [i] "s"(i)
+        : "scc");
+}
+
+// One row of the scalar cell (wave-uniform state and inputs):
 //   dg = H(i-1, d) + (match ? ma + mi : 0)      (row-shifted domain)
 //   f  = max(zprev - cH, kNeg - cF)               (no band cell at d+1)
 //   e  = g63 + max(d63, 0) - (go + (C-1)*ge)      (lane 63's prefix, one on)
 //   h  = max(dg, f, zrow, e); out of the matrix (xl < 0): h = zrow, f = kNeg
 // then the plane bits (f_cont 0: F(i-1, d+1) is out of band; not_diag,
 // not_f, e_cont = sign of lane 63's A - G) and the first strict maximum.
-// Written as one asm block: compiled from C, the compares and the max chain
-// went to the vector unit.
+template <bool SLOW>
 __device__ __forceinline__ void scalar_cell(int &Hs, int &Fs, uint32_t &acc, int &bK, int &bi,
                                             int tf, int qc, int mf, int mms, int zprev, int zrow,
                                             int cH, int bv, int d63, int g63, int k1, int imi5,
                                             int i, int xl, int neg)
 {
-    int t0, t1, dg, fs, hs;
     // the asm takes SGPRs; values the compiler placed in VGPRs (the carried
     // state also feeds the DPP "old" operand) move over here, the others
     // cost nothing
@@ -769,48 +810,21 @@ __device__ __forceinline__ void scalar_cell(int &Hs, int &Fs, uint32_t &acc, int
     GMFA_UNI(cH); GMFA_UNI(bv); GMFA_UNI(d63); GMFA_UNI(g63); GMFA_UNI(k1); GMFA_UNI(imi5);
     GMFA_UNI(i); GMFA_UNI(xl); GMFA_UNI(neg);
 #undef GMFA_UNI
-    asm("s_cmp_eq_u32 %[tf], %[qc]\n\t"
-        "s_cselect_b32 %[t0], %[mms], 0\n\t"
-        "s_cmp_eq_u32 %[mf], 0\n\t"
-        "s_cselect_b32 %[t0], %[t0], 0\n\t"
-        "s_add_i32 %[dg], %[Hs], %[t0]\n\t"
-        "s_sub_i32 %[fs], %[zprev], %[cH]\n\t"
-        "s_max_i32 %[fs], %[fs], %[bv]\n\t"
-        "s_max_i32 %[t0], %[d63], 0\n\t"
-        "s_add_i32 %[t0], %[t0], %[g63]\n\t"
-        "s_sub_i32 %[t0], %[t0], %[k1]\n\t"
-        "s_max_i32 %[hs], %[dg], %[fs]\n\t"
-        "s_max_i32 %[hs], %[hs], %[zrow]\n\t"
-        "s_max_i32 %[hs], %[hs], %[t0]\n\t"
-        "s_cmp_lt_i32 %[xl], 0\n\t"
-        "s_cselect_b32 %[hs], %[zrow], %[hs]\n\t"
-        "s_cselect_b32 %[fs], %[neg], %[fs]\n\t"
-        "s_sub_i32 %[t0], %[dg], %[hs]\n\t"
-        "s_lshr_b32 %[t0], %[t0], 31\n\t"
-        "s_lshl_b32 %[t0], %[t0], 2\n\t"
-        "s_sub_i32 %[t1], %[fs], %[hs]\n\t"
-        "s_lshr_b32 %[t1], %[t1], 31\n\t"
-        "s_lshl_b32 %[t1], %[t1], 1\n\t"
-        "s_or_b32 %[t0], %[t0], %[t1]\n\t"
-        "s_lshr_b32 %[t1], %[d63], 31\n\t"
-        "s_or_b32 %[t0], %[t0], %[t1]\n\t"
-        "s_lshl_b32 %[acc], %[acc], 4\n\t"
-        "s_or_b32 %[acc], %[acc], %[t0]\n\t"
-        "s_lshl_b32 %[t0], %[hs], 5\n\t"
-        "s_or_b32 %[t0], %[t0], 31\n\t"
-        "s_sub_i32 %[t0], %[t0], %[imi5]\n\t"
-        "s_or_b32 %[t1], %[bK], 31\n\t"
-        "s_cmp_gt_i32 %[t0], %[t1]\n\t"
-        "s_cselect_b32 %[bK], %[t0], %[bK]\n\t"
-        "s_cselect_b32 %[bi], %[i], %[bi]\n\t"
-        "s_mov_b32 %[Hs], %[hs]\n\t"
-        "s_mov_b32 %[Fs], %[fs]"
-        : [Hs] "+s"(Hs), [Fs] "+s"(Fs), [acc] "+s"(acc), [bK] "+s"(bK), [bi] "+s"(bi),
-          [t0] "=&s"(t0), [t1] "=&s"(t1), [dg] "=&s"(dg), [fs] "=&s"(fs), [hs] "=&s"(hs)
-        : [tf] "s"(tf), [qc] "s"(qc), [mf] "s"(mf), [mms] "s"(mms), [zprev] "s"(zprev),
-          [zrow] "s"(zrow), [cH] "s"(cH), [bv] "s"(bv), [d63] "s"(d63), [g63] "s"(g63),
-          [k1] "s"(k1), [imi5] "s"(imi5), [i] "s"(i), [xl] "s"(xl), [neg] "s"(neg)
-        : "scc");
+    const int sub = s_sel_nz(mf, 0, s_sel_eq(tf, qc, mms));
+    const int dg = (int)((uint32_t)Hs + (uint32_t)sub);
+    int fs = s_max((int)((uint32_t)zprev - (uint32_t)cH), bv);
+    const int es = (int)((uint32_t)s_max(d63, 0) + (uint32_t)g63 - (uint32_t)k1);
+    int hs = s_max(s_max(s_max(dg, fs), zrow), es);
+    if constexpr (SLOW) {
+        hs = s_sel_neg(xl, zrow, hs);
+        fs = s_sel_neg(xl, neg, fs);
+    }
+    const uint32_t bits = ((((uint32_t)dg - (uint32_t)hs) >> 31) << 2) |
+                          ((((uint32_t)fs - (uint32_t)hs) >> 31) << 1) | ((uint32_t)d63 >> 31);
+    acc = (acc << 4) | bits;
+    s_best((int)((((uint32_t)hs << 5) | 31u) - (uint32_t)imi5), i, bK, bi);
+    Hs = hs;
+    Fs = fs;
 }
 
 // SV: serving variant (persistent queue fed by the host; staged requests of
@@ -1146,7 +1160,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                                                 (uint32_t)(tlen + hb - off - i - (W - 1)) |
                                                 (uint32_t)((W - 1) - (hb + 1 - off - i)))
                                         : 0;
-                    scalar_cell(Hs, Fs, accs, bestKs, bestis, tf, (int)qc, mf, (int)mms, zprev,
+                    scalar_cell<SLOW>(Hs, Fs, accs, bestKs, bestis, tf, (int)qc, mf, (int)mms, zprev,
                                 zrow, cH, kNeg - cF, d63, g63, go + (C - 1) * ge,
                                 (int)((uint32_t)(i * mi) << 5), i, xl, kNeg);
                     if constexpr (RK)
