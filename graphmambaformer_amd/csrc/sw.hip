@@ -1596,7 +1596,7 @@ SwKernel pick_kernel(uint32_t flags)
 template <int C, int G, int KS>
 SwKernel pick_serve_kernel(uint32_t flags)
 {
-    if constexpr (G == 64 && KS != C) {
+    if constexpr (G == 64) {
         if (flags & GMFA_SW_NMASK) return sw_forward_kernel<C, G, KS, true, true, false, true>;
         return sw_forward_kernel<C, G, KS, true, false, false, true>;
     }

@@ -31,8 +31,12 @@ from ._lib import ServeClass, ServeReport, SwTile, check, lib, ptr
 
 def serve_tile(half_band: int) -> SwTile:
     """The compiled tile with one pair per wavefront (64 lanes) and the least
-    padding for this band."""
+    padding for this band (for W = 64*C + 1, C cells per lane plus the scalar
+    cell)."""
     W = 2 * int(half_band) + 1
+    t = kernels.resolve_tile(half_band)
+    if t.lanes_per_pair == 64:
+        return t
     fits = [C for C, G in kernels.list_tiles() if G == 64 and C * 64 >= W]
     if not fits:
         raise RuntimeError(f"no 64-lane tile holds a band of width {W}")
