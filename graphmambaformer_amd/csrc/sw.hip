@@ -989,7 +989,8 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         uint32_t acc[C];
 #pragma unroll
         for (int k = 0; k < C; ++k) { Hp[k] = 0; Fp[k] = kNeg; acc[k] = 0u; }
-        int bestK = 31, besti = 0;
+        // lane's first maximum: key H << 8 | (7 - r) << 5 | (31 - k), row besti
+        int bestK = 255, besti = 0;
         // XS: the scalar cell's H and F of the previous row, plane bits and
         // first maximum (wave-uniform)
         int Hs = 0, Fs = kNeg, bestKs = 31, bestis = 0;
@@ -1034,6 +1035,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         for (int b = 0; b < nblk; ++b) {
             const int i0 = 8 * b + 1;
             int rkeys[8];
+            int bk = INT_MIN;  // the block's best key (true H)
             (void)rkeys;
             // align this block's windows, then prefetch the next block
             uint32_t ta[NA], na[NA];
@@ -1193,20 +1195,23 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                         acc[k] = insert_sign(acc[k], Fk - H);     // not_f
                         acc[k] = insert_sign(acc[k], k ? (Av[k ? k - 1 : 0] - Gkm1) : dprev);  // e_cont
                     }
-                    rk = max(rk, (int)(((uint32_t)H << 5) | (uint32_t)(31 - k)));
+                    rk = max(rk, (int)(((uint32_t)H << 8) | (uint32_t)(31 - k)));
                     Hp[k] = H;  // Hp[k] is not read again in this row
                     Fv[k] = Fk;
                     Gkm1 = Gk;
                 }
 #pragma unroll
                 for (int k = 0; k < C; ++k) Fp[k] = Fv[k];
-                const int tk = rk - (int)((uint32_t)(i * mi) << 5);
-                if (tk > (bestK | 31)) { bestK = tk; besti = i; }
+                // the row's key in true H with the row in bits 5-7 (one scalar:
+                // per-cell constants above 64 would each take a register), then
+                // the block's running maximum (keys order by H, then by smaller
+                // row, then by smaller d)
+                bk = max(bk, rk - (int)(((uint32_t)(i * mi) << 8) - ((uint32_t)(7 - r) << 5)));
                 if constexpr (RK) {
                     // the row's (max H, first d) as one key (keys order by H, then
                     // by smaller d): lane-local key, then a max over the group;
                     // the group's last lane ends up holding it
-                    const int htrue = (rk >> 5) - i * mi;
+                    const int htrue = (rk >> 8) - i * mi;
                     const int d = gl * C + 31 - (rk & 31);
                     // live keys are >= 1 (H >= 0, 2047 - d >= 1), so a dead
                     // lane's key 0 never wins; an AND, not a select: the select
@@ -1225,6 +1230,11 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             } else {
 #pragma unroll
                 for (int r = 0; r < 8; ++r) row(std::true_type{}, r);
+            }
+            // strict improvement in H over the lane's best so far
+            if (bk > (bestK | 255)) {
+                bestK = bk;
+                besti = i0 + 7 - ((bk >> 5) & 7);
             }
             if constexpr (RK) {
                 // group max of each row's lane key, after the block: eight
@@ -1257,7 +1267,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         }
 
         // group reduction of the lane-first maxima: (max H, min i, min d)
-        int bH = bestK >> 5;
+        int bH = bestK >> 8;
         const int bd = gl * C + (31 - (bestK & 31));
         if (!lane_live || besti == 0) bH = 0;
         uint64_t key = ((uint64_t)(uint32_t)bH << 40) |
