@@ -70,6 +70,11 @@ __device__ __forceinline__ int seg_scan_max(int x)
 
 // value of lane-1 (kNeg into lane 0 of the wave)
 __device__ __forceinline__ int from_prev_lane(int x) { return dpp<kWaveShr1>(kNeg, x); }
+// value of lane-1, 0 into lane 0 (DPP bound_ctrl: no "old" operand to set up)
+__device__ __forceinline__ int from_prev_lane_or0(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, kWaveShr1, 0xf, 0xf, true);
+}
 // value of lane+1 (old into lane 63)
 __device__ __forceinline__ int from_next_lane(int old, int x) { return dpp<kWaveShl1>(old, x); }
 

@@ -1138,7 +1138,10 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 int T = Av[0];
 #pragma unroll
                 for (int k = 1; k < C; k += 2) T = (k + 1 < C) ? max(max(T, Av[k]), Av[(k + 1) % C]) : max(T, Av[k]);
-                int X = from_prev_lane(seg_scan_max<G>(T + lCe));
+                // lane 0's prefix is 0, not -inf: every live A = M + k*e >= zrow
+                // >= 0 (row-shifted domain), so a zero prefix never wins E (E
+                // <= 0 <= zrow <= M), and a dead cell's A stays below it
+                int X = from_prev_lane_or0(seg_scan_max<G>(T + lCe));
                 if constexpr (G < 64) X = (gl == 0) ? kNeg : X;
                 const int Xl = X - lCe;
                 // exclusive prefix inside the lane: G_0 = X, G_{k+1} = max(G_k, A_k)
@@ -1147,7 +1150,9 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 for (int k = 1; k < C; ++k) Gv[k] = max(Gv[k - 1], Av[k - 1]);
                 const int Glast = Gv[C - 1];
                 const int dlast = Av[C - 1] - Glast;
-                const int dprev = from_prev_lane(dlast);
+                // lane 0's e_cont bit at d = 0 is never read (no band cell
+                // left of it)
+                const int dprev = from_prev_lane_or0(dlast);
                 if constexpr (XS) {
                     // the scalar cell d = W-1 as register C of lane 63: E from
                     // that lane's prefix, no band cell at d+1 (H 0 and F -inf
