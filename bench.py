@@ -490,6 +490,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--tile", default=None,
+                    help="C,G: run the long-read batch on this tile instead of the autotuner's "
+                         "pick (profiling runs pin the tile they measure)")
     ap.add_argument("--walk", choices=["separate", "fused"], default="fused",
                     help="CIGAR walk inside the forward launch (fused) or as its own launch "
                          "(one lane per pair)")
@@ -539,6 +542,9 @@ def main():
     inp_t = K.upload_pairs(sub_t.queries, sub_t.targets, sub_t.band_offset, dev, alphabet="2bit")
     tuned, tune_ms = K.autotune_tile(inp_t, w.half_band, scoring(w),
                                      order=torch.from_numpy(lpt_order(sub_t.cells())).to(dev))
+    if args.tile:
+        c_, g_ = (int(x) for x in args.tile.split(","))
+        tuned = K.resolve_tile(w.half_band, c_, g_)
     del inp_t
     torch.cuda.empty_cache()
     log(f"[bench] autotune (ms per sample launch): {tune_ms} -> "
