@@ -1471,6 +1471,78 @@ __global__ __launch_bounds__(256) void sw_xdrop_wide_kernel(
     }
 }
 
+// The same x-drop for small groups (at most kXdropRowsPairs pairs, e.g. one
+// read's chains): the 64 lanes take 64 consecutive rows, the pairs are a
+// loop.  Per pair and tile: one coalesced load of 64 row keys, a wave prefix
+// max for each row's best-so-far (carried across tiles), and the rows where
+// that minus the row's best exceeds x_drop; the stop row is the first row
+// where it holds for every pair (the first set bit of a ballot).  Each
+// pair's first strict maximum over the rows up to the stop is a wave max of
+// (h << 6 | 63 - lane) keys against the carry.  Per-pair state sits in lane
+// k of a register.  The lane-per-pair kernel below walks the rows one at a
+// time with a pair per lane, which leaves most lanes idle for such groups.
+constexpr int kXdropRowsPairs = 16;
+
+__global__ __launch_bounds__(64) void sw_xdrop_rows_kernel(
+    const int32_t *rowkey, const int64_t *rk_off, const int32_t *qlen, const int32_t *boff,
+    const int32_t *group_off, int32_t thr, int32_t hb, int32_t *score, int32_t *qend,
+    int32_t *tend)
+{
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int p0 = group_off[g];
+    const int n = min(group_off[g + 1] - p0, kXdropRowsPairs);
+    const int ql_l = lane < n ? qlen[p0 + lane] : 0;
+    const int64_t base_l = lane < n ? rk_off[p0 + lane] : 0;
+    const int maxq = wave_max(ql_l);
+    int seen_l = 0, bh_l = 0, bi_l = 0, bd_l = 0;  // lane k: pair k's state
+    for (int t0 = 1; t0 <= maxq; t0 += 64) {
+        const int i = t0 + lane;  // this lane's row
+        int kk[kXdropRowsPairs];
+        bool all = thr >= 0 && i <= maxq;
+#pragma unroll
+        for (int k = 0; k < kXdropRowsPairs; ++k) {
+            kk[k] = 0;
+            if (k < n) {
+                const int ql = __builtin_amdgcn_readlane(ql_l, k);
+                const int64_t base = readlane64(base_l, k);
+                const int key = i <= ql ? rowkey[base + i - 1] : 0;  // rows past qlen: H = 0
+                kk[k] = key;
+                const int h = key >> 11;
+                const int pm = max(__builtin_amdgcn_readlane(seen_l, k), seg_scan_max<64>(h));
+                all = all && pm - h > thr;
+                const int carry = __builtin_amdgcn_readlane(pm, 63);
+                seen_l = lane == k ? carry : seen_l;
+            }
+        }
+        const uint64_t stop = __builtin_amdgcn_ballot_w64(all);
+        const int last = stop ? (int)__builtin_ctzll(stop) : 63;  // last row lane counted
+#pragma unroll
+        for (int k = 0; k < kXdropRowsPairs; ++k) {
+            if (k < n) {
+                const int v = lane <= last
+                                  ? (int)(((uint32_t)(kk[k] >> 11) << 6) | (uint32_t)(63 - lane))
+                                  : -1;
+                const int best = __builtin_amdgcn_readlane(seg_scan_max<64>(v), 63);
+                const int h = best >> 6, at = 63 - (best & 63);
+                if (h > __builtin_amdgcn_readlane(bh_l, k)) {
+                    const int key = __builtin_amdgcn_readlane(kk[k], at);
+                    bh_l = lane == k ? h : bh_l;
+                    bi_l = lane == k ? t0 + at : bi_l;
+                    bd_l = lane == k ? 2047 - (key & 2047) : bd_l;
+                }
+            }
+        }
+        if (stop) break;
+    }
+    if (lane < n) {
+        const int p = p0 + lane;
+        score[p] = bh_l;
+        qend[p] = bh_l > 0 ? bi_l : 0;
+        tend[p] = bh_l > 0 ? bi_l + boff[p] - hb + bd_l : 0;
+    }
+}
+
 // Batch-wide x-drop (extension.py:186-193) per group of pairs: the reference
 // stops a batch after the first row where every pair's best-so-far minus the
 // row's best exceeds x_drop; the result is the first strict maximum over the
@@ -1990,7 +2062,11 @@ extern "C" int gmfa_sw_xdrop(const int32_t *rowkey, const int64_t *rowkey_off,
                      target_end,
                  "NULL pointer");
     const int thr = x_drop < 0 ? -1 : x_drop;
-    if (max_group <= 64) {
+    if (max_group <= kXdropRowsPairs) {
+        hipLaunchKernelGGL(sw_xdrop_rows_kernel, dim3(n_groups), dim3(64), 0,
+                           (hipStream_t)stream, rowkey, rowkey_off, qlen, band_offset, group_off,
+                           thr, half_band, score, query_end, target_end);
+    } else if (max_group <= 64) {
         hipLaunchKernelGGL(sw_xdrop_kernel, dim3(n_groups), dim3(64), 0, (hipStream_t)stream,
                            rowkey, rowkey_off, qlen, band_offset, group_off, thr, half_band, score,
                            query_end, target_end);

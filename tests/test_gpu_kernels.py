@@ -298,6 +298,56 @@ def test_sw_edge_cases_vs_oracle():
             _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"edge{hb}")
 
 
+def test_xdrop_small_groups_vs_host_restatement():
+    """Groups of 1-16 pairs (the row-parallel x-drop kernel: 64 rows per tile,
+    several tiles per pair, stops inside and at the edges of tiles) against
+    the host restatement of the x-drop over the same row keys, and against
+    the oracle's x-drop for a few groups."""
+    k = K()
+    rng = np.random.default_rng(43)
+    qs, ts, offs = [], [], []
+    for p in range(160):
+        n = int(rng.integers(1, 420))
+        q, t = _random_pair(rng, n)
+        if p % 3 == 0:  # a diverging tail makes the x-drop fire mid-pair
+            t = np.concatenate([t[: n // 2], rng.integers(1, 5, size=n).astype(np.int16)])
+        qs.append(q)
+        ts.append(t)
+        offs.append(int(rng.integers(0, 20)))
+    inp = k.upload_pairs(qs, ts, offs, DEV, alphabet="ref")
+    hb = 24
+    fwd = k.sw_forward(inp, 2, 4, 4, 2, hb, traceback=True, rowkeys=True)
+    keys = fwd.rowkey.cpu().numpy()
+    sizes = []
+    while sum(sizes) < len(qs):
+        sizes.append(min(int(rng.integers(1, 17)), len(qs) - sum(sizes)))
+    go = np.concatenate([[0], np.cumsum(sizes)])
+    assert go.max() == len(qs) and max(sizes) <= 16
+    for xd in (0.0, 3.0, 10.0, 40.0, 1e9):
+        _say("case: x-drop small groups", len(sizes), xd)
+        sc, qe, te = k.sw_xdrop(fwd, inp, go, xd, hb)
+        esc, eqe, ete = O.xdrop_from_rowkeys(keys, fwd.rowkey_off_host, inp.qlen_host, xd, hb,
+                                             inp.boff_host, go)
+        assert np.array_equal(sc.cpu().numpy(), esc), xd
+        assert np.array_equal(qe.cpu().numpy(), eqe), xd
+        assert np.array_equal(te.cpu().numpy(), ete), xd
+    for a, b in list(zip(go[:-1], go[1:]))[:6]:
+        for xd in (3.0, 10.0):
+            M = max(len(x) for x in qs[a:b])
+            N = max(len(x) for x in ts[a:b])
+            Q = np.full((b - a, M), -1, np.int16)
+            T = np.full((b - a, N), -1, np.int16)
+            for x in range(a, b):
+                Q[x - a, :len(qs[x])] = qs[x]
+                T[x - a, :len(ts[x])] = ts[x]
+            res = O.sw_banded(Q, T, [len(x) for x in qs[a:b]], [len(x) for x in ts[a:b]],
+                              offs[a:b], hb, 2, 4, 4, 2, x_drop=xd)
+            sc, qe, te = k.sw_xdrop(fwd, inp, go, xd, hb)
+            assert np.array_equal(sc.cpu().numpy()[a:b], res["score"]), (a, b, xd)
+            assert np.array_equal(qe.cpu().numpy()[a:b], res["query_end"]), (a, b, xd)
+            assert np.array_equal(te.cpu().numpy()[a:b], res["target_end"]), (a, b, xd)
+
+
 def test_xdrop_wide_groups_vs_oracle():
     """Groups above 64 pairs (the 256-thread x-drop kernel) and mixed group
     sizes in one call, against the oracle's x-drop on the same pairs."""
