@@ -706,18 +706,19 @@ struct TbKArgs {
     const int64_t *q_off, *t_off;
     const int32_t *boff;
     int32_t n_pairs;
+    int32_t per_wave;  // pairs per wavefront (lanes 0 .. per_wave-1 walk)
     const int32_t *score, *qend, *tend;
     const uint32_t *tb;
     const int64_t *tb_off;
 };
 
-// Standalone walk: one lane per pair (64 pairs per wavefront), the current
-// plane dword and 16-base query / target words cached in registers.
+// Standalone walk: one lane per pair (up to 64 pairs per wavefront), the
+// current plane dword and 16-base query / target words cached in registers.
 template <int C>
 __global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
 {
-    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p >= a.n_pairs) return;
+    const int p = (int)(blockIdx.x * a.per_wave + threadIdx.x);
+    if ((int)threadIdx.x >= a.per_wave || p >= a.n_pairs) return;
     const WalkCtx &c = a.c;
     int i = a.qend[p], j = a.tend[p];
     const int h = a.score[p], off = a.boff[p];
@@ -2108,7 +2109,18 @@ extern "C" int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_param
     a.score = score; a.qend = query_end; a.tend = target_end;
     a.tb = tb; a.tb_off = tb_off;
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, a.c.W);
-    hipLaunchKernelGGL(t->walk, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, a);
+    // 64 pairs per wavefront only while that still gives every SIMD 4
+    // wavefronts: the walks are serial chains, so fewer pairs per wavefront
+    // spread them over idle SIMDs (16,384 long pairs: 256 wavefronts of 64,
+    // or 4,096 of 4)
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    const int64_t slots = (int64_t)4 * 4 * cus;  // 4 wavefronts on each SIMD
+    a.per_wave = (int)std::max<int64_t>(1, std::min<int64_t>(64, n / slots));
+    hipLaunchKernelGGL(t->walk, dim3((n + a.per_wave - 1) / a.per_wave), dim3(64), 0,
+                       (hipStream_t)stream, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }
