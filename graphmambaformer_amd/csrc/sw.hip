@@ -374,14 +374,58 @@ __device__ __forceinline__ uint32_t seq_bits(const uint32_t *src, int64_t words,
     return __builtin_amdgcn_alignbit(c.hi, c.lo, (uint32_t)(e & 15) * 2u);
 }
 
+// Windows of a walk that a whole wavefront runs with wave-uniform state: the
+// lanes load many dwords at once and each step picks its dword with a
+// readlane, so a dependent load is paid once per window instead of once per
+// plane block or 16-base word.  Plane window: 4 blocks x 16 diagonals (lane
+// l holds block top - (l >> 4), diagonal d0 + (l & 15)), plus the 4 blocks
+// above it loaded one window ahead.  Sequence window: 64 words (lane l holds
+// word top - l); the walk only moves towards lower words.
+struct PlaneWin {
+    int top = INT_MIN / 2, d0 = 0;
+    uint32_t v = 0u, pv = 0u;  // this window, the 4 blocks above it
+};
+struct SeqWin {
+    int64_t top = -1;  // -1: nothing loaded
+    uint32_t v = 0u;
+};
+
+__device__ __forceinline__ uint32_t plane_win_load(const uint32_t *tb, int top, int d0,
+                                                   int blkstride, int W)
+{
+    const int l = (int)(threadIdx.x & 63);
+    const int b = top - (l >> 4), d = d0 + (l & 15);
+    // blocks 0 .. top were written (the walk starts at or below top), and
+    // diagonals 0 .. W-1 lie inside every block's blkstride >= W dwords
+    return (b >= 0 && d >= 0 && d < W) ? tb[(int64_t)b * blkstride + d] : 0u;
+}
+
+__device__ __forceinline__ uint32_t seq_bits_win(const uint32_t *src, int64_t words, int64_t e,
+                                                 SeqWin &s)
+{
+    const int64_t w = e >> 4;
+    if (w + 1 > s.top || s.top - w > 63) {
+        s.top = w + 1;
+        const int64_t x = s.top - (int64_t)(threadIdx.x & 63);
+        s.v = (x >= 0 && x < words) ? src[x] : 0u;
+    }
+    const int k = (int)(s.top - w);  // 1 .. 63: word w in lane k, word w+1 in lane k-1
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)s.v, k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)s.v, k - 1);
+    return __builtin_amdgcn_alignbit(hi, lo, (uint32_t)(e & 15) * 2u);
+}
+
 // The walk (extension.py:243-285) shared by both walkers.  `ld(ptr, idx)`
 // loads a dword (plane or sequence); `put(x, run)` stores CIGAR run x.
 // Diagonal moves are taken in bulk: the not_diag bits of the rows below the
 // current one in the same plane dword give the run length (up to 8 rows),
 // one XOR of 2-bit base windows gives its matches and mismatches, and the
 // tracked H only has to stay positive, which h > (m-1)*match guarantees.
-// The plane dword of the same diagonal one block up is prefetched.
-template <int C, typename Ld, typename Put>
+// WIN = false (a lane per pair): the plane dword of the same diagonal one
+// block up is prefetched.  WIN = true (a wavefront per pair, every lane
+// active, state wave-uniform): planes and sequences come from the windows
+// above and `ld` is not called.
+template <int C, bool WIN = false, typename Ld, typename Put>
 __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int h, int off,
                                            int64_t qo, int64_t to, const uint32_t *tb, Ld ld,
                                            Put put, int &nr, int (&cnt)[4])
@@ -393,6 +437,12 @@ __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int
     int64_t cur = -1, pf_addr = -1;
     uint32_t v = 0u, pf = 0u;
     SeqWindow qwin, twin, qmwin, tmwin;
+    PlaneWin pw;
+    SeqWin qsw, tsw, qmsw, tmsw;
+    auto seq = [&](const uint32_t *src, int64_t words, int64_t e, SeqWindow &cw, SeqWin &sw) {
+        if constexpr (WIN) return seq_bits_win(src, words, e, sw);
+        else return seq_bits(src, words, e, cw, ld);
+    };
     auto emit = [&](int op, int n) {
         if (op == run_op) {
             run_len += n;
@@ -405,12 +455,34 @@ __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int
     while (i > 0 && j > 0) {
         const int d = j - i - off + c.hb;
         if (d < 0 || d >= c.W) break;  // unreachable for planes this library wrote
-        const int64_t addr = plane_word<C>(i, d, c.Gs);
-        if (addr != cur) {
-            v = (addr == pf_addr) ? pf : ld(tb, addr);
-            cur = addr;
-            pf_addr = (i > 8) ? addr - blkstride : -1;
-            if (pf_addr >= 0) pf = ld(tb, pf_addr);
+        if constexpr (WIN) {
+            // the walk moves up at most one block per step, so it leaves a
+            // window through its top-4 edge (into the window loaded ahead)
+            // or sideways
+            const int blk = (i - 1) >> 3;
+            int rb = pw.top - blk, rd = d - pw.d0;
+            if ((uint32_t)rb > 3u || (uint32_t)rd > 15u) {
+                if ((uint32_t)(rb - 4) <= 3u && (uint32_t)rd <= 15u) {
+                    pw.v = pw.pv;
+                    pw.top -= 4;
+                } else {
+                    pw.top = blk;
+                    pw.d0 = d - 8;
+                    pw.v = plane_win_load(tb, pw.top, pw.d0, blkstride, c.W);
+                }
+                pw.pv = plane_win_load(tb, pw.top - 4, pw.d0, blkstride, c.W);
+                rb = pw.top - blk;
+                rd = d - pw.d0;
+            }
+            v = (uint32_t)__builtin_amdgcn_readlane((int)pw.v, (rb << 4) | rd);
+        } else {
+            const int64_t addr = plane_word<C>(i, d, c.Gs);
+            if (addr != cur) {
+                v = (addr == pf_addr) ? pf : ld(tb, addr);
+                cur = addr;
+                pf_addr = (i > 8) ? addr - blkstride : -1;
+                if (pf_addr >= 0) pf = ld(tb, pf_addr);
+            }
         }
         const int r0 = (i - 1) & 7;
         const uint32_t nib = (v >> (28 - 4 * r0)) & 15u;
@@ -424,12 +496,12 @@ __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int
                 if (h <= (m - 1) * c.ma) m = 1;
                 const uint32_t lenmask = (m >= 16) ? ~0u : ((1u << (2 * m)) - 1u);
                 const int64_t eq = qo + i - m, et = to + j - m;
-                uint32_t x = seq_bits(c.qseq, c.q_words, eq, qwin, ld) ^
-                             seq_bits(c.tseq, c.t_words, et, twin, ld);
+                uint32_t x = seq(c.qseq, c.q_words, eq, qwin, qsw) ^
+                             seq(c.tseq, c.t_words, et, twin, tsw);
                 x = (x | (x >> 1)) & 0x55555555u;
                 if (c.nm)
-                    x |= (seq_bits(c.qmask, c.q_words, eq, qmwin, ld) |
-                          seq_bits(c.tmask, c.t_words, et, tmwin, ld)) & 0x55555555u;
+                    x |= (seq(c.qmask, c.q_words, eq, qmwin, qmsw) |
+                          seq(c.tmask, c.t_words, et, tmwin, tmsw)) & 0x55555555u;
                 x &= lenmask;  // bit 2t: base i-m+t mismatches
                 const int nmis = __builtin_popcount(x);
                 h -= c.ma * (m - nmis) - c.mi * nmis;
@@ -565,7 +637,7 @@ __device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, in
     auto put = [&](int x, uint32_t vrun) {
         if (lane == 0) out[x] = vrun;
     };
-    walk_steps<C>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    walk_steps<C, true>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     for (int x = lane; x < nr / 2; x += 64) {
@@ -649,7 +721,7 @@ __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, 
         }
     };
     const int score = h, qend = i, tend = j;
-    walk_steps<C>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    walk_steps<C, true>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
     if (lane < (nr & 63)) {
         const int y = cap - 1 - ((nr & ~63) + lane);
         if (y >= 0) cig[y] = buf;
@@ -2109,16 +2181,18 @@ extern "C" int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_param
     a.score = score; a.qend = query_end; a.tend = target_end;
     a.tb = tb; a.tb_off = tb_off;
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, a.c.W);
-    // 64 pairs per wavefront only while that still gives every SIMD 4
-    // wavefronts: the walks are serial chains, so fewer pairs per wavefront
-    // spread them over idle SIMDs (16,384 long pairs: 256 wavefronts of 64,
-    // or 4,096 of 4)
+    // As few pairs per wavefront as still lets every wavefront be resident
+    // at once (8 per SIMD): each walk is a serial chain of dependent loads,
+    // so a lane's chain is shortest when its wavefront shares the SIMD's
+    // issue slots with the fewest other chains, but a second round of
+    // wavefronts waits for the first (16,384 long pairs: 8,192 wavefronts of
+    // 2; 8,192 noisy pairs: one pair per wavefront)
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
-    const int64_t slots = (int64_t)4 * 4 * cus;  // 4 wavefronts on each SIMD
-    a.per_wave = (int)std::max<int64_t>(1, std::min<int64_t>(64, n / slots));
+    const int64_t slots = (int64_t)8 * 4 * cus;  // 8 wavefronts on each SIMD
+    a.per_wave = (int)std::max<int64_t>(1, std::min<int64_t>(64, (n + slots - 1) / slots));
     hipLaunchKernelGGL(t->walk, dim3((n + a.per_wave - 1) / a.per_wave), dim3(64), 0,
                        (hipStream_t)stream, a);
     GMFA_CHECK_HIP(hipGetLastError());

@@ -136,6 +136,16 @@ def test_full_batch_properties_and_oracle_sample(config):
                  (a.walk.ref_start, b.walk.ref_start), (a.walk.counts, b.walk.counts)):
         assert torch.equal(x, y), config
     assert K.decode_cigars(a.walk) == K.decode_cigars(b.walk), config
+    del b
+    # the standalone walk over the same planes (a lane per pair, several
+    # pairs per wavefront at these batch sizes) equals the fused walk
+    _say(f"  {config}: standalone walk vs fused walk")
+    sep = K.sw_traceback(inp, a)
+    torch.cuda.synchronize()
+    for x, y in ((a.walk.n_cigar, sep.n_cigar), (a.walk.read_start, sep.read_start),
+                 (a.walk.ref_start, sep.ref_start), (a.walk.counts, sep.counts)):
+        assert torch.equal(x, y), config
+    assert K.decode_cigars(a.walk) == K.decode_cigars(sep), config
     _oracle_sample(w, a, 32, config)
 
 
