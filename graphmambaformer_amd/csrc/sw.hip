@@ -386,7 +386,7 @@ struct PlaneWin {
     uint32_t v = 0u, pv = 0u;  // this window, the 4 blocks above it
 };
 struct SeqWin {
-    int64_t top = -1;  // -1: nothing loaded
+    int top = -1;  // -1: nothing loaded
     uint32_t v = 0u;
 };
 
@@ -400,13 +400,14 @@ __device__ __forceinline__ uint32_t plane_win_load(const uint32_t *tb, int top, 
     return (b >= 0 && d >= 0 && d < W) ? tb[(int64_t)b * blkstride + d] : 0u;
 }
 
-__device__ __forceinline__ uint32_t seq_bits_win(const uint32_t *src, int64_t words, int64_t e,
-                                                 SeqWin &s)
+// `src`, `words` and `e` count from the pair's first sequence word, so the
+// positions fit 32 bits.
+__device__ __forceinline__ uint32_t seq_bits_win(const uint32_t *src, int words, int e, SeqWin &s)
 {
-    const int64_t w = e >> 4;
+    const int w = e >> 4;
     if (w + 1 > s.top || s.top - w > 63) {
         s.top = w + 1;
-        const int64_t x = s.top - (int64_t)(threadIdx.x & 63);
+        const int x = s.top - (int)(threadIdx.x & 63);
         s.v = (x >= 0 && x < words) ? src[x] : 0u;
     }
     const int k = (int)(s.top - w);  // 1 .. 63: word w in lane k, word w+1 in lane k-1
@@ -424,8 +425,9 @@ __device__ __forceinline__ uint32_t seq_bits_win(const uint32_t *src, int64_t wo
 // WIN = false (a lane per pair): the plane dword of the same diagonal one
 // block up is prefetched.  WIN = true (a wavefront per pair, every lane
 // active, state wave-uniform): planes and sequences come from the windows
-// above and `ld` is not called.
-template <int C, bool WIN = false, typename Ld, typename Put>
+// above and `ld` is not called.  NMK = false: the caller's batch has no N
+// masks (c.nm is 0), so the mask windows and their registers go away.
+template <int C, bool WIN = false, bool NMK = true, typename Ld, typename Put>
 __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int h, int off,
                                            int64_t qo, int64_t to, const uint32_t *tb, Ld ld,
                                            Put put, int &nr, int (&cnt)[4])
@@ -439,10 +441,11 @@ __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int
     SeqWindow qwin, twin, qmwin, tmwin;
     PlaneWin pw;
     SeqWin qsw, tsw, qmsw, tmsw;
-    auto seq = [&](const uint32_t *src, int64_t words, int64_t e, SeqWindow &cw, SeqWin &sw) {
-        if constexpr (WIN) return seq_bits_win(src, words, e, sw);
-        else return seq_bits(src, words, e, cw, ld);
-    };
+    // WIN: sequence positions from the pair's first words (32-bit)
+    const int64_t qw0 = qo >> 4, tw0 = to >> 4;
+    const int qb = (int)(qo & 15), tbs = (int)(to & 15);
+    const int qwn = (int)min(c.q_words - qw0, (int64_t)INT_MAX);
+    const int twn = (int)min(c.t_words - tw0, (int64_t)INT_MAX);
     auto emit = [&](int op, int n) {
         if (op == run_op) {
             run_len += n;
@@ -495,13 +498,23 @@ __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int
                 m = min(m, min(i, j));
                 if (h <= (m - 1) * c.ma) m = 1;
                 const uint32_t lenmask = (m >= 16) ? ~0u : ((1u << (2 * m)) - 1u);
-                const int64_t eq = qo + i - m, et = to + j - m;
-                uint32_t x = seq(c.qseq, c.q_words, eq, qwin, qsw) ^
-                             seq(c.tseq, c.t_words, et, twin, tsw);
-                x = (x | (x >> 1)) & 0x55555555u;
-                if (c.nm)
-                    x |= (seq(c.qmask, c.q_words, eq, qmwin, qmsw) |
-                          seq(c.tmask, c.t_words, et, tmwin, tmsw)) & 0x55555555u;
+                uint32_t x, xm = 0u;
+                if constexpr (WIN) {
+                    const int eq = qb + i - m, et = tbs + j - m;
+                    x = seq_bits_win(c.qseq + qw0, qwn, eq, qsw) ^
+                        seq_bits_win(c.tseq + tw0, twn, et, tsw);
+                    if (NMK && c.nm)
+                        xm = seq_bits_win(c.qmask + qw0, qwn, eq, qmsw) |
+                             seq_bits_win(c.tmask + tw0, twn, et, tmsw);
+                } else {
+                    const int64_t eq = qo + i - m, et = to + j - m;
+                    x = seq_bits(c.qseq, c.q_words, eq, qwin, ld) ^
+                        seq_bits(c.tseq, c.t_words, et, twin, ld);
+                    if (NMK && c.nm)
+                        xm = seq_bits(c.qmask, c.q_words, eq, qmwin, ld) |
+                             seq_bits(c.tmask, c.t_words, et, tmwin, ld);
+                }
+                x = ((x | (x >> 1)) | xm) & 0x55555555u;
                 x &= lenmask;  // bit 2t: base i-m+t mismatches
                 const int nmis = __builtin_popcount(x);
                 h -= c.ma * (m - nmis) - c.mi * nmis;
@@ -622,7 +635,8 @@ __device__ __forceinline__ void walk_steps_paged(const WalkCtx &c, int &i, int &
 }
 
 // Fused-path walk of one pair by a whole wavefront (state wave-uniform).
-template <int C>
+// NM: the batch may hold N masks.
+template <int C, bool NM>
 __device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, int p, int i, int j,
                                           int h, int off, int64_t qo, int64_t to,
                                           const uint32_t *tb)
@@ -637,7 +651,7 @@ __device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, in
     auto put = [&](int x, uint32_t vrun) {
         if (lane == 0) out[x] = vrun;
     };
-    walk_steps<C, true>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    walk_steps<C, true, NM>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     for (int x = lane; x < nr / 2; x += 64) {
@@ -699,7 +713,7 @@ __device__ __forceinline__ void walk_pair_paged(const WalkCtx &c, const WalkOut 
 // in lane x & 63) and stored by the whole wavefront at once.  Runs past cap
 // are counted but not stored.  The outputs record follows as one store per
 // field lane.
-template <int C>
+template <int C, bool NM>
 __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, int cap,
                                                ServeRes *res, int i, int j, int h, int off,
                                                int64_t qo, int64_t to, const uint32_t *tb,
@@ -721,7 +735,7 @@ __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, 
         }
     };
     const int score = h, qend = i, tend = j;
-    walk_steps<C, true>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
+    walk_steps<C, true, NM>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
     if (lane < (nr & 63)) {
         const int y = cap - 1 - ((nr & ~63) + lane);
         if (y >= 0) cig[y] = buf;
@@ -1402,7 +1416,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                                             a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s,
                                             off, qo, to, ldp, claim_tick);
                 } else {
-                    walk_pair_ring<C>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap,
+                    walk_pair_ring<C, NM>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap,
                                       a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s, off,
                                       qo, to, a.tb + tb_base, claim_tick);
                 }
@@ -1459,7 +1473,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                                            __builtin_amdgcn_readlane(res_s, L0), offg, qog, tog,
                                            ldp);
                     } else {
-                        walk_pair<C>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
+                        walk_pair<C, NM>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
                                      __builtin_amdgcn_readlane(res_j, L0),
                                      __builtin_amdgcn_readlane(res_s, L0), offg, qog, tog,
                                      a.tb + tbg);
