@@ -298,6 +298,47 @@ def test_sw_edge_cases_vs_oracle():
             _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"edge{hb}")
 
 
+@pytest.mark.parametrize("with_n", [False, True])
+def test_walk_window_edges_vs_oracle(with_n):
+    """The wavefront walk reads its planes through a 4-block x 16-diagonal
+    window (plus the 4 blocks above) and its sequences through 64-word
+    windows.  Cases at the windows' edges: one gap of 12-40 bases per pair
+    (the walk leaves the window sideways), pairs of 3-60 rows (the window
+    ahead reaches past block 0), pairs packed back to back so that sequence
+    offsets fall at every position inside a 16-base word, and N bases (the
+    N-mask kernels and their mask windows).  Fused walk and standalone walk
+    against the oracle, bit for bit."""
+    rng = np.random.default_rng(77 + int(with_n))
+    qs, ts, offs = [], [], []
+    for k in range(64):
+        n = int(rng.integers(3, 61)) if k % 4 == 0 else int(rng.integers(100, 700))
+        q = rng.integers(1, 5, size=n).astype(np.int16)
+        t = q.tolist()
+        g = int(rng.integers(12, 41))
+        p = int(rng.integers(0, n))
+        if k % 2:
+            t[p:p] = rng.integers(1, 5, size=g).tolist()  # reference insertion: D run
+        else:
+            del t[p:p + g]  # reference deletion: I run
+        for _ in range(int(rng.integers(0, 6))):
+            if t:
+                t[int(rng.integers(0, len(t)))] = int(rng.integers(1, 5))
+        t = rng.integers(1, 5, size=int(rng.integers(0, 17))).tolist() + t
+        t = np.array(t, np.int16)
+        if with_n and k % 3 == 0:
+            q[int(rng.integers(0, n))] = 5
+            if len(t):
+                t[int(rng.integers(0, len(t)))] = 5
+        qs.append(q)
+        ts.append(t)
+        offs.append(int(rng.integers(-8, 9)))
+    for hb in (48, 200):
+        for fused in (False, True):
+            _say("case: walk window edges, N", with_n, "hb", hb, "fused", fused)
+            got = _gpu_align(qs, ts, offs, hb, (2, 4, 4, 2), fused=fused)
+            _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"win{hb}{fused}")
+
+
 def test_xdrop_small_groups_vs_host_restatement():
     """Groups of 1-16 pairs (the row-parallel x-drop kernel: 64 rows per tile,
     several tiles per pair, stops inside and at the edges of tiles) against
