@@ -507,6 +507,8 @@ __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int
                         xm = seq_bits_win(c.qmask + qw0, qwn, eq, qmsw) |
                              seq_bits_win(c.tmask + tw0, twn, et, tmsw);
                 } else {
+                    // a lane per pair: absolute 64-bit positions on the shared
+                    // stream bases (pair-relative bases per lane measured slower)
                     const int64_t eq = qo + i - m, et = to + j - m;
                     x = seq_bits(c.qseq, c.q_words, eq, qwin, ld) ^
                         seq_bits(c.tseq, c.t_words, et, twin, ld);
@@ -828,6 +830,19 @@ __global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
     a.o.counts[4 * p + 1] = cnt[1];
     a.o.counts[4 * p + 2] = cnt[2];
     a.o.counts[4 * p + 3] = cnt[3];
+}
+
+// Standalone walk of a batch too small to fill the SIMDs: a wavefront per
+// pair (every lane active, state wave-uniform) reading planes and sequences
+// through the lane windows, so each pair's dependent loads come once per
+// window instead of once per block or 16-base word.
+template <int C>
+__global__ __launch_bounds__(64) void sw_traceback_wave_kernel(TbKArgs a)
+{
+    const int p = (int)blockIdx.x;
+    if (p >= a.n_pairs) return;
+    walk_pair<C, true>(a.c, a.o, p, a.qend[p], a.tend[p], a.score[p], a.boff[p], a.q_off[p],
+                       a.t_off[p], a.tb + a.tb_off[p]);
 }
 
 // Scalar-unit operations for the scalar cell of KS == C tiles.  Compiled
@@ -1784,10 +1799,12 @@ struct TileEntry {
     SwKernel (*get)(uint32_t);
     void (*walk)(TbKArgs);
     SwKernel (*serve)(uint32_t);
+    void (*walk_wave)(TbKArgs);
 };
 
-#define GMFA_TILE(C, G, KS) \
-    {C, G, KS, &pick_kernel<C, G, KS>, &sw_traceback_kernel<C>, &pick_serve_kernel<C, G, KS>}
+#define GMFA_TILE(C, G, KS)                                                                   \
+    {C, G, KS, &pick_kernel<C, G, KS>, &sw_traceback_kernel<C>, &pick_serve_kernel<C, G, KS>, \
+     &sw_traceback_wave_kernel<C>}
 static const TileEntry kTiles[] = {
     // exact fits of the benchmark bands: W=65 (5x13), W=513 (9x57, or 8x64 + the scalar
     // cell); W=1503 (24x63, 15 live registers in the last lane)
@@ -2207,8 +2224,11 @@ extern "C" int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_param
         cus = 256;
     const int64_t slots = (int64_t)8 * 4 * cus;  // 8 wavefronts on each SIMD
     a.per_wave = (int)std::max<int64_t>(1, std::min<int64_t>(64, (n + slots - 1) / slots));
-    hipLaunchKernelGGL(t->walk, dim3((n + a.per_wave - 1) / a.per_wave), dim3(64), 0,
-                       (hipStream_t)stream, a);
+    if ((int64_t)n <= (int64_t)4 * cus)  // at most one pair per SIMD: a wavefront per pair
+        hipLaunchKernelGGL(t->walk_wave, dim3(n), dim3(64), 0, (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(t->walk, dim3((n + a.per_wave - 1) / a.per_wave), dim3(64), 0,
+                           (hipStream_t)stream, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }
