@@ -832,10 +832,11 @@ __global__ __launch_bounds__(64) void sw_traceback_kernel(TbKArgs a)
     a.o.counts[4 * p + 3] = cnt[3];
 }
 
-// Standalone walk of a batch too small to fill the SIMDs: a wavefront per
-// pair (every lane active, state wave-uniform) reading planes and sequences
-// through the lane windows, so each pair's dependent loads come once per
-// window instead of once per block or 16-base word.
+// Standalone walk of a batch whose pairs would each get a wavefront of their
+// own anyway (at most 8 wavefronts per SIMD): a wavefront per pair (every
+// lane active, state wave-uniform) reading planes and sequences through the
+// lane windows, so each pair's dependent loads come once per window instead
+// of once per block or 16-base word.
 template <int C>
 __global__ __launch_bounds__(64) void sw_traceback_wave_kernel(TbKArgs a)
 {
@@ -2224,7 +2225,8 @@ extern "C" int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_param
         cus = 256;
     const int64_t slots = (int64_t)8 * 4 * cus;  // 8 wavefronts on each SIMD
     a.per_wave = (int)std::max<int64_t>(1, std::min<int64_t>(64, (n + slots - 1) / slots));
-    if ((int64_t)n <= (int64_t)4 * cus)  // at most one pair per SIMD: a wavefront per pair
+    // one pair per wavefront anyway: let the whole wavefront walk it (windows)
+    if (a.per_wave == 1)
         hipLaunchKernelGGL(t->walk_wave, dim3(n), dim3(64), 0, (hipStream_t)stream, a);
     else
         hipLaunchKernelGGL(t->walk, dim3((n + a.per_wave - 1) / a.per_wave), dim3(64), 0,
