@@ -1599,18 +1599,33 @@ __global__ __launch_bounds__(64) void sw_xdrop_rows_kernel(
     const int64_t base_l = lane < n ? rk_off[p0 + lane] : 0;
     const int maxq = wave_max(ql_l);
     int seen_l = 0, bh_l = 0, bi_l = 0, bd_l = 0;  // lane k: pair k's state
-    for (int t0 = 1; t0 <= maxq; t0 += 64) {
-        const int i = t0 + lane;  // this lane's row
-        int kk[kXdropRowsPairs];
-        bool all = thr >= 0 && i <= maxq;
+    // the keys of the next 64 rows are loaded while this chunk is scanned
+    // (two wavefronts per SIMD cannot hide the load latency otherwise)
+    int nx[kXdropRowsPairs];
+    auto load_rows = [&](int t0v) {
 #pragma unroll
         for (int k = 0; k < kXdropRowsPairs; ++k) {
-            kk[k] = 0;
+            nx[k] = 0;
             if (k < n) {
                 const int ql = __builtin_amdgcn_readlane(ql_l, k);
                 const int64_t base = readlane64(base_l, k);
-                const int key = i <= ql ? rowkey[base + i - 1] : 0;  // rows past qlen: H = 0
-                kk[k] = key;
+                const int iv = t0v + lane;
+                nx[k] = iv <= ql ? rowkey[base + iv - 1] : 0;  // rows past qlen: H = 0
+            }
+        }
+    };
+    load_rows(1);
+    for (int t0 = 1; t0 <= maxq; t0 += 64) {
+        const int i = t0 + lane;  // this lane's row
+        int kk[kXdropRowsPairs];
+#pragma unroll
+        for (int k = 0; k < kXdropRowsPairs; ++k) kk[k] = nx[k];
+        if (t0 + 64 <= maxq) load_rows(t0 + 64);
+        bool all = thr >= 0 && i <= maxq;
+#pragma unroll
+        for (int k = 0; k < kXdropRowsPairs; ++k) {
+            if (k < n) {
+                const int key = kk[k];
                 const int h = key >> 11;
                 const int pm = max(__builtin_amdgcn_readlane(seen_l, k), seg_scan_max<64>(h));
                 all = all && pm - h > thr;
