@@ -21,6 +21,7 @@ GMFA_E_INVALID = -1
 GMFA_E_RUNTIME = -2
 GMFA_E_UNSUPPORTED = -3
 GMFA_E_AGAIN = -4
+GMFA_NOALLOC_SCRATCH_BYTES = 256 << 20
 GMFA_SERVE_TRUNCATED = 1
 GMFA_SERVE_NO_PAGES = 2
 
@@ -117,6 +118,9 @@ SIGNATURES = {
     "gmfa_pack_flat": (ctypes.c_int, [_P, _i64, _i32, _P, _P, _P, _P]),
     "gmfa_sw_forward": (ctypes.c_int, [ctypes.POINTER(SwBatch), ctypes.POINTER(SwParams),
                                        ctypes.POINTER(SwOut), _P]),
+    "gmfa_sw_forward_noalloc": (ctypes.c_int, [ctypes.POINTER(SwBatch),
+                                               ctypes.POINTER(SwParams),
+                                               ctypes.POINTER(SwOut), _P]),
     "gmfa_sw_traceback": (ctypes.c_int, [ctypes.POINTER(SwBatch), ctypes.POINTER(SwParams),
                                          _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gmfa_sw_xdrop": (ctypes.c_int, [_P, _P, _P, _P, _P, _i32, _i32, _i32, _i32, _P, _P, _P,

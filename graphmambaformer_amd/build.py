@@ -19,7 +19,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libgmfa.so")
-SOURCES = ["api.hip", "sw.hip", "chain.hip", "chain_post.hip", "serve.hip", "seed.hip", "wfa.hip"]
+SOURCES = ["api.hip", "sw.hip", "chain.hip", "chain_post.hip", "serve.hip", "seed.hip", "wfa.hip",
+           "noalloc.hip"]
 ARCH = "gfx950"
 
 

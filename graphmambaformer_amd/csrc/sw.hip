@@ -1102,9 +1102,14 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 
         const int nblk = nopages ? 0 : (maxq + 7) >> 3;
         const int nblk_mine = (qlen + 7) >> 3;
-        // serving: each wave slot owns one plane slab, reused pair after pair
-        const int64_t tb_base = (TB && has) ? (SV ? (int64_t)blockIdx.x * a.slab_words : a.tb_off[p])
-                                            : 0;
+        // serving, and offline batches walked in the launch with slab planes
+        // (slab_words > 0): each wave slot (pair group of a wave) owns one
+        // plane slab, reused pair after pair
+        const int64_t tb_base =
+            (TB && has) ? ((SV || a.slab_words > 0)
+                               ? ((int64_t)blockIdx.x * (64 / G) + grp) * a.slab_words
+                               : a.tb_off[p])
+                        : 0;
         const int64_t rk_base = (RK && has) ? a.rk_off[p] : 0;
 
         // target element (0-based in the pair) of cell 0 at the block's first row
@@ -1998,7 +2003,7 @@ int serve_reset_pages(int32_t *owner, int32_t n_pages, hipStream_t s)
 }
 
 int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gmfa_sw_out *out,
-              const ServeLaunch *serve, hipStream_t s)
+              const ServeLaunch *serve, hipStream_t s, const SlabPlanes *slabs)
 {
     GMFA_REQUIRE(batch && params && out, "NULL argument");
     const gmfa_sw_params &P = *params;
@@ -2030,10 +2035,14 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                  "per-row keys need query length x match below 2^19");
     const bool nm = P.flags & GMFA_SW_NMASK;
     GMFA_REQUIRE(!nm || (batch->qmask && batch->tmask), "GMFA_SW_NMASK needs qmask and tmask");
-    GMFA_REQUIRE(online || (out->score && out->query_end && out->target_end && out->work_counter),
+    GMFA_REQUIRE(online || (out->score && out->query_end && out->target_end &&
+                            (out->work_counter || slabs)),
                  "NULL output pointer");
-    GMFA_REQUIRE(!(P.flags & GMFA_SW_TRACEBACK) || (out->tb && (out->tb_off || serve)),
+    GMFA_REQUIRE(!(P.flags & GMFA_SW_TRACEBACK) || slabs || (out->tb && (out->tb_off || serve)),
                  "GMFA_SW_TRACEBACK needs tb and tb_off");
+    GMFA_REQUIRE(!slabs || (!serve && (P.flags & GMFA_SW_TRACEBACK) && (P.flags & GMFA_SW_CIGAR) &&
+                            !(P.flags & GMFA_SW_ROWKEYS)),
+                 "slab planes need GMFA_SW_TRACEBACK | GMFA_SW_CIGAR and no row keys");
     GMFA_REQUIRE(!(P.flags & GMFA_SW_ROWKEYS) || (out->rowkey && out->rowkey_off),
                  "GMFA_SW_ROWKEYS needs rowkey and rowkey_off");
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, 2 * P.half_band + 1);
@@ -2050,9 +2059,10 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     a.hb = P.half_band; a.W = 2 * P.half_band + 1;
     a.ma = P.match; a.mi = P.mismatch; a.go = P.gap_open; a.ge = P.gap_extend;
     a.score = out->score; a.qend = out->query_end; a.tend = out->target_end;
-    a.tb = out->tb; a.tb_off = out->tb_off;
+    a.tb = slabs ? slabs->tb : out->tb;
+    a.tb_off = slabs ? nullptr : out->tb_off;
     a.rowkey = out->rowkey; a.rk_off = out->rowkey_off;
-    a.counter = out->work_counter;
+    a.counter = slabs ? slabs->counter : out->work_counter;
     a.walk = 0;
     a.wc = make_walk_ctx(batch, params, tile, nm);
     a.wo = WalkOut{};
@@ -2080,6 +2090,19 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     int waves = 1;
     if (!serve) {
         waves = std::max(1, std::min(a.n_items, resident_waves(k)));
+        if (slabs) {
+            // one slab per pair group, sized for the longest query (rounded
+            // to 256 bytes); as many wavefronts as the slabs fit
+            const int64_t need = (gmfa_sw_tb_bytes(batch->max_qlen, &tile) / 4 + 63) / 64 * 64;
+            a.slab_words = std::max<int64_t>(need, 64);
+            const int64_t fit = slabs->cap_words / (a.slab_words * ppw);
+            GMFA_REQUIRE(fit >= 1,
+                         "the traceback planes of %d pair(s) of query length %d need %lld bytes, "
+                         "above the %lld bytes of the library's static scratch",
+                         ppw, batch->max_qlen, (long long)(a.slab_words * ppw * 4),
+                         (long long)(slabs->cap_words * 4));
+            waves = (int)std::min<int64_t>(waves, fit);
+        }
     } else {
         GMFA_REQUIRE(t->G == 64 && k != nullptr,
                      "serving needs a tile with one pair per wavefront (64 lanes per pair), got %d",

@@ -95,11 +95,23 @@ constexpr int kMaxPages = 256;
 // caller might capture).
 int serve_reset_pages(int32_t *owner, int32_t n_pages, hipStream_t s);
 
+// Offline batch whose traceback planes live in per-wave slabs instead of one
+// region per pair: the CIGAR walk runs in the same launch, so a slab is free
+// again once its pair is walked.  Pair group g of wavefront w writes its
+// planes at tb + (w * pairs_per_wave + g) * slab_words; the launch takes as
+// many wavefronts as the slabs fit in cap_words (gmfa_sw_forward_noalloc).
+struct SlabPlanes {
+    uint32_t *tb;        // cap_words words of device memory
+    uint32_t *counter;   // work counter (one word, reset by the launch)
+    int64_t cap_words;   // words at tb
+};
+
 // Validates and launches the forward kernel (work-counter reset + the kernel)
-// on stream s; serve == nullptr is the offline batch of gmfa_sw_forward.  The
-// online form does not reset its head (relaunches continue the ticket count).
+// on stream s; serve == nullptr and slabs == nullptr is the offline batch of
+// gmfa_sw_forward.  The online form does not reset its head (relaunches
+// continue the ticket count).
 int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gmfa_sw_out *out,
-              const ServeLaunch *serve, hipStream_t s);
+              const ServeLaunch *serve, hipStream_t s, const SlabPlanes *slabs = nullptr);
 
 // Query words a stage slot holds for queries up to max_qlen bases.
 inline int32_t serve_stage_words(int32_t max_qlen) { return (max_qlen + 15) / 16 + 2; }

@@ -379,6 +379,53 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
                      rowkey=rk, rowkey_off_host=rk_off_host, tile=tile, params=params, walk=walk)
 
 
+def sw_forward_noalloc(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int,
+                       order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
+                       outputs: Optional[dict] = None) -> SwForward:
+    """Forward pass with the CIGAR walk fused, through gmfa_sw_forward_noalloc:
+    the work counter and the traceback planes live in the library's static
+    device scratch (one slab per wavefront), so the call allocates no device
+    memory of its own.  ``outputs`` may hold preallocated output tensors
+    (keys score, query_end, target_end and those of ``_cigar_buffers``'s
+    record: cigar, cigar_off, n_cigar, read_start, ref_start, counts); the
+    missing ones are allocated here by torch.  Calls that overlap on one GPU
+    share the scratch and race."""
+    dev = inp.device
+    n = inp.n_pairs
+    tile = tile or batch_tile(half_band, n, dev)
+    flags = _lib.SW_TRACEBACK | _lib.SW_CIGAR
+    if inp.has_n:
+        flags |= _lib.SW_NMASK
+    o = dict(outputs or {})
+    for key in ("score", "query_end", "target_end", "n_cigar", "read_start", "ref_start"):
+        if key not in o:
+            o[key] = torch.empty(n, dtype=torch.int32, device=dev)
+    if "counts" not in o:
+        o["counts"] = torch.empty((n, 4), dtype=torch.int32, device=dev)
+    if "cigar" not in o:
+        cap = inp.qlen_host.astype(np.int64) + inp.tlen_host.astype(np.int64) + 1
+        offs = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(cap, out=offs[1:])
+        o["cigar"] = torch.empty(max(int(offs[-1]), 1), dtype=torch.int32, device=dev)
+        o["cigar_off"] = torch.from_numpy(offs[:-1]).to(dev)
+    out = SwOut()
+    out.score, out.query_end, out.target_end = ptr(o["score"]), ptr(o["query_end"]), \
+        ptr(o["target_end"])
+    out.cigar_off, out.cigar = ptr(o["cigar_off"]), ptr(o["cigar"])
+    out.n_cigar, out.read_start = ptr(o["n_cigar"]), ptr(o["read_start"])
+    out.ref_start, out.counts = ptr(o["ref_start"]), ptr(o["counts"])
+    params = make_params(ma, mi, go, ge, half_band, flags, tile)
+    batch = inp.batch_struct(order)
+    check(lib().gmfa_sw_forward_noalloc(ctypes.byref(batch), ctypes.byref(params),
+                                        ctypes.byref(out), stream_of(dev)),
+          "gmfa_sw_forward_noalloc")
+    walk = SwTraceback(cigar=o["cigar"], cigar_off=o["cigar_off"], n_cigar=o["n_cigar"],
+                       read_start=o["read_start"], ref_start=o["ref_start"], counts=o["counts"])
+    return SwForward(score=o["score"], query_end=o["query_end"], target_end=o["target_end"],
+                     tb=None, tb_off=None, rowkey=None, rowkey_off_host=None, tile=tile,
+                     params=params, walk=walk)
+
+
 class SwGraph:
     """One batch step on a resident batch -- the work-counter reset and the
     persistent forward kernel, with the CIGAR walk fused when ``cigar`` --

@@ -173,6 +173,25 @@ int gmfa_pack_flat(const uint8_t *codes, int64_t n, int32_t two_bit, uint32_t *s
 int gmfa_sw_forward(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
                     const gmfa_sw_out *out, void *stream);
 
+/* Size of the static device scratch of gmfa_sw_forward_noalloc (per GPU). */
+#define GMFA_NOALLOC_SCRATCH_BYTES (256ll << 20)
+
+/* gmfa_sw_forward with GMFA_SW_TRACEBACK | GMFA_SW_CIGAR, allocating nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  The
+ * work counter and the traceback planes live in a static device array of
+ * GMFA_NOALLOC_SCRATCH_BYTES per GPU, the planes as one slab per wavefront
+ * sized for batch->max_qlen and reused pair after pair (the CIGAR walk runs
+ * in the same launch), so the launch takes as many wavefronts as the slabs
+ * fit: short queries run at full occupancy, long ones on fewer wavefronts.
+ * out: score, query_end, target_end and the CIGAR outputs as for
+ * gmfa_sw_forward; out->tb, tb_off, work_counter and rowkey must be NULL.
+ * GMFA_E_INVALID if one wavefront's slabs exceed the scratch.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams
+ * or threads) race. */
+int gmfa_sw_forward_noalloc(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
+                            const gmfa_sw_out *out, void *stream);
+
 /* Batch-wide x-drop (graphmambaformer/alignment/extension.py:186-193) per
  * group of pairs, from the per-row keys of a GMFA_SW_ROWKEYS forward pass.
  * Group g = pairs group_off[g] .. group_off[g+1]-1 (device int32 array, at

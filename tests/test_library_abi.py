@@ -185,3 +185,31 @@ def test_serve_session_validates_before_touching_the_device():
     assert L.gmfa_serve_poll(None, None, 0, None, 0, ctypes.byref(n)) == E
     assert L.gmfa_serve_close(None, None) == E
     assert L.gmfa_serve_stage_words(100) == (100 + 15) // 16 + 2
+
+
+def test_noalloc_validates_before_touching_the_device():
+    """gmfa_sw_forward_noalloc rejects a batch whose planes exceed its static
+    scratch, and caller-supplied plane / counter buffers, before any HIP call."""
+    import ctypes
+
+    from graphmambaformer_amd import _lib
+
+    L = _lib.lib()
+    E = _lib.GMFA_E_INVALID
+    b = _lib.SwBatch()
+    p = _lib.SwParams()
+    o = _lib.SwOut()
+    b.n_pairs, b.max_qlen = 1, 1_000_000
+    p.match, p.mismatch, p.gap_open, p.gap_extend = 2, 4, 4, 2
+    p.half_band = 1023
+    p.flags = _lib.SW_TRACEBACK  # no fused walk
+    assert L.gmfa_sw_forward_noalloc(ctypes.byref(b), ctypes.byref(p), ctypes.byref(o), None) == E
+    p.flags = _lib.SW_TRACEBACK | _lib.SW_CIGAR
+    o.tb = 0x1000  # planes are the library's
+    assert L.gmfa_sw_forward_noalloc(ctypes.byref(b), ctypes.byref(p), ctypes.byref(o), None) == E
+    assert b"must be NULL" in L.gmfa_last_error()
+    o.tb = None
+    # 125,000 blocks x 2,048 plane dwords: about 1 GB for one wavefront
+    assert L.gmfa_sw_forward_noalloc(ctypes.byref(b), ctypes.byref(p), ctypes.byref(o), None) == E
+    assert b"static scratch" in L.gmfa_last_error()
+    assert _lib.GMFA_NOALLOC_SCRATCH_BYTES == 256 << 20
