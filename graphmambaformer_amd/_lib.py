@@ -41,7 +41,8 @@ _i64 = ctypes.c_int64
 class SwParams(ctypes.Structure):
     _fields_ = [("match", _i32), ("mismatch", _i32), ("gap_open", _i32),
                 ("gap_extend", _i32), ("half_band", _i32), ("flags", ctypes.c_uint32),
-                ("cells_per_lane", _i32), ("lanes_per_pair", _i32)]
+                ("cells_per_lane", _i32), ("lanes_per_pair", _i32),
+                ("split_rows", _i32), ("split_slices", _i32)]
 
 
 class SwBatch(ctypes.Structure):
@@ -114,6 +115,7 @@ SIGNATURES = {
     "gmfa_sw_resolve_tile": (ctypes.c_int, [_i32, _i32, _i32, ctypes.POINTER(SwTile)]),
     "gmfa_sw_list_tiles": (ctypes.c_int, [_P, _P, _i32]),
     "gmfa_sw_tb_bytes": (_i64, [_i32, ctypes.POINTER(SwTile)]),
+    "gmfa_sw_split_variants": (ctypes.c_int, [_i32, _i32, _P, _P, _i32]),
     "gmfa_pack_codes": (ctypes.c_int, [_P, _i64, _i64, _P, _P, _i64, _P, _P, _P, _P]),
     "gmfa_pack_flat": (ctypes.c_int, [_P, _i64, _i32, _P, _P, _P, _P]),
     "gmfa_sw_forward": (ctypes.c_int, [ctypes.POINTER(SwBatch), ctypes.POINTER(SwParams),
@@ -183,7 +185,7 @@ def lib():
                 fn = getattr(L, name)
                 fn.restype = res
                 fn.argtypes = args
-            if L.gmfa_abi_version() != 1:
+            if L.gmfa_abi_version() != 2:
                 raise LibraryMissing("libgmfa ABI version mismatch")
             _lib = L
     return _lib

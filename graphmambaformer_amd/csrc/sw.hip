@@ -47,23 +47,13 @@
 // tracks its own first maximum; the group reduces (max H, min i, min d).
 #include "common.h"
 #include "sw_internal.h"
+#include "sw_walk.h"
 
 #include <algorithm>
 #include <climits>
 
 namespace gmfa {
 
-struct WalkCtx {
-    const uint32_t *qseq, *tseq, *qmask, *tmask;
-    int64_t q_words, t_words;
-    int32_t nm, hb, W, Gs, ma, mi, go, ge;
-};
-
-struct WalkOut {
-    const int64_t *cig_off;
-    uint32_t *cigar;
-    int32_t *ncig, *rstart, *tstart, *counts;
-};
 
 struct SwKArgs {
     const uint32_t *qseq, *tseq, *qmask, *tmask;
@@ -79,6 +69,8 @@ struct SwKArgs {
     const int64_t *rk_off;
     uint32_t *counter;
     int32_t walk;  // walk every pair right after its forward pass (CIGAR out)
+    int32_t blkstride;   // plane dwords per 8-row block (lanes_stored * C of the tile)
+    int32_t split_rows;  // split launches: pairs with at least this many rows run as slices
     WalkCtx wc;
     WalkOut wo;
     // continuous serving (serve_tail == nullptr: one offline batch).  Request
@@ -109,13 +101,6 @@ struct SwKArgs {
     int32_t *page_owner;
     uint64_t alloc_ticks;
 };
-
-__device__ __forceinline__ int64_t uni64(int64_t v)
-{
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
 
 // The freshest known tail of a serving queue, and whether the host closed it.
 // Reads the device mirrors; when they are older than poll_ticks, the one
@@ -289,267 +274,6 @@ __device__ __forceinline__ uint32_t page_of(const int32_t *pl, int page)
     return (uint32_t)__builtin_amdgcn_readfirstlane(pl[page]);
 }
 
-__device__ __forceinline__ int64_t clampi64(int64_t x, int64_t hi)
-{
-    return x < 0 ? 0 : (x > hi ? hi : x);
-}
-
-template <int G>
-__device__ __forceinline__ int uni(int x)
-{
-    if constexpr (G == 64) return __builtin_amdgcn_readfirstlane(x);
-    return x;
-}
-
-template <int G>
-__device__ __forceinline__ uint32_t uniu(uint32_t x)
-{
-    if constexpr (G == 64) return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
-    return x;
-}
-
-// wave-wide "all lanes true"
-__device__ __forceinline__ bool wave_all(bool p)
-{
-    return __builtin_amdgcn_ballot_w64(!p) == 0ull;
-}
-
-__device__ __forceinline__ int wave_max(int x)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x = max(x, __shfl_xor(x, o));
-    return x;
-}
-
-// a * b + c on 24-bit unsigned operands in one VALU op.  Written as asm
-// because the compiler proves a is 0/1 and would lower the product to a
-// compare + select + add.
-__device__ __forceinline__ int mad_u24(uint32_t a, uint32_t b, int c)
-{
-    int r;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
-
-__device__ __forceinline__ int64_t readlane64(int64_t v, int l)
-{
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-// -------------------------------------------------------------------------
-// traceback walk (extension.py:249-285 decisions replayed from the planes)
-// -------------------------------------------------------------------------
-// Decision nibble of cell (i, d) of a pair's planes: bit 3 f_cont, bit 2
-// not_diag, bit 1 not_f, bit 0 e_cont.
-template <int C>
-__device__ __forceinline__ int64_t plane_word(int i, int d, int Gs)
-{
-    const int l = d / C;
-    return ((int64_t)((i - 1) >> 3) * Gs + l) * C + (d - l * C);
-}
-
-// 32 bases of a 2-bit stream starting at base e (two cached words).
-struct SeqWindow {
-    int64_t w = -2;
-    uint32_t lo = 0u, hi = 0u;
-};
-
-template <typename Ld>
-__device__ __forceinline__ uint32_t seq_bits(const uint32_t *src, int64_t words, int64_t e,
-                                             SeqWindow &c, Ld ld)
-{
-    const int64_t w = e >> 4;
-    if (w != c.w) {
-        if (w == c.w - 1) {  // walking backwards: slide the window down one word
-            c.hi = c.lo;
-            c.lo = ld(src, w);
-        } else {
-            c.lo = ld(src, w);
-            c.hi = (w + 1 < words) ? ld(src, w + 1) : 0u;
-        }
-        c.w = w;
-    }
-    return __builtin_amdgcn_alignbit(c.hi, c.lo, (uint32_t)(e & 15) * 2u);
-}
-
-// Windows of a walk that a whole wavefront runs with wave-uniform state: the
-// lanes load many dwords at once and each step picks its dword with a
-// readlane, so a dependent load is paid once per window instead of once per
-// plane block or 16-base word.  Plane window: 4 blocks x 16 diagonals (lane
-// l holds block top - (l >> 4), diagonal d0 + (l & 15)), plus the 4 blocks
-// above it loaded one window ahead.  Sequence window: 64 words (lane l holds
-// word top - l); the walk only moves towards lower words.
-struct PlaneWin {
-    int top = INT_MIN / 2, d0 = 0;
-    uint32_t v = 0u, pv = 0u;  // this window, the 4 blocks above it
-};
-struct SeqWin {
-    int top = -1;  // -1: nothing loaded
-    uint32_t v = 0u;
-};
-
-__device__ __forceinline__ uint32_t plane_win_load(const uint32_t *tb, int top, int d0,
-                                                   int blkstride, int W)
-{
-    const int l = (int)(threadIdx.x & 63);
-    const int b = top - (l >> 4), d = d0 + (l & 15);
-    // blocks 0 .. top were written (the walk starts at or below top), and
-    // diagonals 0 .. W-1 lie inside every block's blkstride >= W dwords
-    return (b >= 0 && d >= 0 && d < W) ? tb[(int64_t)b * blkstride + d] : 0u;
-}
-
-// `src`, `words` and `e` count from the pair's first sequence word, so the
-// positions fit 32 bits.
-__device__ __forceinline__ uint32_t seq_bits_win(const uint32_t *src, int words, int e, SeqWin &s)
-{
-    const int w = e >> 4;
-    if (w + 1 > s.top || s.top - w > 63) {
-        s.top = w + 1;
-        const int x = s.top - (int)(threadIdx.x & 63);
-        s.v = (x >= 0 && x < words) ? src[x] : 0u;
-    }
-    const int k = (int)(s.top - w);  // 1 .. 63: word w in lane k, word w+1 in lane k-1
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)s.v, k);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)s.v, k - 1);
-    return __builtin_amdgcn_alignbit(hi, lo, (uint32_t)(e & 15) * 2u);
-}
-
-// The walk (extension.py:243-285) shared by both walkers.  `ld(ptr, idx)`
-// loads a dword (plane or sequence); `put(x, run)` stores CIGAR run x.
-// Diagonal moves are taken in bulk: the not_diag bits of the rows below the
-// current one in the same plane dword give the run length (up to 8 rows),
-// one XOR of 2-bit base windows gives its matches and mismatches, and the
-// tracked H only has to stay positive, which h > (m-1)*match guarantees.
-// WIN = false (a lane per pair): the plane dword of the same diagonal one
-// block up is prefetched.  WIN = true (a wavefront per pair, every lane
-// active, state wave-uniform): planes and sequences come from the windows
-// above and `ld` is not called.  NMK = false: the caller's batch has no N
-// masks (c.nm is 0), so the mask windows and their registers go away.
-template <int C, bool WIN = false, bool NMK = true, typename Ld, typename Put>
-__device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int h, int off,
-                                           int64_t qo, int64_t to, const uint32_t *tb, Ld ld,
-                                           Put put, int &nr, int (&cnt)[4])
-{
-    const bool econt_ok = c.go != c.ge;
-    const int blkstride = c.Gs * C;  // dwords between vertically adjacent blocks
-    int state = 0;  // 0 H, 1 F (I), 2 E (D)
-    int run_op = -1, run_len = 0;
-    int64_t cur = -1, pf_addr = -1;
-    uint32_t v = 0u, pf = 0u;
-    SeqWindow qwin, twin, qmwin, tmwin;
-    PlaneWin pw;
-    SeqWin qsw, tsw, qmsw, tmsw;
-    // WIN: sequence positions from the pair's first words (32-bit)
-    const int64_t qw0 = qo >> 4, tw0 = to >> 4;
-    const int qb = (int)(qo & 15), tbs = (int)(to & 15);
-    const int qwn = (int)min(c.q_words - qw0, (int64_t)INT_MAX);
-    const int twn = (int)min(c.t_words - tw0, (int64_t)INT_MAX);
-    auto emit = [&](int op, int n) {
-        if (op == run_op) {
-            run_len += n;
-        } else {
-            if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
-            run_op = op;
-            run_len = n;
-        }
-    };
-    while (i > 0 && j > 0) {
-        const int d = j - i - off + c.hb;
-        if (d < 0 || d >= c.W) break;  // unreachable for planes this library wrote
-        if constexpr (WIN) {
-            // the walk moves up at most one block per step, so it leaves a
-            // window through its top-4 edge (into the window loaded ahead)
-            // or sideways
-            const int blk = (i - 1) >> 3;
-            int rb = pw.top - blk, rd = d - pw.d0;
-            if ((uint32_t)rb > 3u || (uint32_t)rd > 15u) {
-                if ((uint32_t)(rb - 4) <= 3u && (uint32_t)rd <= 15u) {
-                    pw.v = pw.pv;
-                    pw.top -= 4;
-                } else {
-                    pw.top = blk;
-                    pw.d0 = d - 8;
-                    pw.v = plane_win_load(tb, pw.top, pw.d0, blkstride, c.W);
-                }
-                pw.pv = plane_win_load(tb, pw.top - 4, pw.d0, blkstride, c.W);
-                rb = pw.top - blk;
-                rd = d - pw.d0;
-            }
-            v = (uint32_t)__builtin_amdgcn_readlane((int)pw.v, (rb << 4) | rd);
-        } else {
-            const int64_t addr = plane_word<C>(i, d, c.Gs);
-            if (addr != cur) {
-                v = (addr == pf_addr) ? pf : ld(tb, addr);
-                cur = addr;
-                pf_addr = (i > 8) ? addr - blkstride : -1;
-                if (pf_addr >= 0) pf = ld(tb, pf_addr);
-            }
-        }
-        const int r0 = (i - 1) & 7;
-        const uint32_t nib = (v >> (28 - 4 * r0)) & 15u;
-        if (state == 0) {
-            if (h <= 0) break;
-            if (!(nib & 4u)) {
-                // rows r0, r0-1, ... of this dword while not_diag stays 0
-                const uint32_t y = (v & 0x44444444u) & (~0u << (30 - 4 * r0));
-                int m = y ? r0 - ((30 - (int)__builtin_ctz(y)) >> 2) : r0 + 1;
-                m = min(m, min(i, j));
-                if (h <= (m - 1) * c.ma) m = 1;
-                const uint32_t lenmask = (m >= 16) ? ~0u : ((1u << (2 * m)) - 1u);
-                uint32_t x, xm = 0u;
-                if constexpr (WIN) {
-                    const int eq = qb + i - m, et = tbs + j - m;
-                    x = seq_bits_win(c.qseq + qw0, qwn, eq, qsw) ^
-                        seq_bits_win(c.tseq + tw0, twn, et, tsw);
-                    if (NMK && c.nm)
-                        xm = seq_bits_win(c.qmask + qw0, qwn, eq, qmsw) |
-                             seq_bits_win(c.tmask + tw0, twn, et, tmsw);
-                } else {
-                    // a lane per pair: absolute 64-bit positions on the shared
-                    // stream bases (pair-relative bases per lane measured slower)
-                    const int64_t eq = qo + i - m, et = to + j - m;
-                    x = seq_bits(c.qseq, c.q_words, eq, qwin, ld) ^
-                        seq_bits(c.tseq, c.t_words, et, twin, ld);
-                    if (NMK && c.nm)
-                        xm = seq_bits(c.qmask, c.q_words, eq, qmwin, ld) |
-                             seq_bits(c.tmask, c.t_words, et, tmwin, ld);
-                }
-                x = ((x | (x >> 1)) | xm) & 0x55555555u;
-                x &= lenmask;  // bit 2t: base i-m+t mismatches
-                const int nmis = __builtin_popcount(x);
-                h -= c.ma * (m - nmis) - c.mi * nmis;
-                cnt[0] += m - nmis;
-                cnt[1] += nmis;
-                if (x == 0u) {
-                    emit(GMFA_OP_EQ, m);
-                } else {
-                    for (int t = m - 1; t >= 0; --t)
-                        emit(((x >> (2 * t)) & 1u) ? GMFA_OP_X : GMFA_OP_EQ, 1);
-                }
-                i -= m;
-                j -= m;
-            } else {
-                state = (nib & 2u) ? 2 : 1;
-            }
-        } else if (state == 1) {
-            emit(GMFA_OP_I, 1);
-            cnt[2]++;
-            --i;
-            if (nib & 8u) h += c.ge;
-            else { h += c.go; state = 0; }
-        } else {
-            emit(GMFA_OP_D, 1);
-            cnt[3]++;
-            --j;
-            if (econt_ok && (nib & 1u)) h += c.ge;
-            else { h += c.go; state = 0; }
-        }
-    }
-    if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
-}
-
 // The same walk reading the planes through a plane-load functor ldp(block,
 // diagonal): serving with paged planes.
 template <int C, typename Ld, typename Ldp, typename Put>
@@ -634,43 +358,6 @@ __device__ __forceinline__ void walk_steps_paged(const WalkCtx &c, int &i, int &
         }
     }
     if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
-}
-
-// Fused-path walk of one pair by a whole wavefront (state wave-uniform).
-// NM: the batch may hold N masks.
-template <int C, bool NM>
-__device__ __forceinline__ void walk_pair(const WalkCtx &c, const WalkOut &o, int p, int i, int j,
-                                          int h, int off, int64_t qo, int64_t to,
-                                          const uint32_t *tb)
-{
-    const int lane = (int)(threadIdx.x & 63);
-    uint32_t *out = o.cigar + o.cig_off[p];
-    int nr = 0;
-    int cnt[4] = {0, 0, 0, 0};
-    auto ld = [&](const uint32_t *src, int64_t idx) {
-        return (uint32_t)__builtin_amdgcn_readfirstlane((int)src[idx]);
-    };
-    auto put = [&](int x, uint32_t vrun) {
-        if (lane == 0) out[x] = vrun;
-    };
-    walk_steps<C, true, NM>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int x = lane; x < nr / 2; x += 64) {
-        const int y = nr - 1 - x;
-        const uint32_t va = out[x], vb = out[y];
-        out[x] = vb;
-        out[y] = va;
-    }
-    if (lane == 0) {
-        o.ncig[p] = nr;
-        o.rstart[p] = i;
-        o.tstart[p] = j;
-        o.counts[4 * p + 0] = cnt[0];
-        o.counts[4 * p + 1] = cnt[1];
-        o.counts[4 * p + 2] = cnt[2];
-        o.counts[4 * p + 3] = cnt[3];
-    }
 }
 
 // walk_pair for paged planes.
@@ -930,6 +617,381 @@ __device__ __forceinline__ void scalar_cell(int &Hs, int &Fs, uint32_t &acc, int
     Fs = fs;
 }
 
+// -------------------------------------------------------------------------
+// Split pairs: the longest pairs of a batch over several wavefronts
+// -------------------------------------------------------------------------
+// One pair on one wavefront cannot finish before its own row chain does: the
+// longest pair of a batch of uneven lengths sets the batch's end.  Here a
+// workgroup of K wavefronts takes such a pair and gives wavefront s the band
+// slice of diagonals [s*64*C, (s+1)*64*C): lane l of slice s owns diagonals
+// s*64*C + l*C .. +C-1, as lane s*64 + l of one K*64-lane wavefront would.
+// Two values cross a slice boundary per query row:
+//   * to the right, the horizontal prefix P(i) = max over the diagonals left
+//     of the boundary of M(i,d) + d*e (and the e_cont difference of the last
+//     diagonal), needed by the whole row i of the next slice;
+//   * to the left, the vertical input H(i-1, dN), F(i-1, dN) of the slice's
+//     last diagonal, where dN is the next slice's first diagonal.
+// The first makes slice s depend on slice s-1 in the same row, the second on
+// slice s+1 in the row above, so the two cannot be ordered row by row.  The
+// slice therefore computes the cell (i-1, dN) itself, a "ghost" cell on the
+// scalar unit, from its own prefix P(i-1) and slice s+1's row i-2 values at
+// dN and dN+1.  Then slice s runs row i at step i + s, reading values its
+// neighbours wrote in step i + s - 1 through LDS (two parity slots), with one
+// workgroup barrier per step.  The ghost cell equals what slice s+1 computes
+// for the same cell (exact int32), so every cell, plane nibble and best key is
+// the one-wavefront result; planes keep the tile's layout (dword d of a block
+// = band diagonal d), so the walkers are unchanged.
+//
+// A split launch is one persistent grid of K-wavefront workgroups.  Phase 1: a
+// workgroup claims the next pair from the longest-first work counter and, if
+// its query has at least split_rows rows, runs it as K slices (the slices
+// hand their best keys to wavefront 0, which then walks the CIGAR).  Then, or
+// at once for a shorter pair, the workgroup's wavefronts go their own way
+// (phase 2): each claims pairs from the same counter and runs them alone with
+// the regular tile, the rest of sw_forward_kernel.
+
+// LDS words per slice and parity slot: P, e_cont difference, H(dN), H(dN+1),
+// F(dN+1) (slot s + 1 holds slice s's, read by slices s - 1 and s + 1)
+constexpr int kXw = 8;
+
+// Forward pass of one pair on slice s of K.  Returns the wavefront's best-cell key
+// (H << 40 | (2^20-1-i) << 12 | (4095-d)), 0 if no positive cell.
+// KST: compile-time register of diagonal W-1 in its lane (-1: runtime).
+template <int C, int K, int KST>
+__device__ __forceinline__ uint64_t slice_forward(const SwKArgs &a, int s, int qlen, int tlen,
+                                                  int off, int64_t qo, int64_t to,
+                                                  int64_t tb_base, int *xch)
+{
+    constexpr int NCW = (C + 15) / 16;  // target words feeding the cells
+    constexpr int NA = NCW + 1;         // aligned window words
+    constexpr int NWR = NA + 1;         // raw words loaded per block
+    static_assert(K > 1 && C >= 2 && C <= 9, "slices: K > 1, 2 <= C <= 9 (ghost bases in one window word)");
+    const int gl = lane_id();  // (threadIdx.x & 63 took 256 VGPRs + AGPRs here)
+    const int W = a.W, hb = a.hb, ma = a.ma, mi = a.mi, go = a.go, ge = a.ge;
+    const int sbase = s * 64 * C;      // first diagonal of the slice
+    const int gbase = sbase + gl * C;  // first diagonal of this lane
+    const int glane = s * 64 + gl;     // lane index over the whole band
+    const int lstar = (W - 1) / C;
+    const int kstar = KST >= 0 ? KST : (W - 1) % C;
+    const bool dead_hi = glane >= lstar;
+    const bool guard = (kstar == C - 1) && (glane == lstar);
+    const bool lane_live = glane <= lstar;
+    const int lCe = gbase * ge;
+    const int cH = go - mi, cF = ge - mi;
+    const uint32_t mms = (uint32_t)(ma + mi);
+    const int64_t t_hi = a.t_words - 1, q_hi = a.q_words - 1;
+    const bool has_left = K > 1 && s > 0;
+    const bool has_right = K > 1 && s + 1 < K;
+    const int dN = sbase + 64 * C;  // the next slice's first diagonal
+
+    int Hp[C], Fp[C];
+    uint32_t acc[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) { Hp[k] = 0; Fp[k] = kNeg; acc[k] = 0u; }
+    int bestK = 255, besti = 0;
+    // split state: this slice's prefix through its last diagonal in the
+    // previous row (the ghost cell's E), and the previous row's query base
+    int Pprev = 0;
+    uint32_t qlast = 0u;
+
+    const int nblk = (qlen + 7) >> 3;
+    int64_t e_t = to + (int64_t)(off - hb + gbase);  // target element of cell 0, first row
+    uint32_t wt[NWR];
+    auto load_t = [&](int64_t e) __attribute__((always_inline)) {
+        const int64_t wb = e >> 4;
+#pragma unroll
+        for (int x = 0; x < NWR; ++x) wt[x] = a.tseq[clampi64(wb + x, t_hi)];
+    };
+    if (nblk > 0) load_t(e_t);
+    int fb0 = ((hb + 1 - off) - 1 + 7) >> 3;
+    fb0 = max(fb0, 0);
+    int fb1 = min(qlen, tlen - hb - off);
+    fb1 = fb1 >= 8 ? (fb1 >> 3) : 0;
+    fb1 = min(max(fb1, fb0), nblk);
+    fb0 = min(fb0, fb1);
+
+    // steps before this slice's first row (the slices to its left start first)
+    if constexpr (K > 1) {
+        for (int x = 0; x < s; ++x) __syncthreads();
+    }
+
+    for (int b = 0; b < nblk; ++b) {
+        const int i0 = 8 * b + 1;
+        int bk = INT_MIN;
+        uint32_t ta[NA];
+        {
+            const uint32_t sh = (uint32_t)(e_t & 15) * 2u;
+#pragma unroll
+            for (int x = 0; x < NA; ++x) ta[x] = __builtin_amdgcn_alignbit(wt[x + 1], wt[x], sh);
+        }
+        // lane 63's bases: the ghost cell of row i-1 sits at position C + r - 1
+        uint32_t tw63 = 0u;
+        if constexpr (K > 1) tw63 = (uint32_t)__builtin_amdgcn_readlane((int)ta[0], 63);
+        e_t += 8;
+        if (b + 1 < nblk) load_t(e_t);
+        uint32_t qbits;
+        {
+            const int64_t eq = qo + 8 * b;
+            const int64_t wq = eq >> 4;
+            const uint32_t sh = (uint32_t)(eq & 15) * 2u;
+            const uint32_t w0 = a.qseq[clampi64(wq, q_hi)];
+            const uint32_t w1 = a.qseq[clampi64(wq + 1, q_hi)];
+            qbits = (uint32_t)__builtin_amdgcn_readfirstlane((int)__builtin_amdgcn_alignbit(w1, w0, sh));
+        }
+
+        auto row = [&](auto slow_tag, const int r) __attribute__((always_inline)) {
+            constexpr bool SLOW = decltype(slow_tag)::value;
+            const int i = i0 + r;
+            const int zrow = i * mi;  // H = 0 in this row's domain
+            const int zprev = zrow - mi;
+            const uint32_t qc = (qbits >> (2 * r)) & 3u;
+            const uint32_t qpat = qc * 0x55555555u;
+            // this step's inputs, written by the neighbours in the previous step
+            int Pin = 0, dlin = 0, ghH = zprev, ghF = kNeg;
+            const int par = (i + s) & 1;
+            if constexpr (K > 1) {
+                // slot x + 1 holds slice x's outputs; slots 0 and K + 1 stand
+                // for the missing neighbours of the edge slices (P = 0), so
+                // both reads are unconditional and share one wait
+                const int *in = xch + (par ^ 1) * (K + 2) * kXw;
+                const int vP = in[s * kXw + 0], vD = in[s * kXw + 1];
+                const int vH0 = in[(s + 2) * kXw + 2], vH1 = in[(s + 2) * kXw + 3];
+                const int vF1 = in[(s + 2) * kXw + 4];
+                Pin = __builtin_amdgcn_readfirstlane(vP);
+                dlin = __builtin_amdgcn_readfirstlane(vD);
+                // the ghost cell (i-1, dN) from slice s+1's row i-2 (the band
+                // guard for the last slice)
+                const int rH0 = __builtin_amdgcn_readfirstlane(vH0);
+                const int rH1 = __builtin_amdgcn_readfirstlane(vH1);
+                const int rF1 = __builtin_amdgcn_readfirstlane(vF1);
+                const int ig = i - 1;
+                const int jg = ig + off - hb + dN;  // 1-based column
+                const bool inside = has_right && ig >= 1 && ig <= qlen && jg >= 1 && jg <= tlen;
+                const uint32_t tq = (tw63 >> (2 * (C + r - 1))) & 3u;
+                const uint32_t qq = r > 0 ? ((qbits >> (2 * (r - 1))) & 3u) : qlast;
+                const int hd = rH0 + (tq == qq ? (int)mms : 0);
+                const int fg = max(rH1 - cH, rF1 - cF);
+                const int eg = Pprev - go - (dN - 1) * ge;
+                const int hg = max(max(hd, fg), max(zprev, eg));
+                ghH = inside ? hg : zprev;
+                ghF = inside ? fg : kNeg;
+            }
+            uint32_t z[NCW];
+#pragma unroll
+            for (int x = 0; x < NCW; ++x) {
+                uint32_t y = __builtin_amdgcn_alignbit(ta[x + 1], ta[x], 2 * r) ^ qpat;
+                y |= y >> 1;
+                z[x] = ~y & 0x55555555u;  // 2-bit field k = 1 <=> cell k matches
+            }
+            int lo = 0;
+            uint32_t span = 0;
+            if constexpr (SLOW) {
+                const int dlo = hb + 1 - off - i;
+                const int dhi = tlen + hb - off - i;
+                const int l0 = max(dlo, 0);
+                const int h0 = (i <= qlen) ? min(dhi, W - 1) : -1;
+                lo = l0 - gbase;
+                span = (uint32_t)max(h0 - l0 + 1, 0);
+            }
+            // lane 63's d+1: the ghost cell, or the band guard
+            int hN = from_next_lane(ghH, Hp[0]);
+            int fN = from_next_lane(ghF, Fp[0]);
+            hN = guard ? zprev : hN;
+            fN = guard ? kNeg : fN;
+
+            int Mv[C], Fv[C], Dg[C], Av[C], Gv[C];
+#pragma unroll
+            for (int k = 0; k < C; ++k) {
+                const int uh = (k + 1 < C) ? Hp[(k + 1) % C] : hN;
+                const int uf = (k + 1 < C) ? Fp[(k + 1) % C] : fN;
+                const int av = uh - cH;
+                const int bv = uf - cF;
+                Fv[k] = max(av, bv);
+                acc[k] = insert_sign(acc[k], av - bv);  // f_cont
+                const uint32_t mt = __builtin_amdgcn_ubfe(z[k >> 4], 2 * (k & 15), 2);
+                Dg[k] = mad_u24(mt, mms, Hp[k]);
+                const int m = max(max(Dg[k], Fv[k]), zrow);
+                Mv[k] = m;
+                int A = m + k * ge;
+                if constexpr (SLOW) {
+                    const bool live = (uint32_t)(k - lo) < span;
+                    A = live ? A : kNeg;
+                }
+                Av[k] = A;
+            }
+            int T = Av[0];
+#pragma unroll
+            for (int k = 1; k < C; k += 2)
+                T = (k + 1 < C) ? max(max(T, Av[k]), Av[(k + 1) % C]) : max(T, Av[k]);
+            const int S = seg_scan_max<64>(T + lCe);
+            int X = from_prev_lane_or0(S);
+            if constexpr (K > 1) X = max(X, has_left ? Pin : INT_MIN);
+            const int Xl = X - lCe;
+            Gv[0] = Xl;
+#pragma unroll
+            for (int k = 1; k < C; ++k) Gv[k] = max(Gv[k - 1], Av[k - 1]);
+            const int Glast = Gv[C - 1];
+            const int dlast = Av[C - 1] - Glast;
+            int dprev = from_prev_lane_or0(dlast);
+            if constexpr (K > 1) dprev = (has_left && gl == 0) ? dlin : dprev;
+            int rk = INT_MIN;
+            int Gkm1 = Xl;
+#pragma unroll
+            for (int k = 0; k < C; ++k) {
+                const int Gk = Gv[k];
+                const int E = Gk - (go + (k - 1) * ge);
+                int H = max(Mv[k], E);
+                int Fk = Fv[k];
+                if constexpr (SLOW) {
+                    const bool live = (uint32_t)(k - lo) < span;
+                    H = live ? H : zrow;
+                    Fk = live ? Fk : kNeg;
+                } else {
+                    if (k > kstar) {
+                        H = dead_hi ? zrow : H;
+                        Fk = dead_hi ? kNeg : Fk;
+                    }
+                }
+                acc[k] = insert_sign(acc[k], Dg[k] - H);  // not_diag
+                acc[k] = insert_sign(acc[k], Fk - H);     // not_f
+                acc[k] = insert_sign(acc[k], k ? (Av[k ? k - 1 : 0] - Gkm1) : dprev);  // e_cont
+                rk = max(rk, (int)(((uint32_t)H << 8) | (uint32_t)(31 - k)));
+                Hp[k] = H;
+                Fv[k] = Fk;
+                Gkm1 = Gk;
+            }
+#pragma unroll
+            for (int k = 0; k < C; ++k) Fp[k] = Fv[k];
+            bk = max(bk, rk - (int)(((uint32_t)(i * mi) << 8) - ((uint32_t)(7 - r) << 5)));
+            // finish the row's plane bits and best key here: left free, the
+            // compiler defers them to the end of the block and keeps every
+            // row's temporaries alive meanwhile (256 VGPRs + AGPRs)
+#pragma unroll
+            for (int k = 0; k < C; ++k) asm volatile("" : "+v"(acc[k]));
+            asm volatile("" : "+v"(bk));
+            if constexpr (K > 1) {
+                // this step's outputs, then the step barrier
+                int *out = xch + par * (K + 2) * kXw + (s + 1) * kXw;
+                const int P = max(__builtin_amdgcn_readlane(S, 63), Pin);
+                if (gl == 63) {
+                    out[0] = P;
+                    out[1] = dlast;
+                }
+                Pprev = P;
+                if (gl == 0) {
+                    out[2] = Hp[0];
+                    out[3] = Hp[1];
+                    out[4] = Fp[1];
+                }
+                __syncthreads();
+            }
+        };
+        if (b >= fb0 && b < fb1) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) row(std::false_type{}, r);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) row(std::true_type{}, r);
+        }
+        qlast = (qbits >> 14) & 3u;
+        if (bk > (bestK | 255)) {
+            bestK = bk;
+            besti = i0 + 7 - ((bk >> 5) & 7);
+        }
+        if (lane_live) {
+            uint32_t *dst = a.tb + tb_base + (int64_t)b * a.blkstride + gbase;
+#pragma unroll
+            for (int k = 0; k < C; ++k) dst[k] = acc[k];
+        }
+    }
+    // steps after this slice's last row (the slices to its right end later)
+    if constexpr (K > 1) {
+        for (int x = s + 1; x < K; ++x) __syncthreads();
+    }
+
+    int bH = bestK >> 8;
+    const int bd = gbase + (31 - (bestK & 31));
+    if (!lane_live || besti == 0) bH = 0;
+    uint64_t key = ((uint64_t)(uint32_t)bH << 40) |
+                   ((uint64_t)(0xFFFFFu - (uint32_t)besti) << 12) | (uint64_t)(0xFFFu - (uint32_t)bd);
+    if (bH == 0) key = 0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t lo32 = (uint32_t)__shfl_xor((int)(uint32_t)key, o);
+        const uint32_t hi32 = (uint32_t)__shfl_xor((int)(uint32_t)(key >> 32), o);
+        const uint64_t other = ((uint64_t)hi32 << 32) | lo32;
+        key = other > key ? other : key;
+    }
+    return key;
+}
+
+// Phase 1 of a split launch (see the comment above slice_forward): the
+// workgroup's KSL wavefronts claim one pair; if its query has at least
+// split_rows rows they run it as KSL slices (raised wave priority, so the
+// other waves on their SIMDs fill the slices' stalls instead of delaying
+// them), wavefront 0 walks it and the others go on at once.  A shorter pair
+// goes back to wavefront 0 (the return value; -1 for the other wavefronts or
+// when the counter ran out).  At most one split pair per workgroup: a longer
+// pair claimed later runs on one wavefront.  CW: the regular tile's cells
+// per lane (the plane layout the walk reads).
+template <int CS, int KSL, int KSS, int CW>
+__device__ __forceinline__ int split_phase(const SwKArgs &a)
+{
+    __shared__ int xch[2 * (KSL + 2) * kXw];
+    __shared__ int claim;
+    __shared__ unsigned long long keys[KSL];
+    // wave-uniform (from threadIdx.x the compiler would treat the slice index
+    // as divergent and mask every slice-dependent branch)
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    int carry = -1;
+    {
+        if (threadIdx.x == 0) claim = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(claim);
+        if (item >= a.n_pairs) return -1;
+        const int p = __builtin_amdgcn_readfirstlane(a.order ? a.order[item] : item);
+        const int qlen = __builtin_amdgcn_readfirstlane(a.qlen[p]);
+        if (qlen < a.split_rows) {
+            carry = item;
+            return wave == 0 ? carry : -1;
+        }
+        const int tlen = __builtin_amdgcn_readfirstlane(a.tlen[p]);
+        const int off = __builtin_amdgcn_readfirstlane(a.boff[p]);
+        const int64_t qo = uni64(a.q_off[p]), to = uni64(a.t_off[p]);
+        const int64_t tb_base = uni64(a.tb_off[p]);
+        // row 0 of every slice in both parity slots: H = 0, F = -inf
+        for (int x = threadIdx.x; x < 2 * (KSL + 2) * kXw; x += 64 * KSL)
+            xch[x] = ((x % kXw) == 4) ? kNeg : 0;
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(3);
+        const uint64_t key = slice_forward<CS, KSL, KSS>(a, wave, qlen, tlen, off, qo, to,
+                                                         tb_base, xch);
+        if (lane == 0) keys[wave] = key;
+        __syncthreads();  // every slice's planes and key
+        if (wave == 0) {
+            uint64_t best = 0;
+#pragma unroll
+            for (int w = 0; w < KSL; ++w) best = keys[w] > best ? keys[w] : best;
+            best = (uint64_t)uni64((int64_t)best);
+            int res_s = 0, res_i = 0, res_j = 0;
+            if (best != 0) {
+                res_s = (int)(best >> 40);
+                res_i = (int)(0xFFFFFu - (uint32_t)((best >> 12) & 0xFFFFFu));
+                res_j = res_i + off - a.hb + (int)(0xFFFu - (uint32_t)(best & 0xFFFu));
+            }
+            if (lane == 0) {
+                a.score[p] = res_s;
+                a.qend[p] = res_i;
+                a.tend[p] = res_j;
+            }
+            walk_pair<CW, false>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to, a.tb + tb_base);
+        }
+        __builtin_amdgcn_s_setprio(0);
+    }
+    return -1;
+}
+
 // SV: serving variant (persistent queue fed by the host; staged requests of
 // gmfa_serve_run or the online ring of gmfa_serve_open), a separate
 // instantiation so the offline kernels carry none of its state.
@@ -940,15 +1002,24 @@ __device__ __forceinline__ void scalar_cell(int &Hs, int &Fs, uint32_t &acc, int
 // register 0 of the plane layout) computed on the scalar unit from lane 63's
 // prefix and window, while the vector unit, the bound of this kernel, does
 // 8 cells per lane instead of 9 with 7 dead lanes.
-template <int C, int G, int KS, bool TB, bool NM, bool RK, bool SV = false>
-__global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
+//
+// SPL > 1: a split launch (workgroups of SPL wavefronts; phase 1 runs pairs of
+// at least split_rows query rows as SPL slices of CS cells per lane, KSS the
+// compile-time register of diagonal W-1 of a slice tile), then each
+// wavefront continues below on its own.
+template <int C, int G, int KS, bool TB, bool NM, bool RK, bool SV = false, int SPL = 1,
+          int CS = 1, int KSS = -1>
+__global__ __launch_bounds__(64 * SPL) void sw_forward_kernel(SwKArgs a)
 {
     constexpr bool XS = KS == C;
     static_assert(!XS || (G == 64 && C + 8 <= 32), "scalar-cell tiles: 64 lanes, C <= 24");
     constexpr int NCW = (C + 15) / 16;  // target words feeding the cells
     constexpr int NA = NCW + 1;         // aligned window words
     constexpr int NWR = NA + 1;         // raw words loaded per block
-    const int lane = (int)threadIdx.x;
+    // the lane index: threadIdx.x for one-wavefront workgroups (as lane_id(),
+    // or threadIdx.x & 63, the kernel took 256 VGPRs + AGPRs instead of 202 on
+    // 24 x 64), the mbcnt lane id in split launches
+    const int lane = SPL == 1 ? (int)threadIdx.x : lane_id();
     const int grp = lane / G;
     const int gl = lane % G;
 
@@ -975,6 +1046,11 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         // every class runs side by side)
         if (blockIdx.x == 0 && threadIdx.x == 0 && a.serve_started)
             __hip_atomic_store(a.serve_started, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    int carry = -1;
+    if constexpr (SPL > 1) {
+        static_assert(G == 64 && TB && !NM && !RK && !SV, "split launches: fused walk, 64-lane tile");
+        carry = split_phase<CS, SPL, KSS, C>(a);
     }
     for (;;) {
         int item = 0;
@@ -1037,8 +1113,13 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         } else {
-            if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
-            item = __builtin_amdgcn_readfirstlane(item);
+            if (carry >= 0) {
+                item = carry;  // the pair that ended this wavefront's split phase
+                carry = -1;
+            } else {
+                if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
+                item = __builtin_amdgcn_readfirstlane(item);
+            }
             // serving: one pair per wavefront (G == 64), bounded by what the host
             // has published
             if constexpr (SV) {
@@ -1875,9 +1956,55 @@ static const TileEntry *default_tile(int W)
     return best;
 }
 
+// Split launches: (slice cells per lane CS, slices K, compile-time register
+// of diagonal W-1 in a slice tile) over a regular 64-lane tile (C, KS)
+struct SplitEntry {
+    int CS, K, KSS, C, KS;
+    SwKernel k;
+};
+
+#define GMFA_SPLIT(CS, K, KSS, C, KS) \
+    {CS, K, KSS, C, KS, &sw_forward_kernel<C, 64, KS, true, false, false, false, K, CS, KSS>}
+// W = 1503 (half_band 751) over 24 x 64: four slices of 6 x 64 or eight of 3 x 64
+static const SplitEntry kSplits[] = {
+    GMFA_SPLIT(6, 4, 2, 24, 14),
+    GMFA_SPLIT(3, 8, 2, 24, 14),
+};
+#undef GMFA_SPLIT
+
+static bool split_serves(const SplitEntry &e, int W, const gmfa_sw_tile &t)
+{
+    if (t.lanes_per_pair != 64 || t.cells_per_lane != e.C) return false;
+    if (e.KS >= 0 && (W - 1) % e.C != e.KS) return false;
+    if (e.KSS >= 0 && (W - 1) % e.CS != e.KSS) return false;
+    // exactly K slices, each with a live lane, inside the tile's plane stride
+    if (!((e.K - 1) * 64 * e.CS < W && W <= e.K * 64 * e.CS)) return false;
+    return (W + e.CS - 1) / e.CS * e.CS <= t.lanes_stored * t.cells_per_lane;
+}
+
 }  // namespace gmfa
 
 using namespace gmfa;
+
+extern "C" int gmfa_sw_split_variants(int32_t half_band, int32_t cells_per_lane,
+                                      int32_t *slice_cells, int32_t *slices, int32_t cap)
+{
+    GMFA_REQUIRE(half_band >= 0 && half_band <= 1023, "half_band %d outside [0, 1023]", half_band);
+    gmfa_sw_tile t;
+    if (gmfa_sw_resolve_tile(half_band, cells_per_lane, cells_per_lane > 0 ? 64 : 0, &t) != GMFA_OK)
+        return 0;
+    const int W = 2 * half_band + 1;
+    int n = 0;
+    for (const auto &e : kSplits) {
+        if (!split_serves(e, W, t)) continue;
+        if (n < cap) {
+            if (slice_cells) slice_cells[n] = e.CS;
+            if (slices) slices[n] = e.K;
+        }
+        ++n;
+    }
+    return n;
+}
 
 extern "C" int gmfa_sw_resolve_tile(int32_t half_band, int32_t cells_per_lane,
                                     int32_t lanes_per_pair, gmfa_sw_tile *tile)
@@ -2047,6 +2174,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                  "GMFA_SW_ROWKEYS needs rowkey and rowkey_off");
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, 2 * P.half_band + 1);
     SwKernel k = serve ? t->serve(P.flags) : t->get(P.flags);
+    const SplitEntry *split = nullptr;
 
     SwKArgs a;
     a.qseq = batch->qseq; a.tseq = batch->tseq; a.qmask = batch->qmask; a.tmask = batch->tmask;
@@ -2064,6 +2192,8 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     a.rowkey = out->rowkey; a.rk_off = out->rowkey_off;
     a.counter = slabs ? slabs->counter : out->work_counter;
     a.walk = 0;
+    a.blkstride = tile.lanes_stored * tile.cells_per_lane;
+    a.split_rows = 0;
     a.wc = make_walk_ctx(batch, params, tile, nm);
     a.wo = WalkOut{};
     if (P.flags & GMFA_SW_CIGAR) {
@@ -2090,6 +2220,32 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     int waves = 1;
     if (!serve) {
         waves = std::max(1, std::min(a.n_items, resident_waves(k)));
+        // split launch: the longest pairs over several wavefronts each (needs
+        // the fused walk, no N masks or row keys, a 64-lane tile and a
+        // longest-first order); bands without a split variant run plainly
+        if (!slabs && P.split_rows > 0 && P.flags == (GMFA_SW_TRACEBACK | GMFA_SW_CIGAR) &&
+            batch->order != nullptr) {
+            for (const auto &e : kSplits) {
+                if (!split_serves(e, 2 * P.half_band + 1, tile)) continue;
+                if (P.split_slices > 0 && e.K != P.split_slices) continue;
+                split = &e;
+                break;
+            }
+        }
+        if (split) {
+            k = split->k;
+            int dev = 0, cus = 256, nb = 0;
+            GMFA_CHECK_HIP(hipGetDevice(&dev));
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess)
+                cus = 256;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k, 64 * split->K,
+                                                              0) != hipSuccess ||
+                nb <= 0)
+                nb = 1;
+            a.split_rows = P.split_rows;
+            waves = std::max(1, std::min(cus * nb, n));  // workgroups of split->K wavefronts
+        }
         if (slabs) {
             // one slab per pair group, sized for the longest query (rounded
             // to 256 bytes); as many wavefronts as the slabs fit
@@ -2177,7 +2333,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
         GMFA_CHECK_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k, dim3(waves), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k, dim3(waves), dim3(split ? 64 * split->K : 64), 0, s, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }

@@ -286,14 +286,30 @@ class SwForward:
     walk: Optional["SwTraceback"] = None  # CIGARs when walked inside the forward launch
 
 
-def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile) -> SwParams:
+def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile, split_rows: int = 0,
+                split_slices: int = 0) -> SwParams:
     p = SwParams()
     p.match, p.mismatch, p.gap_open, p.gap_extend = int(ma), int(mi), int(go), int(ge)
     p.half_band = int(half_band)
     p.flags = int(flags)
     p.cells_per_lane = tile.cells_per_lane
     p.lanes_per_pair = tile.lanes_per_pair
+    p.split_rows = int(split_rows)
+    p.split_slices = int(split_slices)
     return p
+
+
+def split_variants(half_band: int, cells_per_lane: int = 0) -> list[tuple[int, int]]:
+    """Compiled split variants for the band: (cells per lane of a slice,
+    slices per pair), over the regular tile of ``cells_per_lane`` (0: the
+    default tile)."""
+    n = lib().gmfa_sw_split_variants(int(half_band), int(cells_per_lane), None, None, 0)
+    if n <= 0:
+        return []
+    c = (ctypes.c_int32 * n)()
+    k = (ctypes.c_int32 * n)()
+    lib().gmfa_sw_split_variants(int(half_band), int(cells_per_lane), c, k, n)
+    return [(int(c[x]), int(k[x])) for x in range(n)]
 
 
 def _cigar_buffers(inp: SwInputs, ws: dict):
@@ -317,9 +333,14 @@ def _cigar_buffers(inp: SwInputs, ws: dict):
 def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int,
                traceback: bool = True, rowkeys: bool = False,
                order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
-               workspace: Optional[dict] = None, cigar: bool = False) -> SwForward:
+               workspace: Optional[dict] = None, cigar: bool = False, split_rows: int = 0,
+               split_slices: int = 0) -> SwForward:
     """Forward pass.  ``cigar=True`` also walks every pair inside the same
-    launch (GMFA_SW_CIGAR) and returns the CIGARs in ``.walk``."""
+    launch (GMFA_SW_CIGAR) and returns the CIGARs in ``.walk``.
+    ``split_rows > 0`` (with ``cigar``, ``order`` and no N masks): pairs of at
+    least that many query rows run on several wavefronts each, a slice of the
+    band per wavefront (``split_slices`` of them, 0 = the first compiled
+    count); the outputs are those of the plain launch."""
     dev = inp.device
     n = inp.n_pairs
     tile = tile or batch_tile(half_band, n, dev)
@@ -370,10 +391,11 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
         out.cigar_off, out.cigar = ptr(walk.cigar_off), ptr(walk.cigar)
         out.n_cigar, out.read_start = ptr(walk.n_cigar), ptr(walk.read_start)
         out.ref_start, out.counts = ptr(walk.ref_start), ptr(walk.counts)
-    params = make_params(ma, mi, go, ge, half_band, flags, tile)
+    params = make_params(ma, mi, go, ge, half_band, flags, tile, split_rows, split_slices)
     batch = inp.batch_struct(order)
     check(lib().gmfa_sw_forward(ctypes.byref(batch), ctypes.byref(params), ctypes.byref(out),
                                 stream_of(dev)), "gmfa_sw_forward")
+    params.split_rows = params.split_slices = 0
     params.flags = int(flags) & ~_lib.SW_CIGAR  # a later standalone walk uses the same params
     return SwForward(score=score, query_end=qend, target_end=tend, tb=tb, tb_off=tb_off,
                      rowkey=rk, rowkey_off_host=rk_off_host, tile=tile, params=params, walk=walk)

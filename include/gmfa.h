@@ -55,7 +55,7 @@
 extern "C" {
 #endif
 
-#define GMFA_ABI_VERSION 1
+#define GMFA_ABI_VERSION 2
 
 #define GMFA_OK 0
 #define GMFA_E_INVALID -1   /* bad argument: maps to ValueError */
@@ -86,6 +86,16 @@ typedef struct gmfa_sw_params {
     uint32_t flags;      /* GMFA_SW_* */
     int32_t cells_per_lane; /* 0 = library default for the band; else a compiled tile */
     int32_t lanes_per_pair; /* 0 = library default; else 16, 32 or 64 */
+    /* Split launch (gmfa_sw_forward with exactly GMFA_SW_TRACEBACK |
+     * GMFA_SW_CIGAR, a 64-lane tile and batch->order set): split_rows > 0 runs
+     * every pair whose query has at least split_rows rows on a workgroup of
+     * several wavefronts, each a slice of the band, and the rest one pair per
+     * wavefront, in one launch; outputs and planes are those of the plain
+     * launch.  split_slices: 0 = the first compiled slice count for the band,
+     * else that count.  Bands without a compiled split variant run the plain
+     * launch.  0 / 0: the plain launch. */
+    int32_t split_rows;
+    int32_t split_slices;
 } gmfa_sw_params;
 
 /* A batch of pairs whose sequences live in 2-bit packed streams
@@ -149,6 +159,13 @@ int gmfa_sw_resolve_tile(int32_t half_band, int32_t cells_per_lane, int32_t lane
 /* Compiled (cells_per_lane, lanes_per_pair) tiles; returns how many exist and
  * fills up to cap entries. */
 int gmfa_sw_list_tiles(int32_t *cells_per_lane, int32_t *lanes_per_pair, int32_t cap);
+
+/* Slice counts of the compiled split variants for a band (half_band, with the
+ * regular tile of cells_per_lane cells per lane, 0 = default): returns how
+ * many exist and fills up to cap entries (cells per lane of a slice, slices
+ * per pair). */
+int gmfa_sw_split_variants(int32_t half_band, int32_t cells_per_lane, int32_t *slice_cells,
+                           int32_t *slices, int32_t cap);
 
 /* Bytes of traceback planes one pair of query length qlen needs with tile. */
 int64_t gmfa_sw_tb_bytes(int32_t qlen, const gmfa_sw_tile *tile);

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     L = _lib.lib()
     for name in declared_symbols():
         assert hasattr(L, name), name
-    assert L.gmfa_abi_version() == 1
+    assert L.gmfa_abi_version() == 2
 
 
 def test_tile_resolution_and_tb_bytes():

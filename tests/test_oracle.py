@@ -300,3 +300,36 @@ def test_backtrack_hand_case():
     assert [(m.tolist(), s) for m, s in chains] == [([0, 1, 2], 12.0), ([3], -2.0)]
     chains = O().chain_backtrack([5.0, 9.0, 12.0, 7.0], [-1, 0, 1, 1], 1, 0.0)
     assert [(m.tolist(), s) for m, s in chains] == [([0, 1, 2], 12.0)]
+
+
+def test_fullsize_digest_matches_the_oracle_on_a_sample():
+    """The committed every-pair digest (tests/golden/fullsize_digest.npz) is the
+    oracle's: the generators reproduce its inputs, and a stratified sample of
+    pairs / reads recomputed here gives the same entries."""
+    import os
+
+    from graphmambaformer_amd import workloads as WL
+    from tests.golden import make_fullsize_digest as D
+
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "fullsize_digest.npz"))
+    for name in ("sw_long", "sw_noisy"):
+        w = getattr(WL, name)()
+        assert D.sw_input_digest(w) == int(d[f"{name}_input"][0]), name
+        cells = w.cells()
+        srt = np.argsort(cells, kind="stable")
+        pick = srt[np.linspace(0, len(srt) // 2, 6).astype(int)]  # short to medium pairs
+        qs = [w.as_ref_codes(int(k))[0] for k in pick]
+        ts = [w.as_ref_codes(int(k))[1] for k in pick]
+        exp = O().sw_align_pairs(qs, ts, np.asarray(w.band_offset)[pick].astype(np.int64),
+                               w.half_band, w.match, w.mismatch, w.gap_open, w.gap_extend,
+                               threads=4)
+        for e, k in zip(exp, pick):
+            assert int(d[f"{name}_score"][k]) == e["score"], (name, k)
+            assert int(d[f"{name}_qend"][k]) == e["read_end"], (name, k)
+            assert int(d[f"{name}_rs"][k]) == e["read_start"], (name, k)
+            assert int(d[f"{name}_cigar"][k]) == D.cigar_digest(D.cigar_runs(e["cigar"])), (name, k)
+    wc = WL.chains()
+    assert D.chains_input_digest(wc) == int(d["chains_input"][0])
+    nk, best, dig = D.chains_reference(wc, wc.max_lookback, wc.max_gap, reads=40)
+    assert np.array_equal(nk[:40], d["chains_nkept"][:40])
+    assert np.array_equal(dig[:40], d["chains_digest"][:40])
