@@ -665,7 +665,7 @@ __device__ __forceinline__ uint64_t slice_forward(const SwKArgs &a, int s, int q
     constexpr int NCW = (C + 15) / 16;  // target words feeding the cells
     constexpr int NA = NCW + 1;         // aligned window words
     constexpr int NWR = NA + 1;         // raw words loaded per block
-    static_assert(K > 1 && C >= 2 && C <= 9, "slices: K > 1, 2 <= C <= 9 (ghost bases in one window word)");
+    static_assert(K > 1 && C >= 2 && C <= 24, "slices: K > 1, 2 <= C <= 24 (ghost bases in two window words)");
     const int gl = lane_id();  // (threadIdx.x & 63 took 256 VGPRs + AGPRs here)
     const int W = a.W, hb = a.hb, ma = a.ma, mi = a.mi, go = a.go, ge = a.ge;
     const int sbase = s * 64 * C;      // first diagonal of the slice
@@ -725,8 +725,10 @@ __device__ __forceinline__ uint64_t slice_forward(const SwKArgs &a, int s, int q
             for (int x = 0; x < NA; ++x) ta[x] = __builtin_amdgcn_alignbit(wt[x + 1], wt[x], sh);
         }
         // lane 63's bases: the ghost cell of row i-1 sits at position C + r - 1
-        uint32_t tw63 = 0u;
-        if constexpr (K > 1) tw63 = (uint32_t)__builtin_amdgcn_readlane((int)ta[0], 63);
+        uint64_t tw63 = 0u;
+        if constexpr (K > 1)
+            tw63 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ta[1], 63) << 32) |
+                   (uint32_t)__builtin_amdgcn_readlane((int)ta[0], 63);
         e_t += 8;
         if (b + 1 < nblk) load_t(e_t);
         uint32_t qbits;
@@ -767,7 +769,7 @@ __device__ __forceinline__ uint64_t slice_forward(const SwKArgs &a, int s, int q
                 const int ig = i - 1;
                 const int jg = ig + off - hb + dN;  // 1-based column
                 const bool inside = has_right && ig >= 1 && ig <= qlen && jg >= 1 && jg <= tlen;
-                const uint32_t tq = (tw63 >> (2 * (C + r - 1))) & 3u;
+                const uint32_t tq = (uint32_t)(tw63 >> (2 * (C + r - 1))) & 3u;
                 const uint32_t qq = r > 0 ? ((qbits >> (2 * (r - 1))) & 3u) : qlast;
                 const int hd = rH0 + (tq == qq ? (int)mms : 0);
                 const int fg = max(rH1 - cH, rF1 - cF);
@@ -1965,10 +1967,12 @@ struct SplitEntry {
 
 #define GMFA_SPLIT(CS, K, KSS, C, KS) \
     {CS, K, KSS, C, KS, &sw_forward_kernel<C, 64, KS, true, false, false, false, K, CS, KSS>}
-// W = 1503 (half_band 751) over 24 x 64: four slices of 6 x 64 or eight of 3 x 64
+// W = 1503 (half_band 751) over 24 x 64: two slices of 12 x 64, three of
+// 8 x 64 or four of 6 x 64
 static const SplitEntry kSplits[] = {
+    GMFA_SPLIT(12, 2, 2, 24, 14),
+    GMFA_SPLIT(8, 3, 6, 24, 14),
     GMFA_SPLIT(6, 4, 2, 24, 14),
-    GMFA_SPLIT(3, 8, 2, 24, 14),
 };
 #undef GMFA_SPLIT
 

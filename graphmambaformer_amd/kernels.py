@@ -460,7 +460,8 @@ class SwGraph:
 
     def __init__(self, inp: SwInputs, scoring: tuple, half_band: int,
                  order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
-                 workspace: Optional[dict] = None, cigar: bool = True):
+                 workspace: Optional[dict] = None, cigar: bool = True, split_rows: int = 0,
+                 split_slices: int = 0):
         dev = inp.device
         self.tile = tile or batch_tile(half_band, inp.n_pairs, dev)
         self.ws = workspace if workspace is not None else make_workspace(inp, self.tile,
@@ -468,7 +469,8 @@ class SwGraph:
 
         def step():
             return sw_forward(inp, *scoring, half_band, traceback=True, order=order,
-                              tile=self.tile, workspace=self.ws, cigar=cigar)
+                              tile=self.tile, workspace=self.ws, cigar=cigar,
+                              split_rows=split_rows, split_slices=split_slices)
 
         # one eager pass on a side stream first (module load, allocator warm-up)
         side = torch.cuda.Stream(device=dev)
@@ -811,6 +813,51 @@ def cigar_bam(cigar: torch.Tensor, cigar_off: torch.Tensor, n_cigar: torch.Tenso
                                *[ptr(t) for t in args], ptr(out), ptr(d_off), ptr(out_n),
                                ptr(flag), stream_of(dev)), "gmfa_cigar_bam")
     return out, d_off, out_n[:n], flag[:n]
+
+
+def autotune_launch(inp: SwInputs, half_band: int, scoring: tuple, order: torch.Tensor,
+                    tile: Optional[SwTile] = None, workspace: Optional[dict] = None,
+                    tops=(16, 32, 64, 128, 256), reps: int = 2, margin: float = 0.01):
+    """Choose between the plain launch and the split launches (the longest
+    pairs over several wavefronts) for this batch: times the plain launch and
+    every compiled split variant with the ``tops`` longest queries split
+    (split_rows = the length of the k-th longest query), on the whole batch
+    itself (the tail that splitting shortens is a property of the whole
+    batch, not of a sample), each the best of ``reps`` launches.  Keeps the
+    plain launch unless a split one is faster by more than ``margin``.
+    Returns ((split_rows, split_slices), {label: ms})."""
+    dev = inp.device
+    tile = tile or batch_tile(half_band, inp.n_pairs, dev)
+    ws = workspace if workspace is not None else make_workspace(inp, tile, cigar=True)
+    s = torch.cuda.current_stream(dev)
+
+    def best_ms(rows, slices):
+        sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
+                   workspace=ws, cigar=True, split_rows=rows, split_slices=slices)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        for e0, e1 in ev:
+            e0.record(s)
+            sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
+                       workspace=ws, cigar=True, split_rows=rows, split_slices=slices)
+            e1.record(s)
+        torch.cuda.synchronize(dev)
+        return min(e0.elapsed_time(e1) for e0, e1 in ev)
+
+    timings = {"plain": round(best_ms(0, 0), 4)}
+    best, best_t = (0, 0), timings["plain"]
+    variants = split_variants(half_band, tile.cells_per_lane) if not inp.has_n else []
+    ql = np.sort(np.asarray(inp.qlen_host))[::-1]
+    for _, slices in variants:
+        for k in tops:
+            if k > len(ql):
+                continue
+            rows = int(ql[k - 1])
+            ms = round(best_ms(rows, slices), 4)
+            timings[f"split{slices}_top{k}_rows{rows}"] = ms
+            if ms < best_t * (1.0 - margin) and ms < timings["plain"] * (1.0 - margin):
+                best, best_t = (rows, slices), ms
+    return best, timings
 
 
 def autotune_tile(inp: SwInputs, half_band: int, scoring: tuple, reps: int = 3,

@@ -23,6 +23,12 @@
  *   oracle_chain_backtrack graphmambaformer/alignment/chaining.py:530-576
  *                         (AffineChainer._backtrack: argsort(-f, stable),
  *                          claimed-walk, partial-chain score)
+ *   oracle_sw_align_batch independent pairs (one reference batch each),
+ *                         forward + traceback, fanned out over host threads
+ *                         as the reference's parallel_map does
+ *                         (pipeline.py:355-363, accel/parallel.py:60-103):
+ *                         the CPU comparator, with no per-pair host-language
+ *                         overhead
  *
  * Number model.  The reference computes SW in float32 with -1e30 standing for
  * minus infinity (extension.py:56).  Every finite value is an exact integer of
@@ -39,6 +45,7 @@
  * KAT-CH1/KAT-CH2 and the O(n^2) brute force (test_alignment_stages.py:372-416,
  * 502-517).
  */
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -368,4 +375,81 @@ int64_t oracle_chain_backtrack(int64_t n, const double *f, const int64_t *parent
     }
     free(order); free(tmp); free(claimed);
     return nc;
+}
+
+/* ------------------------------------------------------------------------
+ * oracle_sw_align_batch: n independent pairs (pair p: query codes
+ * q[q_off[p] .. + q_len[p]), target codes t[t_off[p] .. + t_len[p]), band
+ * offset band_off[p]), each its own batch of one (no x-drop), forward +
+ * traceback, on `threads` POSIX threads taking pairs from a shared counter.
+ * Outputs per pair: score, 1-based ends, 0-based starts, number of CIGAR
+ * columns.  Returns 0, or -1 if an allocation failed.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int64_t n;
+    const int16_t *q, *t;
+    const int64_t *q_off, *q_len, *t_off, *t_len, *band_off;
+    int32_t hb, ma, mi, go, ge;
+    int32_t *score;
+    int64_t *qend, *tend, *rs, *ts, *n_cols;
+    int64_t next;  /* shared work counter */
+    int failed;
+} align_batch;
+
+static void *align_worker(void *arg)
+{
+    align_batch *b = (align_batch *)arg;
+    const int64_t W = 2 * (int64_t)b->hb + 1;
+    for (;;) {
+        const int64_t p = __atomic_fetch_add(&b->next, 1, __ATOMIC_RELAXED);
+        if (p >= b->n) break;
+        const int64_t M = b->q_len[p], N = b->t_len[p];
+        const int16_t *q = b->q + b->q_off[p], *t = b->t + b->t_off[p];
+        const size_t cells = (size_t)(M + 1) * (size_t)W;
+        int32_t *H = malloc(sizeof(int32_t) * cells);
+        int32_t *E = malloc(sizeof(int32_t) * cells);
+        int32_t *F = malloc(sizeof(int32_t) * cells);
+        uint8_t *ops = malloc((size_t)(M + N + 1));
+        int64_t qe = 0, te = 0, rows = 0, rs = 0, ts = 0;
+        int32_t sc = 0;
+        if (!H || !E || !F || !ops ||
+            oracle_sw_banded(1, M, N, q, t, &M, &N, &b->band_off[p], b->hb, b->ma, b->mi, b->go,
+                             b->ge, -1, &sc, &qe, &te, H, E, F, &rows) != 0) {
+            __atomic_store_n(&b->failed, 1, __ATOMIC_RELAXED);
+        } else {
+            b->n_cols[p] = oracle_sw_traceback(H, E, F, M, b->hb, b->band_off[p], q, t, qe, te,
+                                               b->ma, b->mi, b->go, b->ge, ops, &rs, &ts);
+            b->score[p] = sc;
+            b->qend[p] = qe;
+            b->tend[p] = te;
+            b->rs[p] = rs;
+            b->ts[p] = ts;
+        }
+        free(H);
+        free(E);
+        free(F);
+        free(ops);
+    }
+    return NULL;
+}
+
+int oracle_sw_align_batch(int64_t n, const int16_t *q, const int64_t *q_off, const int64_t *q_len,
+                          const int16_t *t, const int64_t *t_off, const int64_t *t_len,
+                          const int64_t *band_off, int32_t hb, int32_t ma, int32_t mi, int32_t go,
+                          int32_t ge, int32_t threads, int32_t *score, int64_t *qend,
+                          int64_t *tend, int64_t *rs, int64_t *ts, int64_t *n_cols)
+{
+    align_batch b = {n, q, t, q_off, q_len, t_off, t_len, band_off, hb, ma, mi, go, ge,
+                     score, qend, tend, rs, ts, n_cols, 0, 0};
+    if (threads < 1) threads = 1;
+    pthread_t *tid = malloc(sizeof(pthread_t) * (size_t)threads);
+    if (!tid) return -1;
+    int started = 0;
+    for (int k = 0; k < threads; ++k)
+        if (pthread_create(&tid[k], NULL, align_worker, &b) == 0) ++started;
+    if (started == 0) align_worker(&b);
+    for (int k = 0; k < threads; ++k)
+        if (k < started) pthread_join(tid[k], NULL);
+    free(tid);
+    return b.failed ? -1 : 0;
 }

@@ -56,6 +56,9 @@ def lib():
         L.oracle_chain_dp_i32.restype = None
         L.oracle_chain_backtrack.argtypes = [i64, P, P, i64, ctypes.c_double, P, P, P]
         L.oracle_chain_backtrack.restype = i64
+        L.oracle_sw_align_batch.argtypes = [i64, P, P, P, P, P, P, P, i32, i32, i32, i32, i32,
+                                            i32, P, P, P, P, P, P]
+        L.oracle_sw_align_batch.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -161,6 +164,38 @@ def sw_align_pairs(queries, targets, offs, hb, ma, mi, go, ge, threads=1):
         return [one(k) for k in range(len(queries))]
     with ThreadPoolExecutor(threads) as ex:
         return list(ex.map(one, range(len(queries))))
+
+
+def sw_align_batch(queries, targets, offs, hb, ma, mi, go, ge, threads=1):
+    """sw_align_pairs in one C call (oracle_sw_align_batch): independent pairs,
+    forward + traceback on ``threads`` POSIX threads, no per-pair Python work.
+    Returns a dict of arrays: score, read_end, ref_end, read_start, ref_start,
+    n_cols (CIGAR columns)."""
+    n = len(queries)
+    ql = np.array([len(x) for x in queries], np.int64)
+    tl = np.array([len(x) for x in targets], np.int64)
+    qo = np.zeros(n, np.int64)
+    to = np.zeros(n, np.int64)
+    if n:
+        qo[1:] = np.cumsum(ql)[:-1]
+        to[1:] = np.cumsum(tl)[:-1]
+    qc = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int16) for x in queries])
+                              if n else np.zeros(0, np.int16))
+    tc = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int16) for x in targets])
+                              if n else np.zeros(0, np.int16))
+    bo = np.ascontiguousarray(offs, dtype=np.int64).reshape(n)
+    out = {k: np.zeros(n, np.int64) for k in ("read_end", "ref_end", "read_start", "ref_start",
+                                               "n_cols")}
+    score = np.zeros(n, np.int32)
+    rc = lib().oracle_sw_align_batch(n, _p(qc), _p(qo), _p(ql), _p(tc), _p(to), _p(tl), _p(bo),
+                                     hb, ma, mi, go, ge, int(threads), _p(score),
+                                     _p(out["read_end"]), _p(out["ref_end"]),
+                                     _p(out["read_start"]), _p(out["ref_start"]),
+                                     _p(out["n_cols"]))
+    if rc != 0:
+        raise MemoryError("oracle_sw_align_batch allocation failed")
+    out["score"] = score
+    return out
 
 
 def sw_score_pairs(queries, targets, offs, hb, ma, mi, go, ge, threads=1):

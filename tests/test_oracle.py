@@ -333,3 +333,29 @@ def test_fullsize_digest_matches_the_oracle_on_a_sample():
     nk, best, dig = D.chains_reference(wc, wc.max_lookback, wc.max_gap, reads=40)
     assert np.array_equal(nk[:40], d["chains_nkept"][:40])
     assert np.array_equal(dig[:40], d["chains_digest"][:40])
+
+
+def test_align_batch_matches_per_pair_driver():
+    """oracle_sw_align_batch (the C-threaded CPU comparator) gives the same
+    results as the per-pair driver sw_align_pairs."""
+    rng = np.random.default_rng(11)
+    qs, ts, offs = [], [], []
+    for _ in range(40):
+        n = int(rng.integers(1, 400))
+        q = rng.integers(1, 5, size=n).astype(np.int16)
+        t = np.concatenate([rng.integers(1, 5, size=int(rng.integers(0, 30))).astype(np.int16), q,
+                            rng.integers(1, 5, size=int(rng.integers(0, 30))).astype(np.int16)])
+        t[rng.integers(0, len(t), size=max(1, len(t) // 20))] = rng.integers(1, 5)
+        qs.append(q)
+        ts.append(t)
+        offs.append(int(rng.integers(-5, 40)))
+    for hb in (8, 32, 100):
+        exp = O().sw_align_pairs(qs, ts, offs, hb, 2, 4, 4, 2)
+        got = O().sw_align_batch(qs, ts, offs, hb, 2, 4, 4, 2, threads=4)
+        for p, e in enumerate(exp):
+            assert int(got["score"][p]) == e["score"], (hb, p)
+            assert int(got["read_end"][p]) == e["read_end"], (hb, p)
+            assert int(got["ref_end"][p]) == e["ref_end"], (hb, p)
+            assert int(got["read_start"][p]) == e["read_start"], (hb, p)
+            assert int(got["ref_start"][p]) == e["ref_start"], (hb, p)
+            assert int(got["n_cols"][p]) == sum(n for _, n in e["cigar"]), (hb, p)
