@@ -3,15 +3,19 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-extras]
 
-A step is one pass of the hot path over the whole long-read batch (16,384
-pairs, lengths lognormal median 12 kb clipped 5-25 kb, 0.5% errors, band
-half-width 256, scoring 2/-4/-4/-2): the forward kernel writes scores, ends
-and 4-bit traceback planes and walks every pair into a CIGAR.  Inputs are
-resident in HBM before the timed region.  ``value`` is GCUPS over in-band
-cells.  Beside it the line carries the per-pair-launch GPU form, the CPU
-oracle on the host cores (bounded stratified sample), the short-read batch,
-the noisy wide-band batch, the chaining DP, and the roofline of the dominant
-kernel.
+A step is one pass of the hot path over the whole noisy batch (8,192 pairs,
+lengths lognormal median 8 kb clipped 0.5-100 kb, 6% errors with
+homopolymer-weighted indels, band half-width 751, scoring 2/-4/-4/-2): one
+launch of the forward kernel writes scores, ends and 4-bit traceback planes
+and walks every pair into a CIGAR, longest pair first, with the tile each
+length bin's autotune picks and the launch (plain, longest pairs at raised
+priority, or split over several wavefronts) the whole-batch autotune picks.
+Inputs are resident in HBM before the timed region.  ``value`` is GCUPS over
+in-band cells; ``roofline`` is the int32 VALU roof of the forward kernel (the
+binding one), ``roofline_hbm`` the HBM view.  Beside it the line carries the
+per-pair-launch GPU form, the CPU oracle on every host core (bounded
+stratified sample, one C call), the HIP-graph replay, the long-read and
+short-read batches, serving latency, chaining, seeding and WFA.
 """
 
 from __future__ import annotations
@@ -35,6 +39,8 @@ METRIC = "banded affine-SW GCUPS batched vs per-pair; p50/p99 latency at fixed Q
 HBM_PEAK_GBS = 8000.0
 INT32_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 39.3 T int32 VALU lane-ops/s
 OPS_PER_CELL_TB = 20  # int32 ops per cell with traceback (score/end 14 + traceback 6)
+CHAIN_OPS_PER_EVAL = 20  # int32 ops per chaining pair evaluation
+CHAIN_BYTES_PER_ANCHOR = 24  # ref_end 8 + read_end 4 + weight 4 read, f 4 + parent 4 written
 TB_BYTES_PER_CELL = 0.5
 
 
@@ -42,20 +48,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(cells: int, fused: bool, tile=None):
+def measured_traffic(name: str, cells: int, tile=None, launch=None):
     """HBM bytes per forward launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs (calibrated against a plain 1 GiB copy), if they were
-    taken on this exact batch, tile and walk mode; else None."""
+    WRITE_SIZE runs (profiles/sw_forward_<name>_traffic.json, calibrated
+    against a plain 1 GiB copy), if they were taken on this exact batch, tile
+    and launch; else None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "sw_forward_sw_long_traffic.json")
+                        f"sw_forward_{name}_traffic.json")
     try:
         with open(path) as fh:
             rec = json.load(fh)
     except (OSError, ValueError):
         return None
-    if rec.get("cells") != cells or not fused:
+    if rec.get("cells") != cells:
         return None
     if tile is not None and rec.get("tile") != [tile.cells_per_lane, tile.lanes_per_pair]:
+        return None
+    if launch is not None and rec.get("launch", launch) != launch:
         return None
     return rec.get("traffic_bytes_per_launch")
 
@@ -87,8 +96,9 @@ class BatchRun:
     order, tile and every buffer preallocated; ``step()`` is one forward
     launch (+ the walk, fused or as a second launch)."""
 
-    def __init__(self, torch, K, w, dev, fused=True, tile=None):
+    def __init__(self, torch, K, w, dev, fused=True, tile=None, launch=None):
         self.w, self.K, self.fused = w, K, fused
+        self.launch = dict(launch or {})
         self.cells = w.cells()
         self.total_cells = int(self.cells.sum())
         self.inp = K.upload_pairs(w.queries, w.targets, w.band_offset, dev, alphabet="2bit")
@@ -100,7 +110,8 @@ class BatchRun:
     def forward(self):
         K, w = self.K, self.w
         return K.sw_forward(self.inp, *scoring(w), w.half_band, traceback=True, order=self.order,
-                            tile=self.tile, workspace=self.ws, cigar=self.fused)
+                            tile=self.tile, workspace=self.ws, cigar=self.fused,
+                            **(self.launch if self.fused else {}))
 
     def walk(self, fwd):
         return fwd.walk if self.fused else self.K.sw_traceback(self.inp, fwd, workspace=self.ws)
@@ -136,7 +147,9 @@ def per_pair_gcups(torch, K, w, idx, dev):
 
 
 def cpu_gcups(w, idx, threads):
-    """The C oracle (forward + traceback, one pair per thread task)."""
+    """The C oracle (forward + traceback of independent pairs, one C call:
+    POSIX threads take pairs from a shared counter, as the reference's
+    parallel_map fans reads over host cores)."""
     from oracle import oracle as O
 
     if not os.path.exists(O._LIB_PATH):
@@ -144,7 +157,7 @@ def cpu_gcups(w, idx, threads):
     qs = [w.as_ref_codes(int(k))[0] for k in idx]
     ts = [w.as_ref_codes(int(k))[1] for k in idx]
     t1 = time.perf_counter()
-    O.sw_align_pairs(qs, ts, w.band_offset[idx], w.half_band, *scoring(w), threads=threads)
+    O.sw_align_batch(qs, ts, w.band_offset[idx], w.half_band, *scoring(w), threads=threads)
     s = time.perf_counter() - t1
     return float(w.cells()[idx].sum()) / s / 1e9, s
 
@@ -471,6 +484,106 @@ def wfa_extra(torch, K, WL, dev):
     return res
 
 
+def long_read_extra(torch, K, WL, dev, args, threads):
+    """The long-read batch (16,384 pairs, 5-25 kb, band 513): batched GCUPS on
+    the autotuned tile, its HIP-graph replay, the x-drop extension path, the
+    rate with host buffers in and out, the per-pair form and the CPU oracle."""
+    t0 = time.time()
+    w = WL.sw_long()
+    log(f"[bench] sw_long generated: {w.n_pairs} pairs, {w.cells().sum():.3e} cells, "
+        f"{time.time() - t0:.1f}s")
+    # tile autotuning on every 4th pair (by cell count): 4,096 pairs keep about
+    # 4 wavefronts per SIMD resident, as the batch does
+    sub_t = WL.subsample(w, np.argsort(w.cells(), kind="stable")[::4])
+    inp_t = K.upload_pairs(sub_t.queries, sub_t.targets, sub_t.band_offset, dev, alphabet="2bit")
+    tuned, tune_ms = K.autotune_tile(inp_t, w.half_band, scoring(w),
+                                     order=torch.from_numpy(lpt_order(sub_t.cells())).to(dev))
+    del inp_t
+    torch.cuda.empty_cache()
+    run = BatchRun(torch, K, w, dev, True, tile=tuned)
+    total_cells, cells, tile = run.total_cells, run.cells, run.tile
+    run.step()
+    ms = time_events(torch, run.step, 5)
+    gcups = total_cells / (ms / 1e3) / 1e9
+    gr = K.SwGraph(run.inp, scoring(w), w.half_band, order=run.order, tile=tile,
+                   workspace=run.ws, cigar=True)
+    gms = time_events(torch, gr.replay, 5)
+    del gr
+    valu_tops = total_cells * OPS_PER_CELL_TB / (ms / 1e3) / 1e12
+    out = {
+        "batched_gcups": round(gcups, 2),
+        "batched_ms": round(ms, 3),
+        "graph_gcups": round(total_cells / (gms / 1e3) / 1e9, 2),
+        "graph_ms": round(gms, 3),
+        "valu_frac": round(valu_tops / INT32_PEAK_TOPS, 4),
+        "traffic": measured_traffic("sw_long", total_cells, tile),
+        "tile": [tile.cells_per_lane, tile.lanes_per_pair],
+        "autotune_ms": {f"{c}x{g}": v for (c, g), v in tune_ms.items()},
+        "workload": "16384 pairs, lognormal(12 kb, 0.4) clipped 5-25 kb, 0.5% errors, "
+                    "half_band 256, seed 2, fused walk",
+    }
+    log(f"[bench] sw_long: {ms:.1f} ms -> {gcups:.1f} GCUPS, graph {gms:.1f} ms")
+
+    # host buffers in and out: upload + device packing, the step, the CIGAR
+    # runs back to the host
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    inp_h = K.upload_pairs(w.queries, w.targets, w.band_offset, dev, alphabet="2bit")
+    fwd_h = K.sw_forward(inp_h, *scoring(w), w.half_band, traceback=True, order=run.order,
+                         tile=tile, workspace=run.ws, cigar=True)
+    tb_h = fwd_h.walk
+    nc_h = tb_h.n_cigar.to(torch.int64)
+    tot = int(nc_h.sum().item())
+    owner = torch.repeat_interleave(torch.arange(w.n_pairs, device=dev), nc_h)
+    within = torch.arange(tot, device=dev) - (torch.cumsum(nc_h, 0) - nc_h)[owner]
+    runs_h = tb_h.cigar[tb_h.cigar_off[owner] + within].cpu()
+    _ = (fwd_h.score.cpu(), fwd_h.query_end.cpu(), fwd_h.target_end.cpu(), nc_h.cpu())
+    host_s = time.perf_counter() - t1
+    out["host_inclusive"] = {
+        "gcups": round(total_cells / host_s / 1e9, 2),
+        "s": round(host_s, 3),
+        "cigar_runs": int(runs_h.numel()),
+        "note": "host arrays in: concatenation, one upload per stream, 2-bit packing on the "
+                "device, forward with fused walk, scores/ends/CIGAR runs copied back",
+    }
+    del inp_h, fwd_h, tb_h, runs_h
+
+    # the x-drop extension path (drop-in form): forward with planes and per-row
+    # keys, the batch-wide x-drop of each group of 8 pairs, the walk from the
+    # x-drop ends
+    gofs = np.arange(0, w.n_pairs + 1, 8)
+    xd = {}
+
+    def xdrop_step():
+        f = K.sw_forward(run.inp, *scoring(w), w.half_band, traceback=True, rowkeys=True,
+                         order=run.order, tile=tile, workspace=run.ws)
+        sc_, qe_, te_ = K.sw_xdrop(f, run.inp, gofs, 600.0, w.half_band)
+        xd["tb"] = K.sw_traceback(run.inp, f, sc_, qe_, te_, workspace=run.ws)
+        return f
+
+    xdrop_step()
+    xms = time_events(torch, xdrop_step, 3)
+    out["xdrop"] = {
+        "gcups": round(total_cells / (xms * 1e-3) / 1e9, 2),
+        "ms": round(xms, 3),
+        "workload": "forward with traceback planes + per-row keys (longest first), batch "
+                    "x-drop 600 over groups of 8 pairs, standalone walk from the x-drop ends",
+    }
+    del xd, run
+    torch.cuda.empty_cache()
+    sub = np.arange(0, w.n_pairs, 256)[:64]
+    out["per_pair_gcups"] = round(per_pair_gcups(torch, K, w, sub, dev), 4)
+    srt = np.argsort(cells, kind="stable")
+    subc = srt[::args.cpu_sample_stride]
+    cg, cs = cpu_gcups(w, subc, threads)
+    out["cpu_gcups"] = round(cg, 4)
+    out["cpu_threads"] = threads
+    out["cpu_sample"] = (f"{len(subc)} pairs (every {args.cpu_sample_stride}th by cell count, "
+                         f"{int(cells[subc].sum()):.3e} cells), one C call, {cs:.1f} s")
+    log(f"[bench] sw_long extras: {out}")
+    return out
+
+
 def host_info():
     model = "unknown"
     try:
@@ -491,11 +604,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--tile", default=None,
-                    help="C,G: run the long-read batch on this tile instead of the autotuner's "
+                    help="C,G: run the headline batch on this tile instead of the autotuner's "
                          "pick (profiling runs pin the tile they measure)")
-    ap.add_argument("--walk", choices=["separate", "fused"], default="fused",
-                    help="CIGAR walk inside the forward launch (fused) or as its own launch "
-                         "(one lane per pair)")
+    ap.add_argument("--launch", default=None,
+                    help="split_rows=R,split_slices=S,prio_rows=P: run the headline batch with "
+                         "this launch instead of the autotuner's pick")
     ap.add_argument("--cpu-sample-stride", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=200_000,
                     help="requests of the serving stream (0: skip the serving measurement)")
@@ -530,46 +643,73 @@ def main():
         print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 3}), flush=True)
         return
 
-    # ------------------------------------------------------------------ batch
+    threads = os.cpu_count() or 1
+    # ------------------------------------------------------------ headline
+    # the noisy uneven-length batch (8,192 pairs, 0.5-100 kb, band 1503): the
+    # largest single-GPU configuration, and the one whose tail the launch
+    # choices address
     t0 = time.time()
-    w = WL.sw_long()
-    log(f"[bench] sw_long generated: {w.n_pairs} pairs, {w.cells().sum():.3e} cells, "
+    wn = WL.sw_noisy()
+    ncells = wn.cells()
+    log(f"[bench] sw_noisy generated: {wn.n_pairs} pairs, {ncells.sum():.3e} cells, "
         f"{time.time() - t0:.1f}s")
-    fused = args.walk == "fused"
-    # tile autotuning on every 4th pair (by cell count) of the batch: 4,096
-    # pairs keep about 4 wavefronts per SIMD resident, as the batch does
-    sub_t = WL.subsample(w, np.argsort(w.cells(), kind="stable")[::4])
-    inp_t = K.upload_pairs(sub_t.queries, sub_t.targets, sub_t.band_offset, dev, alphabet="2bit")
-    tuned, tune_ms = K.autotune_tile(inp_t, w.half_band, scoring(w),
-                                     order=torch.from_numpy(lpt_order(sub_t.cells())).to(dev))
+    # tile per length bin: 8 log-spaced bins, each autotuned on itself
+    lens = np.array([len(q) for q in wn.queries])
+    edges = np.unique(np.round(np.geomspace(lens.min(), lens.max() + 1, 9)).astype(np.int64))
+    bin_idx = [np.flatnonzero((lens >= lo) & (lens < hi)) for lo, hi in zip(edges[:-1], edges[1:])]
+    bins = []
+    for (lo, hi), idx in zip(zip(edges[:-1], edges[1:]), bin_idx):
+        if not len(idx):
+            continue
+        if args.tile:  # pinned tile (profiling runs): no per-bin autotune launches
+            bins.append({"len": [int(lo), int(hi)], "pairs": int(len(idx)),
+                         "cells": int(ncells[idx].sum()),
+                         "tile": [int(x) for x in args.tile.split(",")]})
+            continue
+        sb = WL.subsample(wn, idx)
+        inp_b = K.upload_pairs(sb.queries, sb.targets, sb.band_offset, dev, alphabet="2bit")
+        tb_, tms = K.autotune_tile(inp_b, wn.half_band, scoring(wn), reps=2,
+                                   order=torch.from_numpy(lpt_order(sb.cells())).to(dev))
+        del inp_b
+        bins.append({"len": [int(lo), int(hi)], "pairs": int(len(idx)),
+                     "cells": int(ncells[idx].sum()),
+                     "tile": [tb_.cells_per_lane, tb_.lanes_per_pair],
+                     "autotune_ms": {f"{c}x{g}": v for (c, g), v in tms.items()}})
+    torch.cuda.empty_cache()
+    picks = {tuple(b["tile"]) for b in bins}
+    tile_n = K.resolve_tile(wn.half_band, *next(iter(picks))) if len(picks) == 1 else \
+        K.resolve_tile(wn.half_band)
     if args.tile:
         c_, g_ = (int(x) for x in args.tile.split(","))
-        tuned = K.resolve_tile(w.half_band, c_, g_)
-    del inp_t
-    torch.cuda.empty_cache()
-    log(f"[bench] autotune (ms per sample launch): {tune_ms} -> "
-        f"{tuned.cells_per_lane}x{tuned.lanes_per_pair}")
-    run = BatchRun(torch, K, w, dev, fused, tile=tuned)
-    total_cells, cells, tile = run.total_cells, run.cells, run.tile
-    log(f"[bench] resident: tb planes {run.ws['tb_bytes'] / 1e9:.1f} GB, tile "
-        f"{tile.cells_per_lane}x{tile.lanes_per_pair}")
+        tile_n = K.resolve_tile(wn.half_band, c_, g_)
+    rn = BatchRun(torch, K, wn, dev, True, tile=tile_n)
+    total_cells = rn.total_cells
+    # launch choice on the whole batch: plain, the longest pairs at raised
+    # priority, or split over several wavefronts
+    if args.launch:  # pinned launch (profiling runs): no autotune launches
+        launch, launch_ms = {"split_rows": 0, "split_slices": 0, "prio_rows": 0}, {}
+        for kv in args.launch.split(","):
+            k_, v_ = kv.split("=")
+            launch[k_] = int(v_)
+    else:
+        launch, launch_ms = K.autotune_launch(rn.inp, wn.half_band, scoring(wn), rn.order,
+                                              tile=rn.tile, workspace=rn.ws)
+    rn.launch = launch
+    log(f"[bench] launch autotune (ms): {launch_ms} -> {launch}")
+    log(f"[bench] resident: tb planes {rn.ws['tb_bytes'] / 1e9:.1f} GB, tile "
+        f"{rn.tile.cells_per_lane}x{rn.tile.lanes_per_pair}")
 
     stream = torch.cuda.current_stream()
-    fwd_ev, tb_ev = [], []
+    fwd_ev = []
 
     def step(record):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
-        c = torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        fwd = run.forward()
+        rn.forward()
         b.record(stream)
-        tbk = run.walk(fwd)
-        c.record(stream)
         if record:
             fwd_ev.append((a, b))
-            tb_ev.append((b, c))
-        return fwd, tbk
 
     for k in range(args.warmup):
         step(False)
@@ -583,18 +723,17 @@ def main():
     t_total = time.perf_counter() - t_start
     ms_step = t_total * 1e3 / max(args.steps, 1)
     fwd_ms = float(np.mean([a.elapsed_time(b) for a, b in fwd_ev]))
-    tb_ms = float(np.mean([a.elapsed_time(b) for a, b in tb_ev]))
     gcups = total_cells / (ms_step / 1e3) / 1e9
-    log(f"[bench] timed {args.steps} steps: {ms_step:.1f} ms/step (forward {fwd_ms:.1f} ms, "
-        f"walk {tb_ms:.1f} ms, {args.walk}) -> {gcups:.1f} GCUPS")
+    log(f"[bench] timed {args.steps} steps: {ms_step:.1f} ms/step (launch {fwd_ms:.1f} ms, "
+        f"walk fused) -> {gcups:.1f} GCUPS")
 
-    # algorithmic traffic of the forward kernel: 0.5 B/cell of traceback planes
-    # + the 2-bit sequences read once
-    inp = run.inp
+    inp = rn.inp
     seq_bytes = float((inp.qlen_host.sum() + inp.tlen_host.sum()) / 4)
     alg_bytes = total_cells * TB_BYTES_PER_CELL + seq_bytes
     hbm_gbs = alg_bytes / (fwd_ms / 1e3) / 1e9
     valu_tops = total_cells * OPS_PER_CELL_TB / (fwd_ms / 1e3) / 1e12
+    launch_label = ("plain" if not launch["split_rows"] and not launch["prio_rows"] else
+                    ",".join(f"{k}={v}" for k, v in launch.items() if v))
     result = {
         "metric": METRIC,
         "value": round(gcups, 2),
@@ -609,123 +748,102 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {
-            "workload": "sw_long: 16384 pairs, lognormal(12 kb, 0.4) clipped 5-25 kb, 0.5% errors, "
-                        "half_band 256, scoring 2/-4/-4/-2, traceback + CIGAR, seed 2",
-            "pairs": w.n_pairs,
+            "workload": "sw_noisy: 8192 pairs, lognormal(8 kb, 1.0) clipped 0.5-100 kb, 6% "
+                        "errors (homopolymer-weighted indels), half_band 751, scoring "
+                        "2/-4/-4/-2, traceback + CIGAR, seed 3",
+            "pairs": wn.n_pairs,
             "cells": total_cells,
-            "half_band": w.half_band,
-            "tile": [tile.cells_per_lane, tile.lanes_per_pair],
-            "autotune_ms": {f"{c}x{g}": v for (c, g), v in tune_ms.items()},
-            "step": "sw_forward (scores, ends, 4-bit planes) + CIGAR walk of every pair "
-                    f"({args.walk})",
+            "half_band": wn.half_band,
+            "tile": [rn.tile.cells_per_lane, rn.tile.lanes_per_pair],
+            "length_bins": bins,
+            "launch": launch,
+            "launch_autotune_ms": launch_ms,
+            "step": "one launch: sw_forward (scores, ends, 4-bit planes) with the CIGAR walk of "
+                    "every pair fused, longest first",
         },
-        "kernels_ms": {"sw_forward": round(fwd_ms, 3), "sw_walk": round(tb_ms, 3),
-                       "walk_mode": args.walk},
+        "kernels_ms": {"sw_forward": round(fwd_ms, 3), "walk_mode": "fused"},
         "roofline": {
-            "bound": "hbm",
-            "achieved": round(hbm_gbs, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(hbm_gbs / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(total_cells, fused, tile),
-            "kernel": "sw_forward_kernel",
-            "note": "the kernel is int32-VALU bound; see roofline_valu",
-        },
-        "roofline_valu": {
             "bound": "valu_int32",
             "achieved": round(valu_tops, 3),
             "peak": round(INT32_PEAK_TOPS, 2),
             "unit": "TOP/s",
             "frac": round(valu_tops / INT32_PEAK_TOPS, 4),
             "ops_per_cell": OPS_PER_CELL_TB,
+            "kernel": "sw_forward_kernel",
+            "traffic": measured_traffic("sw_noisy", total_cells, rn.tile, launch_label),
+        },
+        "roofline_hbm": {
+            "bound": "hbm",
+            "achieved": round(hbm_gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(hbm_gbs / HBM_PEAK_GBS, 5),
+            "note": "algorithmic bytes: 0.5 B of planes per cell + the 2-bit sequences",
         },
     }
     if args.skip_extras:
         print(json.dumps(result), flush=True)
         return
-
-    threads = min(os.cpu_count() or 1, 16)
     result["host"] = host_info()
 
-    # ------------------------------------------------- host buffers in and out
-    # the same batch starting from host arrays: the upload and device packing,
-    # the step, and the CIGAR runs back to the host
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    inp_h = K.upload_pairs(w.queries, w.targets, w.band_offset, dev, alphabet="2bit")
-    fwd_h = K.sw_forward(inp_h, *scoring(w), w.half_band, traceback=True, order=run.order,
-                         tile=tile, workspace=run.ws, cigar=True)
-    tb_h = fwd_h.walk
-    nc_h = tb_h.n_cigar.to(torch.int64)
-    tot = int(nc_h.sum().item())
-    owner = torch.repeat_interleave(torch.arange(w.n_pairs, device=dev), nc_h)
-    within = torch.arange(tot, device=dev) - (torch.cumsum(nc_h, 0) - nc_h)[owner]
-    runs_h = tb_h.cigar[tb_h.cigar_off[owner] + within].cpu()
-    _ = (fwd_h.score.cpu(), fwd_h.query_end.cpu(), fwd_h.target_end.cpu(), nc_h.cpu())
-    host_s = time.perf_counter() - t1
-    result["host_inclusive"] = {
-        "gcups": round(total_cells / host_s / 1e9, 2),
-        "s": round(host_s, 3),
-        "cigar_runs": int(runs_h.numel()),
-        "note": "host arrays in: concatenation, one upload per stream, 2-bit packing on the "
-                "device, forward with fused walk, scores/ends/CIGAR runs copied back; not "
-                "the headline value",
-    }
-    log(f"[bench] host-inclusive: {result['host_inclusive']}")
-    del inp_h, fwd_h, tb_h, runs_h
-
-    # ------------------------------------- x-drop extension path (drop-in form)
-    # what ExtensionEngine / banded_affine_sw_batch run with the reference's
-    # x-drop on: forward with planes and per-row keys, the batch-wide x-drop of
-    # each group of 8 pairs (one read's chains), the walk from the x-drop ends
-    gofs = np.arange(0, w.n_pairs + 1, 8)
-    xd = {}
-
-    def xdrop_step():
-        f = K.sw_forward(run.inp, *scoring(w), w.half_band, traceback=True, rowkeys=True,
-                         order=run.order, tile=tile, workspace=run.ws)
-        sc_, qe_, te_ = K.sw_xdrop(f, run.inp, gofs, 600.0, w.half_band)
-        xd["tb"] = K.sw_traceback(run.inp, f, sc_, qe_, te_, workspace=run.ws)
-        return f
-
-    xdrop_step()
-    xms = time_events(torch, xdrop_step, 3)
-    result["sw_long_xdrop"] = {
-        "gcups": round(total_cells / (xms * 1e-3) / 1e9, 2),
-        "ms": round(xms, 3),
-        "workload": "sw_long, forward with traceback planes + per-row keys (longest first), "
-                    "batch x-drop 600 over groups of 8 pairs, standalone walk from the x-drop "
-                    "ends; GCUPS over the full band cells",
-    }
-    log(f"[bench] x-drop path: {result['sw_long_xdrop']}")
-    del xd
-
-    # ------------------------------------------------------------- per-pair form
-    sub = np.arange(0, w.n_pairs, 256)[:64]
-    pp = per_pair_gcups(torch, K, w, sub, dev)
-    result["per_pair_gpu"] = {"value": round(pp, 3), "unit": "GCUPS",
-                              "sample": f"{len(sub)} pairs of sw_long (every 256th), one "
-                                        "forward launch + one walk launch per pair"}
-    result["batched_over_per_pair"] = round(gcups / pp, 1)
-    log(f"[bench] per-pair GPU form: {pp:.3f} GCUPS")
-
-    # --------------------------------------------------------------- CPU oracle
-    srt = np.argsort(cells, kind="stable")
-    sub = srt[::args.cpu_sample_stride]
-    cg, cs = cpu_gcups(w, sub, threads)
+    # the same step captured in a HIP graph
+    gn = K.SwGraph(rn.inp, scoring(wn), wn.half_band, order=rn.order, tile=rn.tile,
+                   workspace=rn.ws, cigar=True, **launch)
+    gnms = time_events(torch, gn.replay, 3)
+    del gn
+    # the longest pair alone, plain and with the chosen launch
+    longest = int(np.argmax(ncells))
+    one = BatchRun(torch, K, WL.subsample(wn, [longest]), dev, True, tile=rn.tile)
+    one.step()
+    ms1 = time_events(torch, one.step, 2)
+    one.launch = {"split_rows": 1, "split_slices": launch["split_slices"]} \
+        if launch["split_rows"] else dict(launch)
+    one.step()
+    ms1l = time_events(torch, one.step, 2)
+    del one
+    # each length bin as a batch of its own (its autotuned tile, plain launch)
+    for b, idx in zip(bins, [i for i in bin_idx if len(i)]):
+        rb = BatchRun(torch, K, WL.subsample(wn, idx), dev, True,
+                      tile=K.resolve_tile(wn.half_band, *b["tile"]))
+        rb.step()
+        bms = time_events(torch, rb.step, 2)
+        b["ms"] = round(bms, 3)
+        b["gcups"] = round(b["cells"] / (bms / 1e3) / 1e9, 2)
+        del rb
+    del rn
+    torch.cuda.empty_cache()
+    srt = np.argsort(ncells, kind="stable")
+    ppn = per_pair_gcups(torch, K, wn, srt[::256], dev)
+    subn = srt[::args.cpu_sample_stride]
+    cgn, csn = cpu_gcups(wn, subn, threads)
+    result["per_pair_gpu"] = {"value": round(ppn, 4), "unit": "GCUPS",
+                              "sample": f"{len(srt[::256])} sw_noisy pairs (every 256th by cell "
+                                        "count), one forward launch + one walk launch per pair"}
+    result["batched_over_per_pair"] = round(gcups / ppn, 1)
     result["cpu_baseline"] = {
-        "value": round(cg, 4),
+        "value": round(cgn, 4),
         "unit": "GCUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(sub)} sw_long pairs (every {args.cpu_sample_stride}th by cell count, "
-                  f"{int(cells[sub].sum()):.3e} cells), forward + traceback, one pair per "
-                  f"thread task, {cs:.1f} s",
+        "sample": f"{len(subn)} sw_noisy pairs (every {args.cpu_sample_stride}th by cell count, "
+                  f"{int(ncells[subn].sum()):.3e} cells), forward + traceback in one C call "
+                  f"on {threads} threads, {csn:.1f} s",
     }
-    result["gpu_over_cpu"] = round(gcups / cg, 1)
-    log(f"[bench] CPU oracle: {cg:.3f} GCUPS on {threads} threads ({cs:.1f} s)")
-    del run
-    torch.cuda.empty_cache()
+    result["gpu_over_cpu"] = round(gcups / cgn, 1)
+    result["sw_noisy"] = {
+        "graph_ms": round(gnms, 3),
+        "graph_gcups": round(total_cells / (gnms / 1e3) / 1e9, 2),
+        "longest_pair_ms": round(ms1, 3),
+        "longest_pair_launch_ms": round(ms1l, 3),
+        "longest_pair_cells": int(ncells[longest]),
+        "longest_pair_frac_of_batch": round(ms1l / ms_step, 3),
+        "length_bins_note": "each bin also run as its own batch (its tile, plain launch)",
+    }
+    log(f"[bench] sw_noisy extras: {result['sw_noisy']}, per-pair {ppn:.3f}, CPU {cgn:.3f} "
+        f"GCUPS on {threads} threads")
+
+    # ------------------------------------------------------- long-read batch
+    result["sw_long"] = long_read_extra(torch, K, WL, dev, args, threads)
 
     # ------------------------------------------------------- short-read batch
     ws_ = WL.sw_short()
@@ -736,7 +854,6 @@ def main():
     rs = BatchRun(torch, K, ws_, dev, True, tile=tuned_s)
     rs.step()
     ms = time_events(torch, rs.step, 10)
-    # the same step captured in a HIP graph (no host work per launch)
     gr = K.SwGraph(rs.inp, scoring(ws_), ws_.half_band, order=rs.order, tile=rs.tile,
                    workspace=rs.ws, cigar=True)
     gms = time_events(torch, gr.replay, 20)
@@ -752,68 +869,12 @@ def main():
         "cpu_threads": threads,
         "tile": [rs.tile.cells_per_lane, rs.tile.lanes_per_pair],
         "autotune_ms": {f"{c}x{g}": v for (c, g), v in tune_s.items()},
-        "workload": "1024 x 150 bp, half_band 32 at offset 32, seed 0 (CPU: whole batch)",
+        "workload": "1024 x 150 bp, half_band 32 at offset 32, seed 0 (CPU: whole batch, one C "
+                    "call)",
     }
     log(f"[bench] sw_short: {result['sw_short']}")
-    del rs
-
-    # ------------------------------------------------- noisy wide-band batch
-    t0 = time.time()
-    wn = WL.sw_noisy()
-    log(f"[bench] sw_noisy generated: {wn.n_pairs} pairs, {wn.cells().sum():.3e} cells, "
-        f"{time.time() - t0:.1f}s")
-    rn = BatchRun(torch, K, wn, dev, True)
-    rn.step()
-    torch.cuda.synchronize()
-    msn = time_events(torch, rn.step, 2)
-    ncells = rn.cells
-    # single longest pair alone: the lower bound a batch cannot beat
-    longest = int(np.argmax(ncells))
-    one = BatchRun(torch, K, WL.subsample(wn, [longest]), dev, True)
-    one.step()
-    ms1 = time_events(torch, one.step, 2)
-    del one
-    lens = np.array([len(q) for q in wn.queries])
-    edges = np.unique(np.round(np.geomspace(lens.min(), lens.max() + 1, 9)).astype(np.int64))
-    bins = []
-    for lo, hi in zip(edges[:-1], edges[1:]):
-        idx = np.flatnonzero((lens >= lo) & (lens < hi))
-        if len(idx):
-            rb = BatchRun(torch, K, WL.subsample(wn, idx), dev, True)
-            rb.step()
-            bms = time_events(torch, rb.step, 2)
-            bins.append({"len": [int(lo), int(hi)], "pairs": int(len(idx)),
-                         "cells": int(ncells[idx].sum()), "ms": round(bms, 3),
-                         "gcups": round(float(ncells[idx].sum()) / (bms / 1e3) / 1e9, 2)})
-            del rb
+    del rs, gr
     torch.cuda.empty_cache()
-    tile_n = [rn.tile.cells_per_lane, rn.tile.lanes_per_pair]
-    del rn
-    torch.cuda.empty_cache()
-    srt = np.argsort(ncells, kind="stable")
-    ppn = per_pair_gcups(torch, K, wn, srt[::256], dev)
-    subn = srt[::16]
-    cgn, csn = cpu_gcups(wn, subn, threads)
-    result["sw_noisy"] = {
-        "batched_gcups": round(int(ncells.sum()) / (msn / 1e3) / 1e9, 3),
-        "batched_ms": round(msn, 3),
-        "longest_pair_ms": round(ms1, 3),
-        "longest_pair_cells": int(ncells[longest]),
-        "per_pair_gcups": round(ppn, 4),
-        "per_pair_sample": "every 256th pair by cell count (32 pairs)",
-        "cpu_gcups": round(cgn, 4),
-        "cpu_threads": threads,
-        "cpu_sample": f"every 16th pair by cell count ({len(subn)} pairs, "
-                      f"{int(ncells[subn].sum()):.3e} cells), {csn:.1f} s",
-        "tile": tile_n,
-        "length_bins": bins,
-        "length_bins_note": "each bin of 8 log-spaced length bins run as its own batch "
-                            "(one launch, fused walk)",
-        "workload": "8192 pairs, lognormal(8 kb, 1.0) clipped 0.5-100 kb, 6% errors "
-                    "(homopolymer-weighted indels), half_band 751, seed 3, fused walk",
-    }
-    log(f"[bench] sw_noisy: {msn:.1f} ms -> {result['sw_noisy']['batched_gcups']} GCUPS, "
-        f"longest pair alone {ms1:.1f} ms, per-pair {ppn:.3f}, CPU {cgn:.3f}")
 
     # ----------------------------------------------------------------- serving
     if args.serve_requests > 0:
@@ -836,7 +897,10 @@ def main():
 
     chain_step()
     cms = time_events(torch, chain_step, 5)
-    lut = torch.from_numpy(O.penalty_lut(6.0, 0.05, 0.5, wc.max_gap)).to(dev)
+    from graphmambaformer_amd.alignment.chaining import penalty_table
+    from graphmambaformer_amd.config import ChainingConfig
+
+    lut = penalty_table(ChainingConfig(max_lookback=wc.max_lookback, max_gap=wc.max_gap), dev)
     wt64 = wt.to(torch.float64)
 
     def chain_step64():
@@ -844,17 +908,23 @@ def main():
 
     chain_step64()
     cms64 = time_events(torch, chain_step64, 5)
-    t1 = time.perf_counter()
-    for s in range(wc.n_reads):
-        a, b = wc.seg_off[s], wc.seg_off[s + 1]
+    # CPU: the oracle's C DP per read, reads fanned over every host core
+    # (chaining.py:657-666 runs reads in a thread pool; the C call releases
+    # the interpreter lock)
+    from concurrent.futures import ThreadPoolExecutor
+
+    def cpu_read(s_):
+        a, b = wc.seg_off[s_], wc.seg_off[s_ + 1]
         O.chain_dp_i32(wc.read_end[a:b], wc.ref_end[a:b], wc.weight[a:b], wc.max_lookback,
                        wc.max_gap, wc.gap_open, wc.gap_extend)
-    ccpu = time.perf_counter() - t1
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(cpu_read, range(64)))
+        t1 = time.perf_counter()
+        list(ex.map(cpu_read, range(wc.n_reads)))
+        ccpu = time.perf_counter() - t1
     # anchors -> chains on the device: sort into DP order, DP (float64 reference
     # defaults), backtrack, select; host arrays in, chain records out
-    from graphmambaformer_amd.alignment.chaining import penalty_table
-    from graphmambaformer_amd.config import ChainingConfig
-
     ccfg = ChainingConfig(max_lookback=wc.max_lookback, max_gap=wc.max_gap)
     nrd = wc.n_reads
     seg_of = np.repeat(np.arange(nrd), np.diff(wc.seg_off))
@@ -895,14 +965,26 @@ def main():
         O.chain_read(rp_[m], fp_[m], np.full(int(m.sum()), 15), np.ones(int(m.sum()), np.int64),
                      max_lookback=ccfg.max_lookback, max_gap=ccfg.max_gap)
     cpu_pipe = (time.perf_counter() - t1) / len(sub)
+    evals = wc.pair_evaluations()
+    n_anchor = int(wc.seg_off[-1])
     result["chains"] = {
         "reads_per_s": round(wc.n_reads / (cms / 1e3), 1),
-        "pair_evals_per_s": round(wc.pair_evaluations() / (cms / 1e3), 1),
+        "pair_evals_per_s": round(evals / (cms / 1e3), 1),
+        "roofline": {
+            "bound": "valu_int32",
+            "frac": round(evals * CHAIN_OPS_PER_EVAL / (cms / 1e3) / 1e12 / INT32_PEAK_TOPS, 4),
+            "ops_per_eval": CHAIN_OPS_PER_EVAL,
+            "hbm_frac": round(n_anchor * CHAIN_BYTES_PER_ANCHOR / (cms / 1e3) / 1e9 /
+                              HBM_PEAK_GBS, 5),
+            "bytes_per_anchor": CHAIN_BYTES_PER_ANCHOR,
+            "note": "integer profile DP; the DP is sequential in the anchor index, so one "
+                    "wavefront per read runs at its latency, not at the VALU or HBM roof",
+        },
         "ms": round(cms, 4),
         "f64_ref_default_ms": round(cms64, 4),
         "f64_reads_per_s": round(wc.n_reads / (cms64 / 1e3), 1),
         "cpu_reads_per_s": round(wc.n_reads / ccpu, 1),
-        "cpu_threads": 1,
+        "cpu_threads": threads,
         "anchors_to_chains_ms": round(pms, 3),
         "anchors_to_chains_reads_per_s": round(nrd / (pms / 1e3), 1),
         "anchors_to_chains_with_host_ms": round(pms_host, 3),
@@ -910,7 +992,8 @@ def main():
         "cpu_anchors_to_chains_reads_per_s": round(1.0 / cpu_pipe, 1),
         "workload": "4096 reads, Poisson(400) anchors, lookback 50, max_gap 5000, "
                     "int profile 6/1 (and float64 6/0.05/0.5), seed 1; CPU: whole batch, "
-                    "int profile, one thread; anchors_to_chains: unsorted anchors resident on "
+                    "int profile, the oracle's C DP per read on every host core; "
+                    "anchors_to_chains: unsorted anchors resident on "
                     "the device, sort + float64 DP + backtrack + select (with_host adds the "
                     "uploads and the readback of the chain records); CPU: the oracle's "
                     "chain_read on every 16th read",

@@ -71,6 +71,7 @@ struct SwKArgs {
     int32_t walk;  // walk every pair right after its forward pass (CIGAR out)
     int32_t blkstride;   // plane dwords per 8-row block (lanes_stored * C of the tile)
     int32_t split_rows;  // split launches: pairs with at least this many rows run as slices
+    int32_t prio_rows;   // offline: pairs with at least this many rows run at raised priority
     WalkCtx wc;
     WalkOut wo;
     // continuous serving (serve_tail == nullptr: one offline batch).  Request
@@ -1142,6 +1143,10 @@ __global__ __launch_bounds__(64 * SPL) void sw_forward_kernel(SwKArgs a)
                                   : (has ? a.q_off[p] : 0);
         const int64_t to = online ? r_to : (has ? a.t_off[p] : 0);
         const int maxq = (G == 64) ? qlen : wave_max(qlen);
+        // the longest pairs set a batch's end: at raised wave priority they
+        // keep their SIMD's issue slots, the other waves there fill their stalls
+        const bool prio = !SV && a.prio_rows > 0 && maxq >= a.prio_rows;
+        if (prio) __builtin_amdgcn_s_setprio(2);
 
         // serving with paged planes: the request's pages before its forward
         // pass (admission by the planes' HBM footprint)
@@ -1585,6 +1590,7 @@ __global__ __launch_bounds__(64 * SPL) void sw_forward_kernel(SwKArgs a)
                 }
             }
         }
+        if (prio) __builtin_amdgcn_s_setprio(0);
         if constexpr (SV) {
             if (nopages && lane == 0) a.wo.ncig[p] = -1;
             __syncthreads();  // the walk's stores by every lane, then the flag
@@ -2198,6 +2204,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     a.walk = 0;
     a.blkstride = tile.lanes_stored * tile.cells_per_lane;
     a.split_rows = 0;
+    a.prio_rows = serve ? 0 : std::max(0, P.prio_rows);
     a.wc = make_walk_ctx(batch, params, tile, nm);
     a.wo = WalkOut{};
     if (P.flags & GMFA_SW_CIGAR) {

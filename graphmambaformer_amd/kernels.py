@@ -287,7 +287,7 @@ class SwForward:
 
 
 def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile, split_rows: int = 0,
-                split_slices: int = 0) -> SwParams:
+                split_slices: int = 0, prio_rows: int = 0) -> SwParams:
     p = SwParams()
     p.match, p.mismatch, p.gap_open, p.gap_extend = int(ma), int(mi), int(go), int(ge)
     p.half_band = int(half_band)
@@ -296,6 +296,7 @@ def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile, split_rows: int 
     p.lanes_per_pair = tile.lanes_per_pair
     p.split_rows = int(split_rows)
     p.split_slices = int(split_slices)
+    p.prio_rows = int(prio_rows)
     return p
 
 
@@ -334,13 +335,14 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
                traceback: bool = True, rowkeys: bool = False,
                order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
                workspace: Optional[dict] = None, cigar: bool = False, split_rows: int = 0,
-               split_slices: int = 0) -> SwForward:
+               split_slices: int = 0, prio_rows: int = 0) -> SwForward:
     """Forward pass.  ``cigar=True`` also walks every pair inside the same
     launch (GMFA_SW_CIGAR) and returns the CIGARs in ``.walk``.
     ``split_rows > 0`` (with ``cigar``, ``order`` and no N masks): pairs of at
     least that many query rows run on several wavefronts each, a slice of the
     band per wavefront (``split_slices`` of them, 0 = the first compiled
-    count); the outputs are those of the plain launch."""
+    count); the outputs are those of the plain launch.  ``prio_rows > 0``:
+    pairs of at least that many query rows run at raised wave priority."""
     dev = inp.device
     n = inp.n_pairs
     tile = tile or batch_tile(half_band, n, dev)
@@ -391,11 +393,12 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
         out.cigar_off, out.cigar = ptr(walk.cigar_off), ptr(walk.cigar)
         out.n_cigar, out.read_start = ptr(walk.n_cigar), ptr(walk.read_start)
         out.ref_start, out.counts = ptr(walk.ref_start), ptr(walk.counts)
-    params = make_params(ma, mi, go, ge, half_band, flags, tile, split_rows, split_slices)
+    params = make_params(ma, mi, go, ge, half_band, flags, tile, split_rows, split_slices,
+                         prio_rows)
     batch = inp.batch_struct(order)
     check(lib().gmfa_sw_forward(ctypes.byref(batch), ctypes.byref(params), ctypes.byref(out),
                                 stream_of(dev)), "gmfa_sw_forward")
-    params.split_rows = params.split_slices = 0
+    params.split_rows = params.split_slices = params.prio_rows = 0
     params.flags = int(flags) & ~_lib.SW_CIGAR  # a later standalone walk uses the same params
     return SwForward(score=score, query_end=qend, target_end=tend, tb=tb, tb_off=tb_off,
                      rowkey=rk, rowkey_off_host=rk_off_host, tile=tile, params=params, walk=walk)
@@ -461,7 +464,7 @@ class SwGraph:
     def __init__(self, inp: SwInputs, scoring: tuple, half_band: int,
                  order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
                  workspace: Optional[dict] = None, cigar: bool = True, split_rows: int = 0,
-                 split_slices: int = 0):
+                 split_slices: int = 0, prio_rows: int = 0):
         dev = inp.device
         self.tile = tile or batch_tile(half_band, inp.n_pairs, dev)
         self.ws = workspace if workspace is not None else make_workspace(inp, self.tile,
@@ -470,7 +473,8 @@ class SwGraph:
         def step():
             return sw_forward(inp, *scoring, half_band, traceback=True, order=order,
                               tile=self.tile, workspace=self.ws, cigar=cigar,
-                              split_rows=split_rows, split_slices=split_slices)
+                              split_rows=split_rows, split_slices=split_slices,
+                              prio_rows=prio_rows)
 
         # one eager pass on a side stream first (module load, allocator warm-up)
         side = torch.cuda.Stream(device=dev)
@@ -817,46 +821,54 @@ def cigar_bam(cigar: torch.Tensor, cigar_off: torch.Tensor, n_cigar: torch.Tenso
 
 def autotune_launch(inp: SwInputs, half_band: int, scoring: tuple, order: torch.Tensor,
                     tile: Optional[SwTile] = None, workspace: Optional[dict] = None,
-                    tops=(16, 32, 64, 128, 256), reps: int = 2, margin: float = 0.01):
-    """Choose between the plain launch and the split launches (the longest
-    pairs over several wavefronts) for this batch: times the plain launch and
-    every compiled split variant with the ``tops`` longest queries split
-    (split_rows = the length of the k-th longest query), on the whole batch
-    itself (the tail that splitting shortens is a property of the whole
-    batch, not of a sample), each the best of ``reps`` launches.  Keeps the
-    plain launch unless a split one is faster by more than ``margin``.
-    Returns ((split_rows, split_slices), {label: ms})."""
+                    tops=(16, 64, 256, 1024), reps: int = 3, margin: float = 0.03):
+    """Choose how a batch of uneven lengths is launched: the plain launch,
+    the plain launch with the ``k`` longest pairs at raised wave priority
+    (``prio_rows``), or a split launch (the ``k`` longest pairs over several
+    wavefronts, every compiled split variant), for ``k`` in ``tops``.  Each
+    candidate is timed on the whole batch itself (the tail these options
+    shorten is a property of the whole batch, not of a sample), the median of
+    ``reps`` launches (single launches of one configuration spread by several
+    percent on an uneven batch); the plain launch is kept unless another is
+    faster by more than ``margin``.  Returns (keyword arguments for sw_forward:
+    split_rows, split_slices, prio_rows; {label: ms})."""
     dev = inp.device
     tile = tile or batch_tile(half_band, inp.n_pairs, dev)
     ws = workspace if workspace is not None else make_workspace(inp, tile, cigar=True)
     s = torch.cuda.current_stream(dev)
 
-    def best_ms(rows, slices):
+    def best_ms(kw):
         sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
-                   workspace=ws, cigar=True, split_rows=rows, split_slices=slices)
+                   workspace=ws, cigar=True, **kw)
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(reps)]
         for e0, e1 in ev:
             e0.record(s)
             sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
-                       workspace=ws, cigar=True, split_rows=rows, split_slices=slices)
+                       workspace=ws, cigar=True, **kw)
             e1.record(s)
         torch.cuda.synchronize(dev)
-        return min(e0.elapsed_time(e1) for e0, e1 in ev)
+        return round(float(np.median([e0.elapsed_time(e1) for e0, e1 in ev])), 4)
 
-    timings = {"plain": round(best_ms(0, 0), 4)}
-    best, best_t = (0, 0), timings["plain"]
-    variants = split_variants(half_band, tile.cells_per_lane) if not inp.has_n else []
+    plain = {"split_rows": 0, "split_slices": 0, "prio_rows": 0}
+    timings = {"plain": best_ms(plain)}
+    best, best_t = plain, timings["plain"] * (1.0 - margin)
     ql = np.sort(np.asarray(inp.qlen_host))[::-1]
+    cands = []
+    for k in tops:
+        if k <= len(ql):
+            cands.append((f"prio_top{k}", dict(plain, prio_rows=int(ql[k - 1]))))
+    variants = split_variants(half_band, tile.cells_per_lane) if not inp.has_n else []
     for _, slices in variants:
         for k in tops:
-            if k > len(ql):
-                continue
-            rows = int(ql[k - 1])
-            ms = round(best_ms(rows, slices), 4)
-            timings[f"split{slices}_top{k}_rows{rows}"] = ms
-            if ms < best_t * (1.0 - margin) and ms < timings["plain"] * (1.0 - margin):
-                best, best_t = (rows, slices), ms
+            if k <= len(ql):
+                cands.append((f"split{slices}_top{k}",
+                              dict(plain, split_rows=int(ql[k - 1]), split_slices=slices)))
+    for label, kw in cands:
+        ms = best_ms(kw)
+        timings[label] = ms
+        if ms < best_t:
+            best, best_t = kw, ms
     return best, timings
 
 

@@ -96,6 +96,11 @@ typedef struct gmfa_sw_params {
      * launch.  0 / 0: the plain launch. */
     int32_t split_rows;
     int32_t split_slices;
+    /* prio_rows > 0 (gmfa_sw_forward): pairs with at least prio_rows query rows
+     * run at raised wave priority, so the longest pairs of an uneven batch
+     * keep their SIMD's issue slots and the other waves fill their stalls. */
+    int32_t prio_rows;
+    int32_t reserved;
 } gmfa_sw_params;
 
 /* A batch of pairs whose sequences live in 2-bit packed streams

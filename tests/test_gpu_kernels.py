@@ -140,6 +140,34 @@ def test_sw_specialised_band_tiles(hb, fused):
     _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"spec{hb}")
 
 
+@pytest.mark.parametrize("tile", [(24, 64), (32, 64)])
+def test_sw_band_1503_on_every_tile_that_holds_it(tile):
+    """half_band 751 (the noisy batch's band) on each compiled tile a length
+    bin's autotune can pick for it, with raised priority for the longer half
+    of the pairs (prio_rows), against the oracle."""
+    rng = np.random.default_rng(1503 + tile[0])
+    qs, ts, offs = [], [], []
+    for p in range(16):
+        n = int(rng.integers(1, 4000)) if p % 3 else int(rng.integers(1, 60))
+        q, t = _random_pair(rng, n)
+        qs.append(q)
+        ts.append(t)
+        offs.append(int(rng.integers(-800, 800)) if p % 2 else int(rng.integers(-5, 25)))
+    _say("case: band 1503 on tile", tile)
+    k = K()
+    inp = k.upload_pairs(qs, ts, offs, DEV, alphabet="ref")
+    lpt = torch.from_numpy(np.argsort(-inp.qlen_host, kind="stable").astype(np.int32)).to(DEV)
+    fwd = k.sw_forward(inp, 2, 4, 4, 2, 751, traceback=True, order=lpt, cigar=True,
+                       tile=k.resolve_tile(751, *tile),
+                       prio_rows=int(np.median(inp.qlen_host)))
+    tbk = fwd.walk
+    got = dict(score=fwd.score.cpu().numpy(), qend=fwd.query_end.cpu().numpy(),
+               tend=fwd.target_end.cpu().numpy(), cigar=k.decode_cigars(tbk),
+               rs=tbk.read_start.cpu().numpy(), ts=tbk.ref_start.cpu().numpy(),
+               counts=tbk.counts.cpu().numpy())
+    _check_against_oracle(qs, ts, offs, 751, (2, 4, 4, 2), got, f"w1503_{tile}")
+
+
 @pytest.mark.parametrize("fused", [False, True])
 def test_sw_scalar_cell_tile_matches_oracle_and_9x64(fused):
     """W = 513 as 8 x 64 lanes plus the scalar cell (diagonal 512 on the scalar
