@@ -43,7 +43,7 @@ class SwParams(ctypes.Structure):
                 ("gap_extend", _i32), ("half_band", _i32), ("flags", ctypes.c_uint32),
                 ("cells_per_lane", _i32), ("lanes_per_pair", _i32),
                 ("split_rows", _i32), ("split_slices", _i32), ("prio_rows", _i32),
-                ("reserved", _i32)]
+                ("max_waves", _i32)]
 
 
 class SwBatch(ctypes.Structure):

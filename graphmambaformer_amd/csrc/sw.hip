@@ -51,6 +51,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <mutex>
 
 namespace gmfa {
 
@@ -928,37 +929,37 @@ __device__ __forceinline__ uint64_t slice_forward(const SwKArgs &a, int s, int q
     return key;
 }
 
-// Phase 1 of a split launch (see the comment above slice_forward): the
-// workgroup's KSL wavefronts claim one pair; if its query has at least
-// split_rows rows they run it as KSL slices (raised wave priority, so the
-// other waves on their SIMDs fill the slices' stalls instead of delaying
-// them), wavefront 0 walks it and the others go on at once.  A shorter pair
-// goes back to wavefront 0 (the return value; -1 for the other wavefronts or
-// when the counter ran out).  At most one split pair per workgroup: a longer
-// pair claimed later runs on one wavefront.  CW: the regular tile's cells
-// per lane (the plane layout the walk reads).
+// The split kernel (see the comment above slice_forward): a workgroup of KSL
+// wavefronts counts itself resident (ctl[2]), then claims the next split pair
+// (ctl[0], up to ctl[1] pairs, the longest-first prefix of the batch's order
+// that sw_split_count_kernel found), runs it as KSL slices at raised wave priority (so the plain
+// kernel's waves on the same SIMDs fill the slices' stalls instead of
+// delaying them), and wavefront 0 walks it while the others wait for the next
+// claim.  CW: the regular tile's cells per lane (the plane layout the walk
+// reads).
 template <int CS, int KSL, int KSS, int CW>
-__device__ __forceinline__ int split_phase(const SwKArgs &a)
+__global__ __launch_bounds__(64 * KSL) void sw_split_kernel(SwKArgs a, uint32_t *ctl)
 {
     __shared__ int xch[2 * (KSL + 2) * kXw];
-    __shared__ int claim;
+    __shared__ int claim[2];
     __shared__ unsigned long long keys[KSL];
     // wave-uniform (from threadIdx.x the compiler would treat the slice index
     // as divergent and mask every slice-dependent branch)
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int lane = (int)(threadIdx.x & 63);
-    int carry = -1;
-    {
-        if (threadIdx.x == 0) claim = (int)atomicAdd(a.counter, 1u);
-        __syncthreads();
-        const int item = __builtin_amdgcn_readfirstlane(claim);
-        if (item >= a.n_pairs) return -1;
-        const int p = __builtin_amdgcn_readfirstlane(a.order ? a.order[item] : item);
-        const int qlen = __builtin_amdgcn_readfirstlane(a.qlen[p]);
-        if (qlen < a.split_rows) {
-            carry = item;
-            return wave == 0 ? carry : -1;
+    const int lane = lane_id();
+    // resident: the plain launch waits for every split workgroup to be here
+    if (threadIdx.x == 0) atomicAdd(&ctl[2], 1u);
+    for (;;) {
+        __syncthreads();  // the previous pair (wavefront 0's walk included) is done
+        if (threadIdx.x == 0) {
+            claim[0] = (int)atomicAdd(&ctl[0], 1u);
+            claim[1] = (int)__hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(claim[0]);
+        if (item >= __builtin_amdgcn_readfirstlane(claim[1])) return;
+        const int p = __builtin_amdgcn_readfirstlane(a.order[item]);
+        const int qlen = __builtin_amdgcn_readfirstlane(a.qlen[p]);
         const int tlen = __builtin_amdgcn_readfirstlane(a.tlen[p]);
         const int off = __builtin_amdgcn_readfirstlane(a.boff[p]);
         const int64_t qo = uni64(a.q_off[p]), to = uni64(a.t_off[p]);
@@ -992,7 +993,45 @@ __device__ __forceinline__ int split_phase(const SwKArgs &a)
         }
         __builtin_amdgcn_s_setprio(0);
     }
-    return -1;
+}
+
+// Before a split launch: n_split = the length of the longest prefix (at most
+// cap) of the longest-first order whose queries have at least split_rows
+// rows.  The plain kernel's work counter starts after it (ctl[0] of the plain
+// launch = counter[0]), the split kernel's at 0 with ctl[1] = n_split.  One
+// workgroup of 256 threads.
+__global__ __launch_bounds__(256) void sw_split_count_kernel(const int32_t *qlen,
+                                                             const int32_t *order, int32_t n,
+                                                             int32_t split_rows, int32_t cap,
+                                                             uint32_t *counter)
+{
+    __shared__ int first_short;
+    if (threadIdx.x == 0) first_short = cap;
+    __syncthreads();
+    for (int k = threadIdx.x; k < cap && k < n; k += 256)
+        if (qlen[order[k]] < split_rows) atomicMin(&first_short, k);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int ns = min(first_short, n);
+        counter[0] = (uint32_t)ns;  // the plain kernel's first item
+        counter[1] = 0u;            // the split kernel's next item
+        counter[2] = (uint32_t)ns;  // split items
+        counter[3] = 0u;            // split workgroups resident
+    }
+}
+
+// Between the count kernel and the plain kernel on the caller's stream: waits
+// until all `groups` split workgroups (launched on the side stream) are
+// resident, at most deadline_ticks of the constant-rate wall clock, so the
+// plain kernel's wavefronts never take the slots the split workgroups need.
+// One thread.
+__global__ void sw_split_wait_kernel(const uint32_t *started, uint32_t groups,
+                                     uint64_t deadline_ticks)
+{
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(started, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < groups &&
+           (uint64_t)wall_clock64() - t0 < deadline_ticks)
+        __builtin_amdgcn_s_sleep(2);
 }
 
 // SV: serving variant (persistent queue fed by the host; staged requests of
@@ -1005,24 +1044,17 @@ __device__ __forceinline__ int split_phase(const SwKArgs &a)
 // register 0 of the plane layout) computed on the scalar unit from lane 63's
 // prefix and window, while the vector unit, the bound of this kernel, does
 // 8 cells per lane instead of 9 with 7 dead lanes.
-//
-// SPL > 1: a split launch (workgroups of SPL wavefronts; phase 1 runs pairs of
-// at least split_rows query rows as SPL slices of CS cells per lane, KSS the
-// compile-time register of diagonal W-1 of a slice tile), then each
-// wavefront continues below on its own.
-template <int C, int G, int KS, bool TB, bool NM, bool RK, bool SV = false, int SPL = 1,
-          int CS = 1, int KSS = -1>
-__global__ __launch_bounds__(64 * SPL) void sw_forward_kernel(SwKArgs a)
+template <int C, int G, int KS, bool TB, bool NM, bool RK, bool SV = false>
+__global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 {
     constexpr bool XS = KS == C;
     static_assert(!XS || (G == 64 && C + 8 <= 32), "scalar-cell tiles: 64 lanes, C <= 24");
     constexpr int NCW = (C + 15) / 16;  // target words feeding the cells
     constexpr int NA = NCW + 1;         // aligned window words
     constexpr int NWR = NA + 1;         // raw words loaded per block
-    // the lane index: threadIdx.x for one-wavefront workgroups (as lane_id(),
-    // or threadIdx.x & 63, the kernel took 256 VGPRs + AGPRs instead of 202 on
-    // 24 x 64), the mbcnt lane id in split launches
-    const int lane = SPL == 1 ? (int)threadIdx.x : lane_id();
+    // (as lane_id() or threadIdx.x & 63 the 24 x 64 tile took 256 VGPRs +
+    // AGPRs instead of 202)
+    const int lane = (int)threadIdx.x;
     const int grp = lane / G;
     const int gl = lane % G;
 
@@ -1049,11 +1081,6 @@ __global__ __launch_bounds__(64 * SPL) void sw_forward_kernel(SwKArgs a)
         // every class runs side by side)
         if (blockIdx.x == 0 && threadIdx.x == 0 && a.serve_started)
             __hip_atomic_store(a.serve_started, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-    int carry = -1;
-    if constexpr (SPL > 1) {
-        static_assert(G == 64 && TB && !NM && !RK && !SV, "split launches: fused walk, 64-lane tile");
-        carry = split_phase<CS, SPL, KSS, C>(a);
     }
     for (;;) {
         int item = 0;
@@ -1116,13 +1143,8 @@ __global__ __launch_bounds__(64 * SPL) void sw_forward_kernel(SwKArgs a)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         } else {
-            if (carry >= 0) {
-                item = carry;  // the pair that ended this wavefront's split phase
-                carry = -1;
-            } else {
-                if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
-                item = __builtin_amdgcn_readfirstlane(item);
-            }
+            if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
+            item = __builtin_amdgcn_readfirstlane(item);
             // serving: one pair per wavefront (G == 64), bounded by what the host
             // has published
             if constexpr (SV) {
@@ -1966,13 +1988,14 @@ static const TileEntry *default_tile(int W)
 
 // Split launches: (slice cells per lane CS, slices K, compile-time register
 // of diagonal W-1 in a slice tile) over a regular 64-lane tile (C, KS)
+using SplitKernel = void (*)(SwKArgs, uint32_t *);
+
 struct SplitEntry {
     int CS, K, KSS, C, KS;
-    SwKernel k;
+    SplitKernel k;
 };
 
-#define GMFA_SPLIT(CS, K, KSS, C, KS) \
-    {CS, K, KSS, C, KS, &sw_forward_kernel<C, 64, KS, true, false, false, false, K, CS, KSS>}
+#define GMFA_SPLIT(CS, K, KSS, C, KS) {CS, K, KSS, C, KS, &sw_split_kernel<CS, K, KSS, C>}
 // W = 1503 (half_band 751) over 24 x 64: two slices of 12 x 64, three of
 // 8 x 64 or four of 6 x 64
 static const SplitEntry kSplits[] = {
@@ -1981,6 +2004,59 @@ static const SplitEntry kSplits[] = {
     GMFA_SPLIT(6, 4, 2, 24, 14),
 };
 #undef GMFA_SPLIT
+
+// split workgroups of a launch, one pair each: at most this many of the
+// longest pairs run split, longer-than-split_rows pairs beyond them run plain
+constexpr int kSplitGroups = 256;
+
+// The library's side stream and fork / join events per device (created on
+// first use, kept until the process ends): the split kernel runs on it beside
+// the plain kernel on the caller's stream.  The mutex keeps one launch's
+// record / wait / launch sequence whole when host threads launch at once.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int kMaxSideDevices = 64;
+static SideStream g_side[kMaxSideDevices];
+static std::mutex g_side_mu;
+
+// count kernel + plain kernel on s, split kernel on the side stream, joined
+// back into s (a caller capturing s into a HIP graph captures all of it)
+static int split_launch(const SplitEntry &e, int groups, const SwKArgs &a, SwKernel plain,
+                        int plain_waves, const gmfa_sw_batch *batch, int split_rows, hipStream_t s)
+{
+    int dev = 0;
+    GMFA_CHECK_HIP(hipGetDevice(&dev));
+    GMFA_REQUIRE(dev >= 0 && dev < kMaxSideDevices, "device %d beyond the %d the library tracks",
+                 dev, kMaxSideDevices);
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    SideStream &ss = g_side[dev];
+    if (!ss.s) {
+        GMFA_CHECK_HIP(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
+        GMFA_CHECK_HIP(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
+        GMFA_CHECK_HIP(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
+    }
+    hipLaunchKernelGGL(sw_split_count_kernel, dim3(1), dim3(256), 0, s, batch->qlen, batch->order,
+                       batch->n_pairs, split_rows, groups, a.counter);
+    GMFA_CHECK_HIP(hipGetLastError());
+    GMFA_CHECK_HIP(hipEventRecord(ss.fork, s));
+    GMFA_CHECK_HIP(hipStreamWaitEvent(ss.s, ss.fork, 0));
+    hipLaunchKernelGGL(e.k, dim3(groups), dim3(64 * e.K), 0, ss.s, a, a.counter + 1);
+    GMFA_CHECK_HIP(hipGetLastError());
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        rate_khz <= 0)
+        rate_khz = 100000;
+    hipLaunchKernelGGL(sw_split_wait_kernel, dim3(1), dim3(1), 0, s, a.counter + 3,
+                       (uint32_t)groups, (uint64_t)rate_khz * 5);  // 5 ms
+    GMFA_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(plain, dim3(plain_waves), dim3(64), 0, s, a);
+    GMFA_CHECK_HIP(hipGetLastError());
+    GMFA_CHECK_HIP(hipEventRecord(ss.join, ss.s));
+    GMFA_CHECK_HIP(hipStreamWaitEvent(s, ss.join, 0));
+    return GMFA_OK;
+}
 
 static bool split_serves(const SplitEntry &e, int W, const gmfa_sw_tile &t)
 {
@@ -2185,6 +2261,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, 2 * P.half_band + 1);
     SwKernel k = serve ? t->serve(P.flags) : t->get(P.flags);
     const SplitEntry *split = nullptr;
+    int split_groups = 0;
 
     SwKArgs a;
     a.qseq = batch->qseq; a.tseq = batch->tseq; a.qmask = batch->qmask; a.tmask = batch->tmask;
@@ -2231,6 +2308,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     int waves = 1;
     if (!serve) {
         waves = std::max(1, std::min(a.n_items, resident_waves(k)));
+        if (P.max_waves > 0) waves = std::min(waves, P.max_waves);
         // split launch: the longest pairs over several wavefronts each (needs
         // the fused walk, no N masks or row keys, a 64-lane tile and a
         // longest-first order); bands without a split variant run plainly
@@ -2244,18 +2322,12 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
             }
         }
         if (split) {
-            k = split->k;
-            int dev = 0, cus = 256, nb = 0;
-            GMFA_CHECK_HIP(hipGetDevice(&dev));
-            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-                hipSuccess)
-                cus = 256;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k, 64 * split->K,
-                                                              0) != hipSuccess ||
-                nb <= 0)
-                nb = 1;
+            // the plain kernel keeps its full grid: sw_split_wait_kernel holds
+            // it back until the split workgroups are resident, its wavefronts
+            // beyond the free slots start as split workgroups retire (those
+            // without a split pair at once)
+            split_groups = std::min(kSplitGroups, n);
             a.split_rows = P.split_rows;
-            waves = std::max(1, std::min(cus * nb, n));  // workgroups of split->K wavefronts
         }
         if (slabs) {
             // one slab per pair group, sized for the longest query (rounded
@@ -2340,11 +2412,13 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
         a.counter = reinterpret_cast<uint32_t *>(a.dctl);
         hipLaunchKernelGGL(reset_serve_ctl_kernel, dim3(1), dim3(64), 0, s, a.dctl);
         GMFA_CHECK_HIP(hipGetLastError());
+    } else if (split) {
+        return split_launch(*split, split_groups, a, k, waves, batch, P.split_rows, s);
     } else if (!serve) {
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
         GMFA_CHECK_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k, dim3(waves), dim3(split ? 64 * split->K : 64), 0, s, a);
+    hipLaunchKernelGGL(k, dim3(waves), dim3(64), 0, s, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }

@@ -255,7 +255,7 @@ def make_workspace(inp: SwInputs, tile: SwTile, traceback: bool = True,
     repeated passes over a resident batch allocate nothing."""
     dev = inp.device
     n = inp.n_pairs
-    ws = {"counter": torch.zeros(1, dtype=torch.int32, device=dev)}
+    ws = {"counter": torch.zeros(4, dtype=torch.int32, device=dev)}  # 4 words: split launches
     if traceback:
         nb = tb_bytes(inp.qlen_host, tile)
         offs = np.zeros(n + 1, dtype=np.int64)
@@ -287,7 +287,7 @@ class SwForward:
 
 
 def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile, split_rows: int = 0,
-                split_slices: int = 0, prio_rows: int = 0) -> SwParams:
+                split_slices: int = 0, prio_rows: int = 0, max_waves: int = 0) -> SwParams:
     p = SwParams()
     p.match, p.mismatch, p.gap_open, p.gap_extend = int(ma), int(mi), int(go), int(ge)
     p.half_band = int(half_band)
@@ -297,6 +297,7 @@ def make_params(ma, mi, go, ge, half_band, flags, tile: SwTile, split_rows: int 
     p.split_rows = int(split_rows)
     p.split_slices = int(split_slices)
     p.prio_rows = int(prio_rows)
+    p.max_waves = int(max_waves)
     return p
 
 
@@ -335,14 +336,15 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
                traceback: bool = True, rowkeys: bool = False,
                order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
                workspace: Optional[dict] = None, cigar: bool = False, split_rows: int = 0,
-               split_slices: int = 0, prio_rows: int = 0) -> SwForward:
+               split_slices: int = 0, prio_rows: int = 0, max_waves: int = 0) -> SwForward:
     """Forward pass.  ``cigar=True`` also walks every pair inside the same
     launch (GMFA_SW_CIGAR) and returns the CIGARs in ``.walk``.
     ``split_rows > 0`` (with ``cigar``, ``order`` and no N masks): pairs of at
     least that many query rows run on several wavefronts each, a slice of the
     band per wavefront (``split_slices`` of them, 0 = the first compiled
     count); the outputs are those of the plain launch.  ``prio_rows > 0``:
-    pairs of at least that many query rows run at raised wave priority."""
+    pairs of at least that many query rows run at raised wave priority.
+    ``max_waves > 0`` caps the persistent launch's wavefronts."""
     dev = inp.device
     n = inp.n_pairs
     tile = tile or batch_tile(half_band, n, dev)
@@ -381,7 +383,9 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
         rk_off_host = offs
     counter = ws.get("counter")
     if counter is None:
-        counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        counter = torch.zeros(4, dtype=torch.int32, device=dev)
+    if split_rows and counter.numel() < 4:
+        raise ValueError("split launches need a work counter of 4 int32 words")
     out = SwOut()
     out.score, out.query_end, out.target_end = ptr(score), ptr(qend), ptr(tend)
     out.tb, out.tb_off = ptr(tb), ptr(tb_off)
@@ -394,11 +398,11 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
         out.n_cigar, out.read_start = ptr(walk.n_cigar), ptr(walk.read_start)
         out.ref_start, out.counts = ptr(walk.ref_start), ptr(walk.counts)
     params = make_params(ma, mi, go, ge, half_band, flags, tile, split_rows, split_slices,
-                         prio_rows)
+                         prio_rows, max_waves)
     batch = inp.batch_struct(order)
     check(lib().gmfa_sw_forward(ctypes.byref(batch), ctypes.byref(params), ctypes.byref(out),
                                 stream_of(dev)), "gmfa_sw_forward")
-    params.split_rows = params.split_slices = params.prio_rows = 0
+    params.split_rows = params.split_slices = params.prio_rows = params.max_waves = 0
     params.flags = int(flags) & ~_lib.SW_CIGAR  # a later standalone walk uses the same params
     return SwForward(score=score, query_end=qend, target_end=tend, tb=tb, tb_off=tb_off,
                      rowkey=rk, rowkey_off_host=rk_off_host, tile=tile, params=params, walk=walk)
@@ -464,7 +468,7 @@ class SwGraph:
     def __init__(self, inp: SwInputs, scoring: tuple, half_band: int,
                  order: Optional[torch.Tensor] = None, tile: Optional[SwTile] = None,
                  workspace: Optional[dict] = None, cigar: bool = True, split_rows: int = 0,
-                 split_slices: int = 0, prio_rows: int = 0):
+                 split_slices: int = 0, prio_rows: int = 0, max_waves: int = 0):
         dev = inp.device
         self.tile = tile or batch_tile(half_band, inp.n_pairs, dev)
         self.ws = workspace if workspace is not None else make_workspace(inp, self.tile,
@@ -474,7 +478,7 @@ class SwGraph:
             return sw_forward(inp, *scoring, half_band, traceback=True, order=order,
                               tile=self.tile, workspace=self.ws, cigar=cigar,
                               split_rows=split_rows, split_slices=split_slices,
-                              prio_rows=prio_rows)
+                              prio_rows=prio_rows, max_waves=max_waves)
 
         # one eager pass on a side stream first (module load, allocator warm-up)
         side = torch.cuda.Stream(device=dev)
@@ -821,11 +825,15 @@ def cigar_bam(cigar: torch.Tensor, cigar_off: torch.Tensor, n_cigar: torch.Tenso
 
 def autotune_launch(inp: SwInputs, half_band: int, scoring: tuple, order: torch.Tensor,
                     tile: Optional[SwTile] = None, workspace: Optional[dict] = None,
-                    tops=(16, 64, 256, 1024), reps: int = 3, margin: float = 0.03):
+                    tops=(64, 128, 192, 256), reps: int = 3, margin: float = 0.03):
     """Choose how a batch of uneven lengths is launched: the plain launch,
     the plain launch with the ``k`` longest pairs at raised wave priority
     (``prio_rows``), or a split launch (the ``k`` longest pairs over several
-    wavefronts, every compiled split variant), for ``k`` in ``tops``.  Each
+    wavefronts, every compiled split variant), for ``k`` in ``tops`` (a split
+    launch only shortens the tail once it takes every pair whose cells are
+    well above one SIMD's share of the batch: on the noisy 8,192-pair batch
+    the 40 longest pairs are all ~100 kb, and splitting fewer than ~100 pairs
+    measured no faster than the plain launch).  Each
     candidate is timed on the whole batch itself (the tail these options
     shorten is a property of the whole batch, not of a sample), the median of
     ``reps`` launches (single launches of one configuration spread by several

@@ -88,11 +88,16 @@ typedef struct gmfa_sw_params {
     int32_t lanes_per_pair; /* 0 = library default; else 16, 32 or 64 */
     /* Split launch (gmfa_sw_forward with exactly GMFA_SW_TRACEBACK |
      * GMFA_SW_CIGAR, a 64-lane tile and batch->order set): split_rows > 0 runs
-     * every pair whose query has at least split_rows rows on a workgroup of
-     * several wavefronts, each a slice of the band, and the rest one pair per
-     * wavefront, in one launch; outputs and planes are those of the plain
-     * launch.  split_slices: 0 = the first compiled slice count for the band,
-     * else that count.  Bands without a compiled split variant run the plain
+     * the longest-first prefix of batch->order whose queries have at least
+     * split_rows rows (at most 256 pairs) each on a workgroup of several
+     * wavefronts, one band slice per wavefront, in a kernel on the library's
+     * own stream beside the plain kernel for the rest (forked from and joined
+     * back into the caller's stream with events, so a graph capture of the
+     * caller's stream holds both; the plain kernel waits, at most 5 ms, until
+     * the split workgroups are resident); out->work_counter must then point
+     * to 4 uint32.  Outputs and planes are those of the plain launch.
+     * split_slices: 0 = the first compiled slice count for the band, else
+     * that count.  Bands without a compiled split variant run the plain
      * launch.  0 / 0: the plain launch. */
     int32_t split_rows;
     int32_t split_slices;
@@ -100,7 +105,9 @@ typedef struct gmfa_sw_params {
      * run at raised wave priority, so the longest pairs of an uneven batch
      * keep their SIMD's issue slots and the other waves fill their stalls. */
     int32_t prio_rows;
-    int32_t reserved;
+    /* max_waves > 0 (gmfa_sw_forward): the persistent launch holds at most this
+     * many wavefronts (default: as many as fit on the device at once). */
+    int32_t max_waves;
 } gmfa_sw_params;
 
 /* A batch of pairs whose sequences live in 2-bit packed streams
@@ -133,7 +140,8 @@ typedef struct gmfa_sw_out {
                               key of row i = H << 11 | (2047 - d) for the row's
                               maximum H at its first band diagonal d */
     const int64_t *rowkey_off; /* [n_pairs] element offset of each pair's keys */
-    uint32_t *work_counter;    /* one device uint32 of scratch for the work queue */
+    uint32_t *work_counter;    /* one device uint32 of scratch for the work queue (4 for
+                                  split launches) */
     /* GMFA_SW_CIGAR outputs, same layout as gmfa_sw_traceback's */
     const int64_t *cigar_off;
     uint32_t *cigar;

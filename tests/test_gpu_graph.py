@@ -103,3 +103,36 @@ def test_graph_replays_match_eager_and_oracle(hb, off):
     torch.cuda.synchronize()
     for rep in range(2):
         _same(_outputs(g.replay()), eager2, f"new pairs, replay {rep}")
+
+
+@pytest.mark.parametrize("slices", [2, 4])
+def test_graph_of_a_split_launch_matches_eager(slices):
+    """A split launch (the longest pairs on the library's side stream, forked
+    from and joined back into the captured stream) captured in a HIP graph:
+    every replay, with eager work between replays, equals the eager split
+    launch and the plain launch."""
+    from graphmambaformer_amd import kernels as K
+
+    _say(f"case: graph of a split launch, {slices} slices")
+    rng = np.random.default_rng(40 + slices)
+    lens = np.concatenate([rng.integers(3000, 9000, size=20), rng.integers(100, 2000, size=200)])
+    qs, ts = _batch(rng, lens, 751, 0)
+    boff = np.zeros(len(qs), np.int32)
+    inp = K.upload_pairs(qs, ts, boff, DEV, alphabet="2bit")
+    tile = K.resolve_tile(751)
+    order = torch.from_numpy(np.argsort(-inp.qlen_host, kind="stable").astype(np.int32)).to(DEV)
+    rows = int(np.sort(inp.qlen_host)[::-1][11])  # the 12 longest pairs split
+    plain = _outputs(K.sw_forward(inp, *SC, 751, traceback=True, order=order, tile=tile,
+                                  cigar=True))
+    eager = _outputs(K.sw_forward(inp, *SC, 751, traceback=True, order=order, tile=tile,
+                                  cigar=True, split_rows=rows, split_slices=slices))
+    _same(eager, plain, "eager split vs plain")
+    g = K.SwGraph(inp, SC, 751, order=order, tile=tile, cigar=True, split_rows=rows,
+                  split_slices=slices)
+    for rep in range(4):
+        K.sw_forward(inp, *SC, 751, traceback=True, order=order, tile=tile, cigar=True,
+                     workspace=K.make_workspace(inp, tile))
+        x = torch.randn(4096, device=DEV).cumsum(0)
+        torch.cuda.synchronize()
+        _same(_outputs(g.replay()), plain, f"split replay {rep}")
+        assert torch.isfinite(x).all()
