@@ -52,6 +52,7 @@
 #include <algorithm>
 #include <climits>
 #include <mutex>
+#include <vector>
 
 namespace gmfa {
 
@@ -2014,15 +2015,23 @@ constexpr int kSplitGroups = 256;
 // the plain kernel on the caller's stream.  The mutex keeps one launch's
 // record / wait / launch sequence whole when host threads launch at once.
 struct SideStream {
-    hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t s = nullptr;  // non-blocking: the split kernel under graph capture
+    // Eager launches: the split kernel on qa, the plain kernel on qb.  Both are
+    // created with an all-CU mask, which gives each a hardware queue of its own
+    // (queues are otherwise shared once a process holds more streams than the
+    // runtime's queue pool, and two kernels on one queue run one after the
+    // other).  Such streams are blocking, so with the null stream as the caller
+    // neither may be the caller's stream.
+    hipStream_t qa = nullptr, qb = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
 };
 constexpr int kMaxSideDevices = 64;
 static SideStream g_side[kMaxSideDevices];
 static std::mutex g_side_mu;
 
-// count kernel + plain kernel on s, split kernel on the side stream, joined
-// back into s (a caller capturing s into a HIP graph captures all of it)
+// Count kernel on s; eager: split kernel on qa, wait + plain kernels on qb;
+// under capture: split kernel on the side stream, wait + plain kernels on s;
+// joined back into s (a caller capturing s into a HIP graph captures all of it)
 static int split_launch(const SplitEntry &e, int groups, const SwKArgs &a, SwKernel plain,
                         int plain_waves, const gmfa_sw_batch *batch, int split_rows, hipStream_t s)
 {
@@ -2033,28 +2042,46 @@ static int split_launch(const SplitEntry &e, int groups, const SwKArgs &a, SwKer
     std::lock_guard<std::mutex> lk(g_side_mu);
     SideStream &ss = g_side[dev];
     if (!ss.s) {
+        int cus = 0;
+        GMFA_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        GMFA_REQUIRE(cus > 0 && cus <= 4096, "device reports %d compute units", cus);
+        std::vector<uint32_t> mask((cus + 31) / 32, ~0u);
+        if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
+        GMFA_CHECK_HIP(hipExtStreamCreateWithCUMask(&ss.qa, (uint32_t)mask.size(), mask.data()));
+        GMFA_CHECK_HIP(hipExtStreamCreateWithCUMask(&ss.qb, (uint32_t)mask.size(), mask.data()));
         GMFA_CHECK_HIP(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
         GMFA_CHECK_HIP(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
         GMFA_CHECK_HIP(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
+        GMFA_CHECK_HIP(hipEventCreateWithFlags(&ss.join2, hipEventDisableTiming));
     }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    GMFA_CHECK_HIP(hipStreamIsCapturing(s, &cap));
+    const bool eager = cap == hipStreamCaptureStatusNone;
+    hipStream_t sa = eager ? ss.qa : ss.s;  // the split kernel
+    hipStream_t sb = eager ? ss.qb : s;     // the wait kernel and the plain kernel
     hipLaunchKernelGGL(sw_split_count_kernel, dim3(1), dim3(256), 0, s, batch->qlen, batch->order,
                        batch->n_pairs, split_rows, groups, a.counter);
     GMFA_CHECK_HIP(hipGetLastError());
     GMFA_CHECK_HIP(hipEventRecord(ss.fork, s));
-    GMFA_CHECK_HIP(hipStreamWaitEvent(ss.s, ss.fork, 0));
-    hipLaunchKernelGGL(e.k, dim3(groups), dim3(64 * e.K), 0, ss.s, a, a.counter + 1);
+    GMFA_CHECK_HIP(hipStreamWaitEvent(sa, ss.fork, 0));
+    if (eager) GMFA_CHECK_HIP(hipStreamWaitEvent(sb, ss.fork, 0));
+    hipLaunchKernelGGL(e.k, dim3(groups), dim3(64 * e.K), 0, sa, a, a.counter + 1);
     GMFA_CHECK_HIP(hipGetLastError());
     int rate_khz = 0;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
         rate_khz <= 0)
         rate_khz = 100000;
-    hipLaunchKernelGGL(sw_split_wait_kernel, dim3(1), dim3(1), 0, s, a.counter + 3,
+    hipLaunchKernelGGL(sw_split_wait_kernel, dim3(1), dim3(1), 0, sb, a.counter + 3,
                        (uint32_t)groups, (uint64_t)rate_khz * 5);  // 5 ms
     GMFA_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(plain, dim3(plain_waves), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(plain, dim3(plain_waves), dim3(64), 0, sb, a);
     GMFA_CHECK_HIP(hipGetLastError());
-    GMFA_CHECK_HIP(hipEventRecord(ss.join, ss.s));
+    GMFA_CHECK_HIP(hipEventRecord(ss.join, sa));
     GMFA_CHECK_HIP(hipStreamWaitEvent(s, ss.join, 0));
+    if (eager) {
+        GMFA_CHECK_HIP(hipEventRecord(ss.join2, sb));
+        GMFA_CHECK_HIP(hipStreamWaitEvent(s, ss.join2, 0));
+    }
     return GMFA_OK;
 }
 
