@@ -398,6 +398,18 @@ __device__ __forceinline__ void walk_pair_paged(const WalkCtx &c, const WalkOut 
     }
 }
 
+// A store into host-coherent memory (the online rings): system scope, so it
+// writes through the L2.  A release at system scope writes the L2's dirty
+// lines back, but the code generated for the completion record's release
+// store has been seen to issue that write-back and then the record's store
+// without waiting in between, so a host that saw the record could still read
+// the entry's old contents; write-through stores, waited for by the release
+// fence before the record, leave nothing to write back.
+__device__ __forceinline__ void host_store(void *p, uint32_t v)
+{
+    __hip_atomic_store(static_cast<uint32_t *>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Online serving walk: the runs go straight to the request's ring entry in
 // host memory.  Run x (counted from the alignment's end) belongs at
 // cig[cap - 1 - x], so the entry ends with the runs in forward order and no
@@ -423,14 +435,14 @@ __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, 
         if (lane == (x & 63)) buf = vrun;
         if ((x & 63) == 63) {
             const int y = cap - 1 - (x - 63 + lane);
-            if (y >= 0) cig[y] = buf;
+            if (y >= 0) host_store(cig + y, buf);
         }
     };
     const int score = h, qend = i, tend = j;
     walk_steps<C, true, NM>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
     if (lane < (nr & 63)) {
         const int y = cap - 1 - ((nr & ~63) + lane);
-        if (y >= 0) cig[y] = buf;
+        if (y >= 0) host_store(cig + y, buf);
     }
     const uint64_t done_tick = (uint64_t)wall_clock64();
     const int v = lane == 0 ? score : lane == 1 ? qend : lane == 2 ? tend : lane == 3 ? i
@@ -438,7 +450,7 @@ __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, 
                 : lane == 8 ? cnt[2] : lane == 9 ? cnt[3]
                 : lane == 10 ? (int)(uint32_t)claim_tick : lane == 11 ? (int)(claim_tick >> 32)
                 : lane == 12 ? (int)(uint32_t)done_tick : (int)(done_tick >> 32);
-    if (lane < 14) reinterpret_cast<int32_t *>(res)[lane] = v;
+    if (lane < 14) host_store(reinterpret_cast<uint32_t *>(res) + lane, (uint32_t)v);
 }
 
 // walk_pair_ring for paged planes.
@@ -460,14 +472,14 @@ __device__ __forceinline__ void walk_pair_ring_paged(const WalkCtx &c, uint32_t 
         if (lane == (x & 63)) buf = vrun;
         if ((x & 63) == 63) {
             const int y = cap - 1 - (x - 63 + lane);
-            if (y >= 0) cig[y] = buf;
+            if (y >= 0) host_store(cig + y, buf);
         }
     };
     const int score = h, qend = i, tend = j;
     walk_steps_paged<C>(c, i, j, h, off, qo, to, ldp, ld, put, nr, cnt);
     if (lane < (nr & 63)) {
         const int y = cap - 1 - ((nr & ~63) + lane);
-        if (y >= 0) cig[y] = buf;
+        if (y >= 0) host_store(cig + y, buf);
     }
     const uint64_t done_tick = (uint64_t)wall_clock64();
     const int v = lane == 0 ? score : lane == 1 ? qend : lane == 2 ? tend : lane == 3 ? i
@@ -475,7 +487,7 @@ __device__ __forceinline__ void walk_pair_ring_paged(const WalkCtx &c, uint32_t 
                 : lane == 8 ? cnt[2] : lane == 9 ? cnt[3]
                 : lane == 10 ? (int)(uint32_t)claim_tick : lane == 11 ? (int)(claim_tick >> 32)
                 : lane == 12 ? (int)(uint32_t)done_tick : (int)(done_tick >> 32);
-    if (lane < 14) reinterpret_cast<int32_t *>(res)[lane] = v;
+    if (lane < 14) host_store(reinterpret_cast<uint32_t *>(res) + lane, (uint32_t)v);
 }
 
 struct TbKArgs {
@@ -1552,10 +1564,12 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                                       a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s, off,
                                       qo, to, a.tb + tb_base, claim_tick);
                 }
-                if (nopages && lane == 0) a.ring_res[ring_r].n_cigar = -1;
+                if (nopages && lane == 0) host_store(&a.ring_res[ring_r].n_cigar, ~0u);
                 // every output of the request (stored by the wave's lanes) before
-                // its done word and its completion-queue record
+                // its done word and its completion-queue record: the system
+                // release fence waits for the write-through stores above
                 __syncthreads();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
                 if (lane == 0) {
                     // the done word needs no release of its own: the record's
                     // release below orders it and the outputs before the record

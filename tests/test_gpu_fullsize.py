@@ -1,16 +1,9 @@
 """The long-read (16,384 pairs, half_band 256) and noisy (8,192 pairs, half_band
 751) batches at their full sizes on the GPU, checked through properties that
-hold at any size, over every pair:
-
-* each CIGAR consumes exactly the query span [read_start, query_end) and the
-  window span [ref_start, target_end) it reports, and its op counts match;
-* every '=' / 'X' column agrees with the actual bases (checked on the device);
-* rescoring each CIGAR with the affine scheme (a gap of length L costs
-  open + (L-1) * extend, extension.py:154-156) gives exactly the reported
-  score, and a positive-score local alignment begins and ends with '=';
-* the results do not depend on the work order (longest-first vs natural);
-
-and every pair bit-for-bit against the oracle's outputs over the whole batch
+hold at any size, over every pair (tests/alignment_checks.py: span
+consumption, '=' / 'X' columns against the bases, affine rescoring equal to
+the score), with results that do not depend on the work order (longest-first
+vs natural), and every pair bit-for-bit against the oracle's outputs over the whole batch
 (score, ends, starts and a CIGAR digest per pair, tests/golden/
 fullsize_digest.npz, written by tests/golden/make_fullsize_digest.py), for
 the plain launch and, on the noisy batch, for split launches (the longest
@@ -25,6 +18,7 @@ import pytest
 import torch
 
 from oracle import oracle as O
+from tests.alignment_checks import check_alignments
 from tests.golden import make_fullsize_digest as D
 
 DIGEST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
@@ -67,65 +61,18 @@ def _against_digest(config, fwd, label):
 
 
 def _properties(w, inp, fwd, label):
-    n = w.n_pairs
+    """Every pair through the size-independent checks of alignment_checks."""
     tbk = fwd.walk
-    score = fwd.score.long()
-    qe, te = fwd.query_end.long(), fwd.target_end.long()
-    rs, ts = tbk.read_start.long(), tbk.ref_start.long()
-    nc = tbk.n_cigar.long()
-    assert (score >= 0).all() and (nc >= 0).all(), label
-    total = int(nc.sum())
-    pair = torch.arange(n, device=DEV)
-    owner = torch.repeat_interleave(pair, nc)
-    first = torch.cumsum(nc, 0) - nc
-    within = torch.arange(total, device=DEV) - first[owner]
-    runs = tbk.cigar[tbk.cigar_off.long()[owner] + within].long() & 0xFFFFFFFF
-    op, ln = runs & 15, runs >> 4
-    assert (ln >= 1).all(), label
-    assert torch.isin(op, torch.tensor([1, 2, 7, 8], device=DEV)).all(), label
-    is_eq, is_x, is_i, is_d = op == 7, op == 8, op == 1, op == 2
-    zero = torch.zeros(n, dtype=torch.long, device=DEV)
-
-    def per_pair(v):
-        return zero.clone().index_add_(0, owner, v)
-
-    q_cons = torch.where(is_eq | is_x | is_i, ln, 0)
-    t_cons = torch.where(is_eq | is_x | is_d, ln, 0)
-    assert torch.equal(per_pair(q_cons), qe - rs), label
-    assert torch.equal(per_pair(t_cons), te - ts), label
-    counts = tbk.counts.long()
-    for k, m in enumerate((is_eq, is_x, is_i, is_d)):
-        assert torch.equal(per_pair(torch.where(m, ln, 0)), counts[:, k]), (label, k)
-    # rescoring
-    ma, mi, go, ge = w.match, w.mismatch, w.gap_open, w.gap_extend
-    contrib = torch.where(is_eq, ma * ln, torch.where(is_x, -mi * ln,
-                                                      -(go + (ln - 1) * ge)))
-    assert torch.equal(per_pair(contrib), score), label
-    pos = score > 0
-    last = first + nc - 1
-    assert (op[first[pos]] == 7).all() and (op[last[pos]] == 7).all(), label
-    assert (nc[~pos] == 0).all(), label
-    # '=' / 'X' columns against the bases
-    gq = torch.cumsum(q_cons, 0) - q_cons
-    gt = torch.cumsum(t_cons, 0) - t_cons
-    q_run = gq - gq[first[owner]]      # query offset of each run inside its alignment
-    t_run = gt - gt[first[owner]]
     ql = torch.tensor([len(q) for q in w.queries], dtype=torch.long, device=DEV)
     tl = torch.tensor([len(t) for t in w.targets], dtype=torch.long, device=DEV)
-    q_base = torch.cumsum(ql, 0) - ql
-    t_base = torch.cumsum(tl, 0) - tl
-    qcodes = torch.from_numpy(np.concatenate(w.queries).astype(np.uint8)).to(DEV)
-    tcodes = torch.from_numpy(np.concatenate(w.targets).astype(np.uint8)).to(DEV)
-    diag = torch.nonzero(is_eq | is_x).squeeze(1)
-    dl = ln[diag]
-    col_run = torch.repeat_interleave(diag, dl)
-    col_k = torch.arange(int(dl.sum()), device=DEV) - (torch.cumsum(dl, 0) - dl).repeat_interleave(dl)
-    p = owner[col_run]
-    qi = q_base[p] + rs[p] + q_run[col_run] + col_k
-    ti = t_base[p] + ts[p] + t_run[col_run] + col_k
-    same = qcodes[qi] == tcodes[ti]
-    assert torch.equal(same, op[col_run] == 7), label
-    return int(dl.sum())
+    return check_alignments(
+        label, q_codes=torch.from_numpy(np.concatenate(w.queries).astype(np.uint8)).to(DEV),
+        q_base=torch.cumsum(ql, 0) - ql,
+        t_codes=torch.from_numpy(np.concatenate(w.targets).astype(np.uint8)).to(DEV),
+        t_base=torch.cumsum(tl, 0) - tl, score=fwd.score, qend=fwd.query_end,
+        tend=fwd.target_end, rs=tbk.read_start, ts=tbk.ref_start, n_cigar=tbk.n_cigar,
+        cigar_off=tbk.cigar_off, cigar=tbk.cigar, counts=tbk.counts,
+        scoring=(w.match, w.mismatch, w.gap_open, w.gap_extend))
 
 
 @pytest.mark.parametrize("config", ["sw_long", "sw_noisy"])
