@@ -91,6 +91,13 @@ def scoring(w):
     return (w.match, w.mismatch, w.gap_open, w.gap_extend)
 
 
+def tile_timings(t: dict) -> dict:
+    """autotune_tile's timings as JSON keys: "CxG" on the tuning input,
+    "full:CxG" re-timed on the whole batch."""
+    return {(f"full:{k[1]}x{k[2]}" if k[0] == "full" else f"{k[0]}x{k[1]}"): v
+            for k, v in t.items()}
+
+
 class BatchRun:
     """One workload resident on the device: packed pairs, longest-first
     order, tile and every buffer preallocated; ``step()`` is one forward
@@ -492,13 +499,16 @@ def long_read_extra(torch, K, WL, dev, args, threads):
     w = WL.sw_long()
     log(f"[bench] sw_long generated: {w.n_pairs} pairs, {w.cells().sum():.3e} cells, "
         f"{time.time() - t0:.1f}s")
-    # tile autotuning on every 4th pair (by cell count): 4,096 pairs keep about
-    # 4 wavefronts per SIMD resident, as the batch does
+    # tile autotuning on every 4th pair (by cell count), the two fastest tiles
+    # and the default then timed again on the whole batch, where the choice is
+    # made (the sample can rank near-equal tiles the other way round)
     sub_t = WL.subsample(w, np.argsort(w.cells(), kind="stable")[::4])
     inp_t = K.upload_pairs(sub_t.queries, sub_t.targets, sub_t.band_offset, dev, alphabet="2bit")
+    inp_f = K.upload_pairs(w.queries, w.targets, w.band_offset, dev, alphabet="2bit")
     tuned, tune_ms = K.autotune_tile(inp_t, w.half_band, scoring(w),
-                                     order=torch.from_numpy(lpt_order(sub_t.cells())).to(dev))
-    del inp_t
+                                     order=torch.from_numpy(lpt_order(sub_t.cells())).to(dev),
+                                     confirm=(inp_f, torch.from_numpy(lpt_order(w.cells())).to(dev)))
+    del inp_t, inp_f
     torch.cuda.empty_cache()
     run = BatchRun(torch, K, w, dev, True, tile=tuned)
     total_cells, cells, tile = run.total_cells, run.cells, run.tile
@@ -518,7 +528,7 @@ def long_read_extra(torch, K, WL, dev, args, threads):
         "valu_frac": round(valu_tops / INT32_PEAK_TOPS, 4),
         "traffic": measured_traffic("sw_long", total_cells, tile),
         "tile": [tile.cells_per_lane, tile.lanes_per_pair],
-        "autotune_ms": {f"{c}x{g}": v for (c, g), v in tune_ms.items()},
+        "autotune_ms": tile_timings(tune_ms),
         "workload": "16384 pairs, lognormal(12 kb, 0.4) clipped 5-25 kb, 0.5% errors, "
                     "half_band 256, seed 2, fused walk",
     }
@@ -674,7 +684,7 @@ def main():
         bins.append({"len": [int(lo), int(hi)], "pairs": int(len(idx)),
                      "cells": int(ncells[idx].sum()),
                      "tile": [tb_.cells_per_lane, tb_.lanes_per_pair],
-                     "autotune_ms": {f"{c}x{g}": v for (c, g), v in tms.items()}})
+                     "autotune_ms": tile_timings(tms)})
     torch.cuda.empty_cache()
     picks = {tuple(b["tile"]) for b in bins}
     tile_n = K.resolve_tile(wn.half_band, *next(iter(picks))) if len(picks) == 1 else \
@@ -868,7 +878,7 @@ def main():
         "cpu_gcups": round(cgs, 4),
         "cpu_threads": threads,
         "tile": [rs.tile.cells_per_lane, rs.tile.lanes_per_pair],
-        "autotune_ms": {f"{c}x{g}": v for (c, g), v in tune_s.items()},
+        "autotune_ms": tile_timings(tune_s),
         "workload": "1024 x 150 bp, half_band 32 at offset 32, seed 0 (CPU: whole batch, one C "
                     "call)",
     }

@@ -62,10 +62,17 @@ def penalty_table(cfg, device) -> torch.Tensor:
     return t
 
 
+MAX_LOOKBACK = 64  # one predecessor per lane of a wavefront
+
+
 def _lookback(cfg) -> int:
+    """The reference takes any max_lookback (chaining.py:154; its default is
+    64, config.py:681); the chaining kernel holds up to 64.  Larger windows
+    are rejected, before any device work."""
     L = max(1, int(cfg.max_lookback))
-    if L > 64:
-        raise ValueError("max_lookback above 64 is not supported by the chaining kernel")
+    if L > MAX_LOOKBACK:
+        raise ValueError(f"max_lookback {L} above {MAX_LOOKBACK}: the chaining kernel holds one "
+                         "predecessor per lane of a 64-lane wavefront")
     return L
 
 
@@ -79,10 +86,10 @@ def _as_int64(a, name) -> np.ndarray:
 
 def _run_segments(read_end: np.ndarray, ref_end: np.ndarray, weight: np.ndarray,
                   seg_off: np.ndarray, cfg, bonus: Optional[np.ndarray] = None):
+    L = _lookback(cfg)
     dev = _device()
     if len(read_end) == 0:
         return np.zeros(0, np.float64), np.zeros(0, np.int64)
-    L = _lookback(cfg)
     re_ = torch.from_numpy(np.ascontiguousarray(read_end, np.int64)).to(dev)
     fe = torch.from_numpy(np.ascontiguousarray(ref_end, np.int64)).to(dev)
     w = torch.from_numpy(np.ascontiguousarray(weight, np.float64)).to(dev)
@@ -146,6 +153,7 @@ class AffineChainer:
     def __init__(self, cfg: ChainingConfig | None = None, backend: str = "hip",
                  workers: int | None = None):
         self.cfg = cfg or ChainingConfig()
+        _lookback(self.cfg)
         self.backend = backend
         self._workers = workers
 

@@ -42,6 +42,19 @@ from .seeding import encode_bases, reverse_complement_codes
 from .types import AnchorSet, Chain, ExtensionResult, merge_cigar
 
 
+MAX_HALF_BAND = 1023  # the forward kernels' tiles hold bands up to 2047 cells wide
+
+
+def _check_half_band(hb: int) -> int:
+    """The reference takes any half-band (extension.py:105; its default
+    max_half_band is 512, config.py:760); the kernels here hold up to 1023.
+    Wider bands are rejected, before any device work."""
+    if not 0 <= int(hb) <= MAX_HALF_BAND:
+        raise ValueError(f"half_band {int(hb)} outside [0, {MAX_HALF_BAND}]: the banded SW "
+                         f"kernels hold bands up to {2 * MAX_HALF_BAND + 1} cells wide")
+    return int(hb)
+
+
 def _hip_device(t: torch.Tensor) -> torch.device:
     if t.device.type == "cuda":
         return t.device
@@ -121,7 +134,7 @@ def banded_affine_sw_batch(query: torch.Tensor, target: torch.Tensor, query_len:
     ma, mi, go, ge = integer_scoring(cfg)
     B, M = query.shape
     N = target.shape[1]
-    hb = int(cfg.half_band if half_band is None else half_band)
+    hb = _check_half_band(cfg.half_band if half_band is None else half_band)
     out_dev = query.device
     qlen_h = torch.as_tensor(query_len).detach().cpu().numpy().astype(np.int64).reshape(B)
     tlen_h = torch.as_tensor(target_len).detach().cpu().numpy().astype(np.int64).reshape(B)
@@ -354,7 +367,7 @@ class ExtensionEngine:
         for r in live:
             L = int(lens[r])
             wins = [self._window(c, anchors[r], L, ref_len) for c in chains[r]]
-            hb = max(w[2] for w in wins)
+            hb = _check_half_band(max(w[2] for w in wins))
             geo[r] = wins
             by_hb.setdefault(hb, []).append(r)
         xd = float(cfg.x_drop)

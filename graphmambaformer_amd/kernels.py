@@ -880,47 +880,76 @@ def autotune_launch(inp: SwInputs, half_band: int, scoring: tuple, order: torch.
     return best, timings
 
 
+def _tile_ms(inp: SwInputs, half_band: int, scoring: tuple, tile: SwTile, reps: int,
+             order: Optional[torch.Tensor]) -> float:
+    """Best of ``reps`` timed launches of the forward pass with the fused
+    walk on ``inp`` with ``tile`` (after one untimed launch)."""
+    ws = make_workspace(inp, tile, cigar=True)
+    sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile, workspace=ws,
+               cigar=True)
+    s = torch.cuda.current_stream(inp.device)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(s)
+        sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
+                   workspace=ws, cigar=True)
+        e1.record(s)
+    torch.cuda.synchronize(inp.device)
+    del ws
+    return min(e0.elapsed_time(e1) for e0, e1 in ev)
+
+
 def autotune_tile(inp: SwInputs, half_band: int, scoring: tuple, reps: int = 3,
-                  order: Optional[torch.Tensor] = None, margin: float = 0.03):
+                  order: Optional[torch.Tensor] = None, margin: float = 0.03,
+                  confirm: Optional[tuple] = None):
     """Time every compiled tile that holds the band (the exact/specialised
     variant where one exists, e.g. 8x64 plus the scalar cell for W = 513) on
     ``inp`` with the fused walk; returns the chosen tile and
     {(cells_per_lane, lanes_per_pair): ms}, each the best of ``reps`` timed
-    launches (the least disturbed one).  The library's default tile for the
-    band (``resolve_tile(half_band)``) is kept unless another tile is faster
-    by more than ``margin``: a sample launch can rank near-equal tiles either
-    way (W = 513: 9x64 24.82 ms against 24.94 ms for 8x64 plus the scalar
-    cell on a quarter of the long-read batch, then 74.8 ms against 66.2 ms on
-    the whole batch)."""
+    launches (the least disturbed one).  The tile an untuned call runs
+    (``batch_tile(half_band, n_pairs)``) is kept unless another tile is faster
+    by more than ``margin``.
+
+    ``confirm`` = (whole-batch SwInputs, its order) when ``inp`` is a sample:
+    the sample's two fastest tiles and the whole batch's default are timed
+    again on the whole batch and the choice is made there
+    ({("full", C, G): ms} join the timings).  A sample ranks near-equal tiles
+    unreliably even when it spans the length distribution: it holds fewer
+    wavefronts per SIMD than the batch, so a tile that holds more pairs per
+    wavefront or needs fewer registers is favoured differently, and the
+    batch's tail (its longest pairs against the rest) is not the sample's.
+    W = 513: 9x64 24.82 ms against 24.94 ms for 8x64 plus the scalar cell on
+    every 4th pair of the long-read batch by cell count, then 74.8 ms against
+    66.2 ms on the whole batch."""
     timings = {}
-    best, best_ms = None, None
-    default = resolve_tile(half_band)
-    default_ms = None
+    default = batch_tile(half_band, inp.n_pairs, inp.device)
+    same = lambda t, u: (t.cells_per_lane, t.lanes_per_pair, t.lanes_stored) == \
+        (u.cells_per_lane, u.lanes_per_pair, u.lanes_stored)  # noqa: E731
+    ranked = []
     for C, G in list_tiles():
         try:  # the specialised variant for this band, or the general tile if it holds it
             tile = resolve_tile(half_band, C, G)
         except (RuntimeError, ValueError):
             continue
-        ws = make_workspace(inp, tile, cigar=True)
-        sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
-                   workspace=ws, cigar=True)
-        s = torch.cuda.current_stream(inp.device)
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(reps)]
-        for e0, e1 in ev:
-            e0.record(s)
-            sw_forward(inp, *scoring, half_band, traceback=True, order=order, tile=tile,
-                       workspace=ws, cigar=True)
-            e1.record(s)
-        torch.cuda.synchronize(inp.device)
-        ms = min(e0.elapsed_time(e1) for e0, e1 in ev)
+        ms = _tile_ms(inp, half_band, scoring, tile, reps, order)
         timings[(C, G)] = round(ms, 4)
-        if best_ms is None or ms < best_ms:
-            best, best_ms = tile, ms
-        if (tile.cells_per_lane, tile.lanes_per_pair, tile.lanes_stored) == \
-                (default.cells_per_lane, default.lanes_per_pair, default.lanes_stored):
-            default_ms = ms
-        del ws
-    if default_ms is not None and default_ms <= best_ms * (1.0 + margin):
-        best = default
+        ranked.append((ms, tile))
+    ranked.sort(key=lambda x: x[0])
+    if confirm is not None:
+        full, full_order = confirm
+        cands = [t for _, t in ranked[:2]]
+        fdefault = batch_tile(half_band, full.n_pairs, full.device)
+        if not any(same(t, fdefault) for t in cands):
+            cands.append(fdefault)
+        fms = []
+        for t in cands:
+            ms = _tile_ms(full, half_band, scoring, t, reps, full_order)
+            timings[("full", t.cells_per_lane, t.lanes_per_pair)] = round(ms, 4)
+            fms.append((ms, t))
+        ranked, default = sorted(fms, key=lambda x: x[0]), fdefault
+    best_ms, best = ranked[0]
+    for ms, t in ranked:
+        if same(t, default) and ms <= best_ms * (1.0 + margin):
+            return t, timings
     return best, timings

@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -213,3 +214,34 @@ def test_noalloc_validates_before_touching_the_device():
     assert L.gmfa_sw_forward_noalloc(ctypes.byref(b), ctypes.byref(p), ctypes.byref(o), None) == E
     assert b"static scratch" in L.gmfa_last_error()
     assert _lib.GMFA_NOALLOC_SCRATCH_BYTES == 256 << 20
+
+
+def test_drop_ins_reject_bands_and_lookbacks_beyond_the_kernels():
+    """The reference takes any half_band (extension.py:105) and max_lookback
+    (chaining.py:154); the kernels here hold half_band <= 1023 and
+    max_lookback <= 64 (the reference defaults are 512 and 64, config.py:681,
+    760).  Beyond the caps the drop-ins raise ValueError before any device
+    work; at the caps they accept."""
+    import torch
+
+    from graphmambaformer_amd import kernels as K
+    from graphmambaformer_amd.alignment import chaining as C
+    from graphmambaformer_amd.alignment import extension as E
+    from graphmambaformer_amd.config import ChainingConfig, ExtensionConfig
+
+    with pytest.raises(ValueError, match="1023"):
+        K.resolve_tile(1024)
+    t = K.resolve_tile(E.MAX_HALF_BAND)
+    assert t.cells_per_lane * t.lanes_per_pair >= 2 * E.MAX_HALF_BAND + 1
+    q = torch.ones((1, 10), dtype=torch.int16)
+    n = torch.tensor([10])
+    with pytest.raises(ValueError, match="half_band 1024"):
+        E.banded_affine_sw_batch(q, q, n, n, ExtensionConfig(), half_band=1024)
+    with pytest.raises(ValueError, match="half_band 1100"):
+        E.banded_affine_sw_batch(q, q, n, n, ExtensionConfig(half_band=1100))
+    with pytest.raises(ValueError, match="max_lookback 65"):
+        C.AffineChainer(ChainingConfig(max_lookback=65))
+    x = np.arange(5, dtype=np.int64)
+    with pytest.raises(ValueError, match="max_lookback 65"):
+        C.chain_dp_numpy(x, x, np.ones(5), ChainingConfig(max_lookback=65))
+    C.AffineChainer(ChainingConfig(max_lookback=C.MAX_LOOKBACK))
