@@ -7,9 +7,12 @@ A step is one pass of the hot path over the whole noisy batch (8,192 pairs,
 lengths lognormal median 8 kb clipped 0.5-100 kb, 6% errors with
 homopolymer-weighted indels, band half-width 751, scoring 2/-4/-4/-2): one
 launch of the forward kernel writes scores, ends and 4-bit traceback planes
-and walks every pair into a CIGAR, longest pair first, with the tile each
-length bin's autotune picks and the launch (plain, longest pairs at raised
-priority, or split over several wavefronts) the whole-batch autotune picks.
+and walks every pair into a CIGAR, longest pair first, on the tile each
+length bin's autotune picks, with the library's default launch for a batch of
+uneven lengths (kernels.default_launch: the pairs above one resident
+wavefront's share of the cells split over several wavefronts each); the
+whole-batch autotune of the alternatives (plain, longest pairs at raised
+priority, other split sizes) is reported beside it.
 Inputs are resident in HBM before the timed region.  ``value`` is GCUPS over
 in-band cells; ``roofline`` is the int32 VALU roof of the forward kernel (the
 binding one), ``roofline_hbm`` the HBM view.  Beside it the line carries the
@@ -618,7 +621,9 @@ def main():
                          "pick (profiling runs pin the tile they measure)")
     ap.add_argument("--launch", default=None,
                     help="split_rows=R,split_slices=S,prio_rows=P: run the headline batch with "
-                         "this launch instead of the autotuner's pick")
+                         "this launch instead of the library's default launch (profiling runs; "
+                         "no launch autotune); 'default': the library's default launch "
+                         "without the autotune; 'auto': the launch autotuner's pick")
     ap.add_argument("--cpu-sample-stride", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=200_000,
                     help="requests of the serving stream (0: skip the serving measurement)")
@@ -696,14 +701,23 @@ def main():
     total_cells = rn.total_cells
     # launch choice on the whole batch: plain, the longest pairs at raised
     # priority, or split over several wavefronts
-    if args.launch:  # pinned launch (profiling runs): no autotune launches
+    if args.launch == "default":  # the library's rule alone (profiling runs): no autotune
+        launch, launch_ms = dict(K.default_launch(rn.inp, wn.half_band, rn.tile), prio_rows=0), {}
+    elif args.launch and args.launch != "auto":  # pinned launch (profiling runs): no autotune
         launch, launch_ms = {"split_rows": 0, "split_slices": 0, "prio_rows": 0}, {}
         for kv in args.launch.split(","):
             k_, v_ = kv.split("=")
             launch[k_] = int(v_)
     else:
-        launch, launch_ms = K.autotune_launch(rn.inp, wn.half_band, scoring(wn), rn.order,
-                                              tile=rn.tile, workspace=rn.ws)
+        # the library's rule (default_launch) for the headline, so every run
+        # times the same launch; the autotuner's table of the alternatives
+        # (timed on the whole batch) is reported beside it, and --launch auto
+        # takes its pick instead
+        tuned, launch_ms = K.autotune_launch(rn.inp, wn.half_band, scoring(wn), rn.order,
+                                             tile=rn.tile, workspace=rn.ws)
+        launch = dict(K.default_launch(rn.inp, wn.half_band, rn.tile), prio_rows=0)
+        if args.launch == "auto":
+            launch = tuned
     rn.launch = launch
     log(f"[bench] launch autotune (ms): {launch_ms} -> {launch}")
     log(f"[bench] resident: tb planes {rn.ws['tb_bytes'] / 1e9:.1f} GB, tile "

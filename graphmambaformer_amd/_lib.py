@@ -117,6 +117,7 @@ SIGNATURES = {
     "gmfa_sw_list_tiles": (ctypes.c_int, [_P, _P, _i32]),
     "gmfa_sw_tb_bytes": (_i64, [_i32, ctypes.POINTER(SwTile)]),
     "gmfa_sw_split_variants": (ctypes.c_int, [_i32, _i32, _P, _P, _i32]),
+    "gmfa_sw_resident_waves": (ctypes.c_int, [ctypes.POINTER(SwParams), ctypes.POINTER(_i32)]),
     "gmfa_pack_codes": (ctypes.c_int, [_P, _i64, _i64, _P, _P, _i64, _P, _P, _P, _P]),
     "gmfa_pack_flat": (ctypes.c_int, [_P, _i64, _i32, _P, _P, _P, _P]),
     "gmfa_sw_forward": (ctypes.c_int, [ctypes.POINTER(SwBatch), ctypes.POINTER(SwParams),

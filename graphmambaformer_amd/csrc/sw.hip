@@ -2022,7 +2022,7 @@ static const SplitEntry kSplits[] = {
 
 // split workgroups of a launch, one pair each: at most this many of the
 // longest pairs run split, longer-than-split_rows pairs beyond them run plain
-constexpr int kSplitGroups = 256;
+constexpr int kSplitGroups = 512;  // split pairs (and workgroups) of one launch, at most
 
 // The library's side stream and fork / join events per device (created on
 // first use, kept until the process ends): the split kernel runs on it beside
@@ -2074,7 +2074,7 @@ static int split_launch(const SplitEntry &e, int groups, const SwKArgs &a, SwKer
     hipStream_t sa = eager ? ss.qa : ss.s;  // the split kernel
     hipStream_t sb = eager ? ss.qb : s;     // the wait kernel and the plain kernel
     hipLaunchKernelGGL(sw_split_count_kernel, dim3(1), dim3(256), 0, s, batch->qlen, batch->order,
-                       batch->n_pairs, split_rows, groups, a.counter);
+                       batch->n_pairs, split_rows, kSplitGroups, a.counter);
     GMFA_CHECK_HIP(hipGetLastError());
     GMFA_CHECK_HIP(hipEventRecord(ss.fork, s));
     GMFA_CHECK_HIP(hipStreamWaitEvent(sa, ss.fork, 0));
@@ -2231,6 +2231,19 @@ static int resident_waves(SwKernel k)
     return cus * nb;
 }
 
+extern "C" int gmfa_sw_resident_waves(const gmfa_sw_params *params, int32_t *waves)
+{
+    GMFA_REQUIRE(params && waves, "NULL argument");
+    gmfa_sw_tile tile;
+    const int rc = gmfa_sw_resolve_tile(params->half_band, params->cells_per_lane,
+                                        params->lanes_per_pair, &tile);
+    if (rc != GMFA_OK) return rc;
+    const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair,
+                                   2 * params->half_band + 1);
+    *waves = resident_waves(t->get(params->flags));
+    return GMFA_OK;
+}
+
 static WalkCtx make_walk_ctx(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
                              const gmfa_sw_tile &tile, bool nm)
 {
@@ -2367,7 +2380,11 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
             // it back until the split workgroups are resident, its wavefronts
             // beyond the free slots start as split workgroups retire (those
             // without a split pair at once)
-            split_groups = std::min(kSplitGroups, n);
+            // at most 3/4 of the resident wavefronts in split workgroups, so
+            // the plain kernel starts beside them; split pairs beyond the
+            // workgroups queue behind the first ones (shorter, longest first)
+            split_groups = std::min({kSplitGroups, n,
+                                     std::max(1, resident_waves(k) * 3 / 4 / split->K)});
             a.split_rows = P.split_rows;
         }
         if (slabs) {

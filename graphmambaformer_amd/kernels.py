@@ -314,6 +314,51 @@ def split_variants(half_band: int, cells_per_lane: int = 0) -> list[tuple[int, i
     return [(int(c[x]), int(k[x])) for x in range(n)]
 
 
+def resident_waves(half_band: int, tile: SwTile, cigar: bool = True) -> int:
+    """Wavefronts of the forward kernel (traceback planes, fused walk when
+    ``cigar``) for this band and tile that the current device holds at once."""
+    flags = _lib.SW_TRACEBACK | (_lib.SW_CIGAR if cigar else 0)
+    p = make_params(0, 0, 0, 0, half_band, flags, tile)
+    w = ctypes.c_int32()
+    check(lib().gmfa_sw_resident_waves(ctypes.byref(p), ctypes.byref(w)), "gmfa_sw_resident_waves")
+    return int(w.value)
+
+
+SPLIT_MAX_PAIRS = 512  # split pairs of one launch, at most (the library's bound)
+SPLIT_SHARE = 0.7      # split pairs hold at least this multiple of a resident wavefront's share
+SPLIT_WAVE_FRAC = 0.75  # split wavefronts (pairs x slices) within this share of the resident ones
+
+
+def default_launch(inp: SwInputs, half_band: int, tile: Optional[SwTile] = None) -> dict:
+    """Launch options (sw_forward keywords) for a batch of uneven lengths:
+    split the pairs whose cells (query rows x band width) reach SPLIT_SHARE of
+    the batch's mean cells per resident wavefront, at most SPLIT_MAX_PAIRS,
+    over the most slices compiled for the band whose wavefronts (pairs x
+    slices) stay within SPLIT_WAVE_FRAC of the resident ones (the fewest
+    slices when none do); the plain launch when no pair qualifies, the band
+    has no split variant or the batch holds N masks.  One wavefront saturates
+    its SIMD, so a pair far above the share sets the batch's end, and the
+    split pairs, started first, must leave room for the rest.  Measured on the
+    noisy 8,192-pair batch at half_band 751 (resident wavefronts 2,048): the
+    512 longest (37-100 kb) over 3 wavefronts each 107.2 ms, over 4 (all
+    2,048 wavefronts) 110.7 ms, the 256 longest over 4 110.2 ms, plain
+    121 ms; splitting only the 40 longest (all ~100 kb) no faster than plain."""
+    tile = tile or batch_tile(half_band, inp.n_pairs, inp.device)
+    variants = split_variants(half_band, tile.cells_per_lane) if not inp.has_n else []
+    plain = {"split_rows": 0, "split_slices": 0}
+    if not variants or tile.lanes_per_pair != 64 or inp.n_pairs == 0:
+        return plain
+    rows = np.sort(np.asarray(inp.qlen_host, np.int64))[::-1]
+    waves = max(1, resident_waves(half_band, tile))
+    share = rows.sum() / waves
+    k = min(int((rows >= SPLIT_SHARE * share).sum()), SPLIT_MAX_PAIRS)
+    if k == 0:
+        return plain
+    counts = sorted({sl for _, sl in variants})
+    fit = [sl for sl in counts if k * sl <= SPLIT_WAVE_FRAC * waves]
+    return {"split_rows": int(rows[k - 1]), "split_slices": max(fit) if fit else counts[0]}
+
+
 def _cigar_buffers(inp: SwInputs, ws: dict):
     n = inp.n_pairs
     dev = inp.device
@@ -825,7 +870,7 @@ def cigar_bam(cigar: torch.Tensor, cigar_off: torch.Tensor, n_cigar: torch.Tenso
 
 def autotune_launch(inp: SwInputs, half_band: int, scoring: tuple, order: torch.Tensor,
                     tile: Optional[SwTile] = None, workspace: Optional[dict] = None,
-                    tops=(64, 128, 192, 256), reps: int = 3, margin: float = 0.03):
+                    tops=(128, 256, 384, 512), reps: int = 3, margin: float = 0.03):
     """Choose how a batch of uneven lengths is launched: the plain launch,
     the plain launch with the ``k`` longest pairs at raised wave priority
     (``prio_rows``), or a split launch (the ``k`` longest pairs over several

@@ -89,8 +89,10 @@ typedef struct gmfa_sw_params {
     /* Split launch (gmfa_sw_forward with exactly GMFA_SW_TRACEBACK |
      * GMFA_SW_CIGAR, a 64-lane tile and batch->order set): split_rows > 0 runs
      * the longest-first prefix of batch->order whose queries have at least
-     * split_rows rows (at most 256 pairs) each on a workgroup of several
-     * wavefronts, one band slice per wavefront, in a kernel on the library's
+     * split_rows rows (at most 512 pairs) each on a workgroup of several
+     * wavefronts, one band slice per wavefront (at most 3/4 of the resident
+     * wavefronts in such workgroups; further split pairs queue behind the
+     * first ones), in a kernel on the library's
      * own stream beside the plain kernel for the rest (forked from and joined
      * back into the caller's stream with events, so a graph capture of the
      * caller's stream holds both; the plain kernel waits, at most 5 ms, until
@@ -179,6 +181,13 @@ int gmfa_sw_list_tiles(int32_t *cells_per_lane, int32_t *lanes_per_pair, int32_t
  * per pair). */
 int gmfa_sw_split_variants(int32_t half_band, int32_t cells_per_lane, int32_t *slice_cells,
                            int32_t *slices, int32_t cap);
+
+/* Wavefronts of the forward kernel that params selects (band, tile, flags)
+ * the current device holds at once: the persistent launch's grid.  A batch's
+ * mean cells per resident wavefront is the share a pair must exceed to set
+ * the batch's end (the pairs worth a split launch).  No counterpart in the
+ * reference (its batches are one pair per workgroup). */
+int gmfa_sw_resident_waves(const gmfa_sw_params *params, int32_t *waves);
 
 /* Bytes of traceback planes one pair of query length qlen needs with tile. */
 int64_t gmfa_sw_tb_bytes(int32_t qlen, const gmfa_sw_tile *tile);
