@@ -19,8 +19,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libgmfa.so")
-SOURCES = ["api.hip", "sw.hip", "chain.hip", "chain_post.hip", "serve.hip", "seed.hip", "wfa.hip",
-           "noalloc.hip"]
+# the forward-kernel variants first: the largest translation units start first
+SOURCES = ["sw_tiles_24.hip", "sw_tiles_wide.hip", "sw_tiles_mid.hip", "sw_tiles_small.hip",
+           "sw_split.hip", "sw.hip", "api.hip", "chain.hip", "chain_post.hip", "serve.hip",
+           "seed.hip", "wfa.hip", "noalloc.hip"]
 ARCH = "gfx950"
 
 
@@ -57,8 +59,10 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 4, verbose: bool = False) -> str:
+def build(jobs: int = 0, verbose: bool = False) -> str:
+    """jobs: parallel compiles (0: one per host core, at most 16)."""
     os.makedirs(BUILD, exist_ok=True)
+    jobs = jobs or min(os.cpu_count() or 4, 16)
     with ThreadPoolExecutor(max(1, jobs)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
@@ -74,6 +78,6 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--jobs", type=int, default=0)
     args = ap.parse_args()
     build(args.jobs, verbose=True)
