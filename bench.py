@@ -597,6 +597,30 @@ def long_read_extra(torch, K, WL, dev, args, threads):
     return out
 
 
+def single_batch(torch, K, WL, dev, args):
+    """--workload long / short: W untimed + K timed steps (forward with the
+    fused walk) of that batch alone, on the --tile tile or the library's
+    default for the batch (profiling runs of that kernel)."""
+    w = WL.sw_long() if args.workload == "long" else WL.sw_short()
+    if args.tile:
+        tile = K.resolve_tile(w.half_band, *(int(x) for x in args.tile.split(",")))
+    else:
+        tile = K.batch_tile(w.half_band, w.n_pairs, dev)
+    run = BatchRun(torch, K, w, dev, True, tile=tile)
+    for _ in range(args.warmup):
+        run.step()
+    torch.cuda.synchronize()
+    ms = time_events(torch, run.step, max(args.steps, 1))
+    gcups = run.total_cells / (ms / 1e3) / 1e9
+    log(f"[bench] {args.workload}: {ms:.4f} ms -> {gcups:.1f} GCUPS on "
+        f"{tile.cells_per_lane}x{tile.lanes_per_pair}")
+    return {"workload": args.workload, "pairs": w.n_pairs, "cells": run.total_cells,
+            "tile": [tile.cells_per_lane, tile.lanes_per_pair], "steps": args.steps,
+            "ms_per_step": round(ms, 4), "gcups": round(gcups, 2),
+            "valu_frac": round(run.total_cells * OPS_PER_CELL_TB / (ms / 1e3) / 1e12 /
+                               INT32_PEAK_TOPS, 4)}
+
+
 def host_info():
     model = "unknown"
     try:
@@ -635,6 +659,9 @@ def main():
                          "saturation capacity")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--workload", choices=("noisy", "long", "short"), default="noisy",
+                    help="long / short: only time the long-read or the short-read batch's step "
+                         "(profiling runs; --tile pins its tile, else the library's default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -659,6 +686,9 @@ def main():
         return
 
     threads = os.cpu_count() or 1
+    if args.workload != "noisy":
+        print(json.dumps(single_batch(torch, K, WL, dev, args)), flush=True)
+        return
     # ------------------------------------------------------------ headline
     # the noisy uneven-length batch (8,192 pairs, 0.5-100 kb, band 1503): the
     # largest single-GPU configuration, and the one whose tail the launch

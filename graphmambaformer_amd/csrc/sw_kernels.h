@@ -1010,6 +1010,16 @@ __global__ __launch_bounds__(64 * KSL) void sw_split_kernel(SwKArgs a, uint32_t 
         __builtin_amdgcn_s_setprio(0);
     }
 }
+// Row modes of the forward kernel (see its row lambda).
+constexpr int kRowFast = 0, kRowEdge = 1, kRowMasked = 2;
+
+// 2-bit fields 0 .. n-1 of a word set (n clamped to [0, 16]).
+__device__ __forceinline__ uint32_t low_fields(int n)
+{
+    n = min(max(n, 0), 16);
+    return n >= 16 ? ~0u : ((1u << (2 * n)) - 1u);
+}
+
 // SV: serving variant (persistent queue fed by the host; staged requests of
 // gmfa_serve_run or the online ring of gmfa_serve_open), a separate
 // instantiation so the offline kernels carry none of its state.
@@ -1253,6 +1263,18 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     nws = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)na[1], G - 1) << 32) |
                           (uint32_t)__builtin_amdgcn_readlane((int)na[0], G - 1);
             }
+            // edge blocks: the window's fields whose columns lie outside the
+            // matrix (j < 1 or j > tlen), as 2-bit fields of 11 (never match)
+            const bool interior = b >= fb0 && b < fb1;
+            uint32_t om[NA];
+            if constexpr (!RK) {
+                if (!interior) {
+                    const int c0 = (int)(e_t - to);  // column - 1 of the window's field 0
+#pragma unroll
+                    for (int x = 0; x < NA; ++x)
+                        om[x] = low_fields(-c0 - 16 * x) | ~low_fields(tlen - c0 - 16 * x);
+                }
+            }
             e_t += 8;
             if (b + 1 < nblk) {
                 load_t(e_t, wt, a.tseq);
@@ -1273,8 +1295,22 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 }
             }
 
-            auto row = [&](auto slow_tag, const int r) __attribute__((always_inline)) {
-                constexpr bool SLOW = decltype(slow_tag)::value;
+            // Row modes.  kRowFast: every band cell of the block is inside the
+            // matrix.  kRowEdge: cells outside it (left of column 1, right of
+            // column tlen, below row qlen) are computed like the others but
+            // never match.  Such a cell then never exceeds a cell before it in
+            // (row, diagonal) order that is inside the matrix (a left cell
+            // holds H = 0 exactly; a right or lower one takes its value from a
+            // cell above or to its left with a non-positive step), so the
+            // first maximum, and every inside cell's value and plane bits,
+            // are those of the masked form; only the outside cells' plane
+            // bits, which no walk reads, differ.  kRowMasked (row keys, which
+            // need each row's maximum over the inside cells alone): outside
+            // cells hold H = 0 and F = -inf.
+            auto row = [&](auto mode_tag, const int r) __attribute__((always_inline)) {
+                constexpr int MODE = decltype(mode_tag)::value;
+                constexpr bool SLOW = MODE == kRowMasked;
+                constexpr bool EDGE = MODE == kRowEdge;
                 const int i = i0 + r;
                 const int zrow = i * mi;  // H = 0 in this row's domain
                 const int zprev = zrow - mi;
@@ -1289,6 +1325,10 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     if (NM) {
                         y |= __builtin_amdgcn_alignbit(na[x + 1], na[x], 2 * r);
                         y = qn ? ~0u : y;
+                    }
+                    if constexpr (EDGE) {
+                        y |= __builtin_amdgcn_alignbit(om[x + 1], om[x], 2 * r);
+                        y = (i > qlen) ? ~0u : y;
                     }
                     z[x] = ~y & 0x55555555u;  // 2-bit field k = 1 <=> cell k matches
                 }
@@ -1334,7 +1374,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 for (int k = 1; k < C; k += 2) T = (k + 1 < C) ? max(max(T, Av[k]), Av[(k + 1) % C]) : max(T, Av[k]);
                 // lane 0's prefix is 0, not -inf: every live A = M + k*e >= zrow
                 // >= 0 (row-shifted domain), so a zero prefix never wins E (E
-                // <= 0 <= zrow <= M), and a dead cell's A stays below it
+                // <= 0 <= zrow <= M), and a masked cell's A stays below it
                 int X = from_prev_lane_or0(seg_scan_max<G>(T + lCe));
                 if constexpr (G < 64) X = (gl == 0) ? kNeg : X;
                 const int Xl = X - lCe;
@@ -1354,13 +1394,14 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     const int d63 = __builtin_amdgcn_readlane(dlast, G - 1);
                     const int g63 = __builtin_amdgcn_readlane(Glast, G - 1);
                     const int tf = (int)((uint32_t)(tws >> (2 * (C + r))) & 3u);
-                    const int mf = NM ? (int)(((uint32_t)qn) | ((uint32_t)(nws >> (2 * (C + r))) & 1u))
-                                      : 0;
-                    // SLOW: the cell is out of the matrix unless every term is >= 0
-                    const int xl = SLOW ? (int)((uint32_t)(qlen - i) |
-                                                (uint32_t)(tlen + hb - off - i - (W - 1)) |
-                                                (uint32_t)((W - 1) - (hb + 1 - off - i)))
-                                        : 0;
+                    int mf = NM ? (int)(((uint32_t)qn) | ((uint32_t)(nws >> (2 * (C + r))) & 1u))
+                                : 0;
+                    // the cell is out of the matrix unless every term is >= 0
+                    const int xl = (SLOW || EDGE) ? (int)((uint32_t)(qlen - i) |
+                                                          (uint32_t)(tlen + hb - off - i - (W - 1)) |
+                                                          (uint32_t)((W - 1) - (hb + 1 - off - i)))
+                                                  : 0;
+                    if constexpr (EDGE) mf |= (int)((uint32_t)xl >> 31);  // outside: never matches
                     scalar_cell<SLOW>(Hs, Fs, accs, bestKs, bestis, tf, (int)qc, mf, (int)mms, zprev,
                                 zrow, cH, kNeg - cF, d63, g63, go + (C - 1) * ge,
                                 (int)((uint32_t)(i * mi) << 5), i, xl, kNeg);
@@ -1370,6 +1411,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 }
 
                 int rk = INT_MIN;
+                int rkh = INT_MIN;  // registers past kstar (compile-time kstar only)
                 int Gkm1 = Xl;
 #pragma unroll
                 for (int k = 0; k < C; ++k) {
@@ -1381,9 +1423,19 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                         const bool live = (uint32_t)(k - lo) < span;
                         H = live ? H : zrow;
                         Fk = live ? Fk : kNeg;
+                    } else if constexpr (KS >= 0) {
+                        // registers past diagonal W-1 (of lane lstar and the
+                        // dead lanes) hold values of no cell: only the first
+                        // of them, the d+1 input of diagonal W-1, must read
+                        // as the band guard; the others feed dead cells alone,
+                        // and the best key leaves them out below
+                        if (k == kstar + 1) {
+                            H = dead_hi ? zrow : H;
+                            Fk = dead_hi ? kNeg : Fk;
+                        }
                     } else {
-                        // registers past diagonal W-1 of lane lstar (and of the
-                        // dead lanes) must read as out of band for the guard
+                        // runtime kstar: every register past diagonal W-1
+                        // reads as out of band
                         if (k > kstar) {
                             H = dead_hi ? zrow : H;
                             Fk = dead_hi ? kNeg : Fk;
@@ -1394,13 +1446,18 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                         acc[k] = insert_sign(acc[k], Fk - H);     // not_f
                         acc[k] = insert_sign(acc[k], k ? (Av[k ? k - 1 : 0] - Gkm1) : dprev);  // e_cont
                     }
-                    rk = max(rk, (int)(((uint32_t)H << 8) | (uint32_t)(31 - k)));
+                    const int key = (int)(((uint32_t)H << 8) | (uint32_t)(31 - k));
+                    if (!SLOW && KS >= 0 && k > kstar)
+                        rkh = max(rkh, key);
+                    else
+                        rk = max(rk, key);
                     Hp[k] = H;  // Hp[k] is not read again in this row
                     Fv[k] = Fk;
                     Gkm1 = Gk;
                 }
 #pragma unroll
                 for (int k = 0; k < C; ++k) Fp[k] = Fv[k];
+                if constexpr (!SLOW && KS >= 0 && KS + 1 < C) rk = max(rk, dead_hi ? INT_MIN : rkh);
                 // the row's key in true H with the row in bits 5-7 (one scalar:
                 // per-cell constants above 64 would each take a register), then
                 // the block's running maximum (keys order by H, then by smaller
@@ -1421,14 +1478,15 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             };
             // Hp[k] is overwritten in pass 2 only after pass 1 consumed it, and
             // Hp[k+1] (read by cell k) is consumed in pass 1 as well.
-            // interior blocks (every band cell of the 8 rows inside the matrix)
-            // skip the per-cell masking
-            if (b >= fb0 && b < fb1) {
+            if (interior) {
 #pragma unroll
-                for (int r = 0; r < 8; ++r) row(std::false_type{}, r);
+                for (int r = 0; r < 8; ++r) row(std::integral_constant<int, kRowFast>{}, r);
+            } else if constexpr (RK) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r) row(std::integral_constant<int, kRowMasked>{}, r);
             } else {
 #pragma unroll
-                for (int r = 0; r < 8; ++r) row(std::true_type{}, r);
+                for (int r = 0; r < 8; ++r) row(std::integral_constant<int, kRowEdge>{}, r);
             }
             // strict improvement in H over the lane's best so far
             if (bk > (bestK | 255)) {
