@@ -177,9 +177,13 @@ SERVE_SLOTS = (512, 1024, 2048)  # persistent wavefronts of the short / long / n
 # admitted when its planes' pages are free); per-slot slabs for each class's
 # longest request would take 0.02 / 6.6 / 156 GB
 SERVE_PLANE_BUDGET = (256 << 20, 8 << 30, 32 << 30)
+# requests of the noisy class with at least this many query rows (the longest
+# ~8% of its lognormal(8 kb, 1.0) lengths) run at raised wave priority: the
+# class's tail latency is its longest requests' own run time
+SERVE_PRIO_ROWS = 32768
 
 
-def serving_extra(torch, WL, dev, n_req, load, online=True):
+def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO_ROWS):
     """p50 / p99 latency of a Poisson request stream at a fixed rate.
 
     A resident 46 Mbp reference; requests 60% short (150 bp, half_band 32), 30%
@@ -209,7 +213,8 @@ def serving_extra(torch, WL, dev, n_req, load, online=True):
         for c, p in enumerate(mix.pools):
             sel = np.flatnonzero(cls == c)
             out.append(S.make_class(p.name, ref, pools[c], idx[sel], arrivals[sel], sc,
-                                    SERVE_SLOTS[c], plane_budget_bytes=SERVE_PLANE_BUDGET[c]))
+                                    SERVE_SLOTS[c], plane_budget_bytes=SERVE_PLANE_BUDGET[c],
+                                    prio_rows=prio_rows if p.name == "noisy" else 0))
         return out
 
     def gather(classes):
@@ -256,6 +261,7 @@ def serving_extra(torch, WL, dev, n_req, load, online=True):
         "completed": r.completed,
         "timed_out": r.timed_out,
         "slots": dict(zip([p.name for p in mix.pools], SERVE_SLOTS)),
+        "noisy_prio_rows": prio_rows,
         "planes": pages,
         "workload": f"{n_req} Poisson requests, 60% short (150 bp, half_band 32) / 30% long "
                     "(lognormal 12 kb, half_band 256) / 10% noisy (lognormal 8 kb, 6% errors, "
@@ -269,14 +275,15 @@ def serving_extra(torch, WL, dev, n_req, load, online=True):
     del pools
     torch.cuda.empty_cache()
     if online:
-        res["online"] = online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc)
+        res["online"] = online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc,
+                                             prio_rows)
     return res
 
 
 ONLINE_CIGAR_CAP = (512, 8192, 16384)  # CIGAR runs kept per request, short / long / noisy
 
 
-def online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc):
+def online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc, prio_rows):
     """The same request stream through the online session API
     (gmfa_serve_submit_many / gmfa_serve_poll): queries packed by the host
     into the class rings as they arrive, results polled back to host memory.
@@ -304,7 +311,8 @@ def online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc):
     bo = np.array([mix.pools[c].band_offset[k] for c, k in zip(cls, idx)], np.int32)
     ocls = [S.OnlineClass(half_band=p.half_band, max_qlen=max(len(q) for q in p.queries),
                           n_slots=SERVE_SLOTS[c], cigar_cap=ONLINE_CIGAR_CAP[c],
-                          plane_budget_bytes=SERVE_PLANE_BUDGET[c])
+                          plane_budget_bytes=SERVE_PLANE_BUDGET[c],
+                          prio_rows=prio_rows if p.name == "noisy" else 0)
             for c, p in enumerate(mix.pools)]
 
     def drive(arrivals):
@@ -654,6 +662,9 @@ def main():
     ap.add_argument("--no-online", action="store_true",
                     help="skip the online-session serving leg (keeps a profiled run short: "
                          "its Python front end makes ~10^5 library calls)")
+    ap.add_argument("--serve-prio-rows", type=int, default=SERVE_PRIO_ROWS,
+                    help="noisy-class requests with at least this many query rows run at "
+                         "raised wave priority (0: none)")
     ap.add_argument("--serve-load", type=float, default=0.65,
                     help="offered load of the paced serving run, as a fraction of the "
                          "saturation capacity")
@@ -933,7 +944,8 @@ def main():
     # ----------------------------------------------------------------- serving
     if args.serve_requests > 0:
         result["serving"] = serving_extra(torch, WL, dev, args.serve_requests, args.serve_load,
-                                          online=not args.no_online)
+                                          online=not args.no_online,
+                                          prio_rows=args.serve_prio_rows)
         torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ chains

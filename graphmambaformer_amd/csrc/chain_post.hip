@@ -79,17 +79,37 @@ struct SortArgs {
     int64_t n;
 };
 
+// Segments of at most this many anchors sort (and backtrack) with their keys
+// and permutations in LDS: the merge sort's binary searches are chains of
+// dependent loads, ~100 cycles each from LDS against ~1 us from L2/HBM.
+// Larger segments (any size) use global scratch.
+constexpr int kLdsAnchors = 1024;
+
 __global__ __launch_bounds__(256) void chain_sort_kernel(SortArgs a)
 {
+    __shared__ int64_t k_fe[kLdsAnchors], k_re[kLdsAnchors];
+    __shared__ int32_t p_a[kLdsAnchors], p_b[kLdsAnchors];
     const int s = blockIdx.x;
     const int64_t base = a.seg_off[s];
     const int n = (int)(a.seg_off[s + 1] - base);
     if (n <= 0) return;
     const int64_t *re = a.re + base, *fe = a.fe + base;
-    int32_t *perm = merge_sort_perm(n, a.scratch + base, a.scratch + a.n + base,
-                                    [&](int32_t x, int32_t y) {
-                                        return fe[x] < fe[y] || (fe[x] == fe[y] && re[x] < re[y]);
-                                    });
+    const int32_t *perm;
+    if (n <= kLdsAnchors) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            k_fe[i] = fe[i];
+            k_re[i] = re[i];
+        }
+        __syncthreads();
+        perm = merge_sort_perm(n, p_a, p_b, [&](int32_t x, int32_t y) {
+            return k_fe[x] < k_fe[y] || (k_fe[x] == k_fe[y] && k_re[x] < k_re[y]);
+        });
+    } else {
+        perm = merge_sort_perm(n, a.scratch + base, a.scratch + a.n + base,
+                               [&](int32_t x, int32_t y) {
+                                   return fe[x] < fe[y] || (fe[x] == fe[y] && re[x] < re[y]);
+                               });
+    }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int64_t src = base + perm[i], dst = base + i;
         a.o_re[dst] = a.re[src];
@@ -119,8 +139,54 @@ struct BtArgs {
     int64_t n;
 };
 
+// Kept chain c of a segment: spans over its members and the members as anchor
+// ids in read order (the walk collected them from the chain's end back),
+// by one wavefront.
+__device__ __forceinline__ void chain_record(const BtArgs &a, int64_t base, int64_t rec,
+                                             const int32_t *walk, int start, int cnt)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    int64_t rs = INT64_MAX, re_ = INT64_MIN, fs = INT64_MAX, fe_ = INT64_MIN;
+    for (int t = lane; t < cnt; t += 64) {
+        const int64_t l = base + walk[start + t];
+        rs = min(rs, a.rp[l]);
+        re_ = max(re_, a.re[l]);
+        fs = min(fs, a.fp[l]);
+        fe_ = max(fe_, a.fe[l]);
+        a.members[base + start + (cnt - 1 - t)] = a.id[l];
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        rs = min(rs, (int64_t)__shfl_xor((long long)rs, o));
+        re_ = max(re_, (int64_t)__shfl_xor((long long)re_, o));
+        fs = min(fs, (int64_t)__shfl_xor((long long)fs, o));
+        fe_ = max(fe_, (int64_t)__shfl_xor((long long)fe_, o));
+    }
+    if (lane == 0) {
+        a.c_span[4 * rec + 0] = rs;
+        a.c_span[4 * rec + 1] = re_;
+        a.c_span[4 * rec + 2] = fs;
+        a.c_span[4 * rec + 3] = fe_;
+        a.c_mem[2 * rec + 0] = base + start;
+        a.c_mem[2 * rec + 1] = cnt;
+    }
+}
+
+// One workgroup per segment.  The stable argsort of -f, then one thread walks
+// the claimed-walk (chaining.py:530-576): anchors by descending f; each
+// unclaimed one follows its parents, claiming, until -1 or a claimed anchor;
+// the chain (score f[i] - f[stop], members) is kept if it has enough members
+// and score.  Segments of up to kLdsAnchors anchors keep f, parent, the
+// order, the claim bits and the walked members in LDS and record the kept
+// chains afterwards, a wavefront per chain; larger ones walk in global
+// memory.
 __global__ __launch_bounds__(256) void chain_backtrack_kernel(BtArgs a)
 {
+    __shared__ double l_f[kLdsAnchors];
+    __shared__ int32_t l_par[kLdsAnchors], p_a[kLdsAnchors], p_b[kLdsAnchors];
+    __shared__ int32_t l_walk[kLdsAnchors], l_start[kLdsAnchors], l_cnt[kLdsAnchors];
+    __shared__ uint8_t l_claim[kLdsAnchors];
+    __shared__ int l_nc;
     const int s = blockIdx.x;
     const int64_t base = a.seg_off[s];
     const int n = (int)(a.seg_off[s + 1] - base);
@@ -129,12 +195,57 @@ __global__ __launch_bounds__(256) void chain_backtrack_kernel(BtArgs a)
         return;
     }
     const double *f = a.f + base;
+    const int64_t *parent = a.parent + base;
+    if (n <= kLdsAnchors) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            l_f[i] = f[i];
+            int64_t pi = parent[i];
+            if (pi >= n) {
+                atomicOr(a.bad_parent, 1);
+                pi = -1;
+            }
+            l_par[i] = (int32_t)pi;
+            l_claim[i] = 0;
+        }
+        __syncthreads();
+        const int32_t *order =
+            merge_sort_perm(n, p_a, p_b, [&](int32_t x, int32_t y) { return l_f[x] > l_f[y]; });
+        if (threadIdx.x == 0) {
+            int nc = 0, nm = 0;
+            for (int r = 0; r < n; ++r) {
+                const int i = order[r];
+                if (l_claim[i]) continue;
+                int node = i;
+                const int start = nm;
+                while (node >= 0 && !l_claim[node]) {
+                    l_claim[node] = 1;
+                    l_walk[nm++] = node;
+                    node = l_par[node];
+                }
+                const double sc = l_f[i] - (node >= 0 ? l_f[node] : 0.0);
+                if (nm - start < a.min_anchors || sc < a.min_score) {
+                    nm = start;
+                    continue;
+                }
+                a.c_score[base + nc] = sc;
+                l_start[nc] = start;
+                l_cnt[nc] = nm - start;
+                ++nc;
+            }
+            l_nc = nc;
+            a.n_chains[s] = nc;
+        }
+        __syncthreads();
+        const int nc = l_nc;
+        for (int c = (int)(threadIdx.x >> 6); c < nc; c += (int)(blockDim.x >> 6))
+            chain_record(a, base, base + c, l_walk, l_start[c], l_cnt[c]);
+        return;
+    }
     uint8_t *claimed = a.claimed + base;
     for (int i = threadIdx.x; i < n; i += blockDim.x) claimed[i] = 0;
     int32_t *order = merge_sort_perm(n, a.scratch + base, a.scratch + a.n + base,
                                      [&](int32_t x, int32_t y) { return f[x] > f[y]; });
     if (threadIdx.x != 0) return;
-    const int64_t *parent = a.parent + base;
     int64_t *mem = a.members + base;
     int nc = 0, nm = 0;
     for (int r = 0; r < n; ++r) {

@@ -338,6 +338,7 @@ static const std::vector<TileEntry> &tiles()
             {5, 16, 4},   {9, 64, 8},   {8, 64, 8},   {24, 64, 14}, {2, 16, -1},  {5, 16, -1},
             {8, 16, -1},  {4, 32, -1},  {8, 32, -1},  {2, 64, -1},  {4, 64, -1},  {8, 64, -1},
             {9, 64, -1},  {12, 64, -1}, {16, 64, -1}, {24, 64, -1}, {32, 64, -1},
+            {1, 64, 1},
         };
         std::vector<TileEntry> pool, v;
         for (auto part : {sw_tiles_24, sw_tiles_wide, sw_tiles_mid, sw_tiles_small}) {
@@ -387,13 +388,16 @@ static const TileEntry *find_tile(int C, int G, int W)
 }
 
 // default: the tile with the least padding (live cells / lane slots), then
-// specialised before general, then more cells per lane
+// specialised before general, then more cells per lane.  A one-cell lane
+// (1 x 64 + the scalar cell for W = 65) is a latency tile for batches too
+// small to fill the device and never the default: a large batch of short
+// pairs runs faster packed several pairs to a wavefront.
 static const TileEntry *default_tile(int W)
 {
     const TileEntry *best = nullptr;
     double best_eff = -1.0;
     for (const auto &t : tiles()) {
-        if (!tile_serves(t, W)) continue;
+        if (!tile_serves(t, W) || t.C == 1) continue;
         const double eff = (double)W / (double)tile_capacity(t) + (t.KS >= 0 ? 1e-3 : 0.0);
         if (eff > best_eff + 1e-9) { best = &t; best_eff = eff; }
     }
@@ -547,11 +551,17 @@ extern "C" int gmfa_sw_resolve_tile(int32_t half_band, int32_t cells_per_lane,
 
 extern "C" int gmfa_sw_list_tiles(int32_t *cells_per_lane, int32_t *lanes_per_pair, int32_t cap)
 {
-    // general tiles only: each (C, G) once; specialised variants are picked
-    // automatically for the bands they serve
+    // each (C, G) shape once: the general tiles, then the shapes compiled only
+    // as a specialised variant (which serve their band alone); a specialised
+    // variant of a general shape is picked automatically for the band it serves
     int n = 0;
+    auto general = [](int C, int G) {
+        for (const auto &u : tiles())
+            if (u.C == C && u.G == G && u.KS < 0) return true;
+        return false;
+    };
     for (const auto &t : tiles()) {
-        if (t.KS >= 0) continue;
+        if (t.KS >= 0 && general(t.C, t.G)) continue;
         if (n < cap) {
             if (cells_per_lane) cells_per_lane[n] = t.C;
             if (lanes_per_pair) lanes_per_pair[n] = t.G;
@@ -715,7 +725,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     a.walk = 0;
     a.blkstride = tile.lanes_stored * tile.cells_per_lane;
     a.split_rows = 0;
-    a.prio_rows = serve ? 0 : std::max(0, P.prio_rows);
+    a.prio_rows = std::max(0, P.prio_rows);
     a.wc = make_walk_ctx(batch, params, tile, nm);
     a.wo = WalkOut{};
     if (P.flags & GMFA_SW_CIGAR) {
@@ -852,10 +862,15 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
         GMFA_CHECK_HIP(hipGetLastError());
     } else if (split) {
         return split_launch(*split, split_groups, a, k, waves, batch, P.split_rows, s);
-    } else if (!serve) {
+    } else if (slabs) {
+        // the library's static scratch: cleared before each launch (the
+        // kernel also leaves it at zero)
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
         GMFA_CHECK_HIP(hipGetLastError());
     }
+    // other offline launches: the kernel's last wave leaves the caller's
+    // counter at zero, so it is zero here (the caller zeroes it once, before
+    // its first launch)
     hipLaunchKernelGGL(k, dim3(waves), dim3(64), 0, s, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;

@@ -1138,7 +1138,16 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 n_items = serve_wait(a.dctl, a.serve_tail, a.serve_closed, a.serve_gave_up,
                                      a.deadline_ticks, a.poll_ticks, item);
             }
-            if (item >= n_items) break;
+            if (item >= n_items) {
+                // every wave ends on one failing claim, so the claim that
+                // returns n_items + grid - 1 is the launch's last: that wave
+                // leaves the counter at zero for the next launch (no reset
+                // kernel per launch; split launches set it up anew)
+                if constexpr (!SV) {
+                    if (lane == 0 && item == n_items + (int)gridDim.x - 1) *a.counter = 0u;
+                }
+                break;
+            }
         }
 
         const int pidx = item * (64 / G) + grp;
@@ -1151,9 +1160,10 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                                   : (has ? a.q_off[p] : 0);
         const int64_t to = online ? r_to : (has ? a.t_off[p] : 0);
         const int maxq = (G == 64) ? qlen : wave_max(qlen);
-        // the longest pairs set a batch's end: at raised wave priority they
-        // keep their SIMD's issue slots, the other waves there fill their stalls
-        const bool prio = !SV && a.prio_rows > 0 && maxq >= a.prio_rows;
+        // the longest pairs set a batch's end (and a served class's tail
+        // latency): at raised wave priority they keep their SIMD's issue
+        // slots, the other waves there fill their stalls
+        const bool prio = a.prio_rows > 0 && maxq >= a.prio_rows;
         if (prio) __builtin_amdgcn_s_setprio(2);
 
         // serving with paged planes: the request's pages before its forward
@@ -1604,6 +1614,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 // after the release above (which wrote this L2's dirty plane
                 // lines back), the pages may go to a wave on another XCD
                 if (paged) free_pages(a.page_owner, pl, need_pages);
+                if (prio) __builtin_amdgcn_s_setprio(0);
                 continue;
             }
         }

@@ -1,6 +1,7 @@
-// Forward-kernel variants of the narrow tiles: 2 and 4 x 64, and the 16- and
+// Forward-kernel variants of the narrow tiles: 2 and 4 x 64, the 16- and
 // 32-lane tiles that pack several short-band pairs into one wavefront (W = 65
-// as 5 x 13 among them)
+// as 5 x 13 among them), and W = 65 as one cell per lane plus the scalar cell
+// (the latency tile of small short-read batches)
 // (one translation unit per group of variants, so the library builds in
 // parallel; the host side that picks among them is sw.hip).
 #include "sw_kernels.h"
@@ -18,6 +19,7 @@ const TileEntry *sw_tiles_small(int *n)
         GMFA_TILE(8, 32, -1),
         GMFA_TILE(2, 64, -1),
         GMFA_TILE(4, 64, -1),
+        GMFA_TILE(1, 64, 1),
     };
     *n = (int)(sizeof(entries) / sizeof(entries[0]));
     return entries;

@@ -212,13 +212,32 @@ def resolve_tile(half_band: int, cells_per_lane: int = 0, lanes_per_pair: int = 
     return t
 
 
+def narrow_tile(half_band: int) -> Optional[SwTile]:
+    """The 64-lane tile with the fewest cells per lane that holds the band
+    (one pair per wavefront): for W = 64*C + 1, C cells per lane plus the
+    scalar cell (W = 65: one cell per lane), else the smallest general
+    64-lane tile; None if none holds it."""
+    W = 2 * int(half_band) + 1
+    shapes = sorted(C for C, G in list_tiles() if G == 64)
+    cands = ([(W - 1) // 64] if (W - 1) % 64 == 0 else []) + [C for C in shapes if C * 64 >= W]
+    for C in cands:
+        if C not in shapes:
+            continue
+        try:  # a scalar-cell variant exists for some (C, W = 64*C + 1) only
+            return resolve_tile(half_band, C, 64)
+        except (RuntimeError, ValueError):
+            continue
+    return None
+
+
 def batch_tile(half_band: int, n_pairs: int, device=None) -> SwTile:
     """Default tile for a batch of ``n_pairs``.  The library's default packs
     several short-band pairs into one wavefront (16- or 32-lane groups), the
     least padding per cell; when that leaves fewer than 4 wavefronts per SIMD,
-    a 64-lane tile (one pair per wavefront, the smallest that holds the band)
-    spreads the pairs over the idle SIMDs and finishes sooner (1,024 pairs of
-    150 bp at half_band 32: 0.078 ms on 2x64 against 0.13 ms on 5x16)."""
+    a 64-lane tile (one pair per wavefront, the fewest cells per lane:
+    ``narrow_tile``) spreads the pairs over the idle SIMDs and finishes sooner
+    (1,024 pairs of 150 bp at half_band 32: 0.078 ms on 2x64 against 0.13 ms
+    on 5x16)."""
     t = resolve_tile(half_band)
     if t.lanes_per_pair == 64:
         return t
@@ -226,11 +245,7 @@ def batch_tile(half_band: int, n_pairs: int, device=None) -> SwTile:
     waves = -(-int(n_pairs) // (64 // t.lanes_per_pair))
     if waves >= 4 * 4 * cus:
         return t
-    W = 2 * int(half_band) + 1
-    for C, G in sorted(list_tiles()):
-        if G == 64 and C * G >= W:
-            return resolve_tile(half_band, C, G)
-    return t
+    return narrow_tile(half_band) or t
 
 
 def list_tiles() -> list[tuple[int, int]]:

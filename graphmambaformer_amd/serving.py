@@ -33,14 +33,13 @@ def serve_tile(half_band: int) -> SwTile:
     """The compiled tile with one pair per wavefront (64 lanes) and the least
     padding for this band (for W = 64*C + 1, C cells per lane plus the scalar
     cell)."""
-    W = 2 * int(half_band) + 1
     t = kernels.resolve_tile(half_band)
     if t.lanes_per_pair == 64:
         return t
-    fits = [C for C, G in kernels.list_tiles() if G == 64 and C * 64 >= W]
-    if not fits:
-        raise RuntimeError(f"no 64-lane tile holds a band of width {W}")
-    return kernels.resolve_tile(half_band, min(fits), 64)
+    t = kernels.narrow_tile(half_band)
+    if t is None:
+        raise RuntimeError(f"no 64-lane tile holds a band of width {2 * int(half_band) + 1}")
+    return t
 
 
 @dataclass
@@ -147,6 +146,7 @@ class ServeClassRun:
     pages: Optional[PageGeometry] = None
     page_owner: Optional[torch.Tensor] = None
     page_wait_s: float = 0.0
+    prio_rows: int = 0  # requests with at least this many query rows run at raised priority
 
     @property
     def n(self) -> int:
@@ -157,7 +157,8 @@ class ServeClassRun:
         s.batch = self.inp.batch_struct(None)
         ma, mi, go, ge = self.scoring
         flags = _lib.SW_TRACEBACK | _lib.SW_CIGAR
-        s.params = kernels.make_params(ma, mi, go, ge, self.half_band, flags, self.tile)
+        s.params = kernels.make_params(ma, mi, go, ge, self.half_band, flags, self.tile,
+                                       prio_rows=self.prio_rows)
         o = s.out
         o.score, o.query_end, o.target_end = ptr(self.score), ptr(self.qend), ptr(self.tend)
         o.tb, o.tb_off = ptr(self.tb), None
@@ -182,11 +183,12 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
                tile: Optional[SwTile] = None,
                plane_budget_bytes: Optional[int] = None,
                page_blocks: Optional[int] = None,
-               page_wait_s: float = 0.0) -> ServeClassRun:
+               page_wait_s: float = 0.0, prio_rows: int = 0) -> ServeClassRun:
     """Request table for pool entries `picks` (in arrival order).  With
     `plane_budget_bytes`, the traceback planes come from a pool of pages of
     that size and requests are admitted by their planes' footprint; without,
-    every slot holds a slab for the class's longest query."""
+    every slot holds a slab for the class's longest query.  Requests with at
+    least `prio_rows` query rows (> 0) run at raised wave priority."""
     dev = reference.words.device
     picks = np.asarray(picks, np.int64)
     n = len(picks)
@@ -233,7 +235,7 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
         tb=torch.empty(pages.n_pages * pages.page_words if pages else int(n_slots) * slab,
                        dtype=torch.int32, device=dev),
         counter=torch.zeros(16, dtype=torch.int32, device=dev), walk=walk, picks=picks,
-        pages=pages, page_owner=owner, page_wait_s=page_wait_s)
+        pages=pages, page_owner=owner, page_wait_s=page_wait_s, prio_rows=int(prio_rows))
 
 
 @dataclass
@@ -289,6 +291,7 @@ class OnlineClass:
     plane_budget_bytes: Optional[int] = None  # paged planes (admission by footprint)
     page_blocks: Optional[int] = None  # 8-row blocks per page (default: about 1 MB pages)
     page_wait_s: float = 0.0  # bound of one wait for pages (0: the library's default)
+    prio_rows: int = 0  # requests with at least this many query rows run at raised priority
 
 
 @dataclass
@@ -359,7 +362,8 @@ class ServeSession:
             self._keep += [tb, stage, head]
             c = cfgs[k]
             c.params = kernels.make_params(ma, mi, go, ge, int(oc.half_band),
-                                           _lib.SW_TRACEBACK | _lib.SW_CIGAR, tile)
+                                           _lib.SW_TRACEBACK | _lib.SW_CIGAR, tile,
+                                           prio_rows=int(oc.prio_rows))
             c.max_qlen, c.ring, c.n_slots, c.cigar_cap = int(oc.max_qlen), ring, int(oc.n_slots), cap
             c.tb, c.slab_words, c.stage, c.head = ptr(tb), slab, ptr(stage), ptr(head)
             if pages is not None:

@@ -103,9 +103,11 @@ typedef struct gmfa_sw_params {
      * launch.  0 / 0: the plain launch. */
     int32_t split_rows;
     int32_t split_slices;
-    /* prio_rows > 0 (gmfa_sw_forward): pairs with at least prio_rows query rows
-     * run at raised wave priority, so the longest pairs of an uneven batch
-     * keep their SIMD's issue slots and the other waves fill their stalls. */
+    /* prio_rows > 0 (gmfa_sw_forward, and a serving class's params): pairs
+     * (requests) with at least prio_rows query rows run at raised wave
+     * priority, so the longest pairs of an uneven batch, or a class's longest
+     * requests, keep their SIMD's issue slots and the other waves fill their
+     * stalls. */
     int32_t prio_rows;
     /* max_waves > 0 (gmfa_sw_forward): the persistent launch holds at most this
      * many wavefronts (default: as many as fit on the device at once). */
@@ -143,7 +145,10 @@ typedef struct gmfa_sw_out {
                               maximum H at its first band diagonal d */
     const int64_t *rowkey_off; /* [n_pairs] element offset of each pair's keys */
     uint32_t *work_counter;    /* one device uint32 of scratch for the work queue (4 for
-                                  split launches) */
+                                  split launches): zero before the first launch that uses
+                                  it; every launch leaves word 0 at zero (its last
+                                  wavefront resets it), so it needs no clearing between
+                                  launches */
     /* GMFA_SW_CIGAR outputs, same layout as gmfa_sw_traceback's */
     const int64_t *cigar_off;
     uint32_t *cigar;
@@ -171,8 +176,11 @@ int gmfa_device_info(int32_t *cu_count, int32_t *warp_size, char *arch_name, int
 int gmfa_sw_resolve_tile(int32_t half_band, int32_t cells_per_lane, int32_t lanes_per_pair,
                          gmfa_sw_tile *tile);
 
-/* Compiled (cells_per_lane, lanes_per_pair) tiles; returns how many exist and
- * fills up to cap entries. */
+/* Compiled (cells_per_lane, lanes_per_pair) tile shapes, each once: the
+ * general tiles (any band up to cells_per_lane x lanes_per_pair), then the
+ * shapes compiled only for one band (1 x 64 plus a scalar cell: W = 65);
+ * returns how many exist and fills up to cap entries.  gmfa_sw_resolve_tile
+ * tells whether a shape holds a given band. */
 int gmfa_sw_list_tiles(int32_t *cells_per_lane, int32_t *lanes_per_pair, int32_t cap);
 
 /* Slice counts of the compiled split variants for a band (half_band, with the

@@ -107,18 +107,30 @@ def test_sw_every_compiled_tile():
     scoring = (2, 4, 4, 2)
     for C, G in K().list_tiles():
         cap = C * G
-        # exact fit (if odd), partial last lane, and a band far narrower than the tile
-        hbs = sorted({(cap - 1) // 2, max(1, (cap - 1) // 2 - 3), max(1, cap // 4)})
+        # exact fit (if odd), partial last lane, and a band far narrower than the tile;
+        # a shape compiled for one band only (1 x 64 + the scalar cell: W = 65) runs
+        # that band
+        hbs = sorted({(cap - 1) // 2, max(1, (cap - 1) // 2 - 3), max(1, cap // 4), cap // 2})
+        served = []
         for hb in hbs:
-            _say("case: tile", C, G, "hb", hb)
+            try:
+                K().resolve_tile(hb, C, G)
+                served.append(hb)
+            except (RuntimeError, ValueError):
+                continue
+        assert served, (C, G)
+        for hb in served:
             qs, ts, offs = [], [], []
             for p in range(12):
                 q, t = _random_pair(rng, int(rng.integers(1, 200)), with_n=(p % 4 == 1))
                 qs.append(q)
                 ts.append(t)
                 offs.append(int(rng.integers(-3, 30)))
-            got = _gpu_align(qs, ts, offs, hb, scoring, tile=(C, G), fused=(hb % 2 == 1))
-            _check_against_oracle(qs, ts, offs, hb, scoring, got, f"tile{C}x{G}hb{hb}")
+            # both walks for a one-band shape, else alternating by band
+            for fused in ((False, True) if len(served) == 1 else (hb % 2 == 1,)):
+                _say("case: tile", C, G, "hb", hb, "fused", fused)
+                got = _gpu_align(qs, ts, offs, hb, scoring, tile=(C, G), fused=fused)
+                _check_against_oracle(qs, ts, offs, hb, scoring, got, f"tile{C}x{G}hb{hb}")
 
 
 @pytest.mark.parametrize("fused", [False, True])

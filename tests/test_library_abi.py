@@ -245,3 +245,29 @@ def test_drop_ins_reject_bands_and_lookbacks_beyond_the_kernels():
     with pytest.raises(ValueError, match="max_lookback 65"):
         C.chain_dp_numpy(x, x, np.ones(5), ChainingConfig(max_lookback=65))
     C.AffineChainer(ChainingConfig(max_lookback=C.MAX_LOOKBACK))
+
+
+def test_narrow_and_batch_tiles_on_the_host():
+    """The 64-lane tile a small batch or a serving class gets for each band
+    (host-side tile tables only, no device work): the scalar-cell tiles for
+    W = 64*C + 1, else the smallest general 64-lane tile that holds the band;
+    large batches keep the library's default."""
+    from graphmambaformer_amd import kernels as K
+
+    def shape(t):
+        return (t.cells_per_lane, t.lanes_per_pair, t.lanes_stored)
+
+    assert shape(K.narrow_tile(32)) == (1, 64, 65)     # W = 65: 1 x 64 + the scalar cell
+    assert shape(K.narrow_tile(64)) == (4, 64, 33)     # W = 129: 2 x 64 holds 128 only
+    assert shape(K.narrow_tile(256)) == (8, 64, 65)    # W = 513: 8 x 64 + the scalar cell
+    assert shape(K.narrow_tile(751)) == (24, 64, 63)
+    assert shape(K.narrow_tile(2)) == (2, 64, 3)
+    # the default for W = 65 packs four pairs to a wavefront; a small batch
+    # (fewer than 4 wavefronts per SIMD of a 256-CU device) takes the narrow tile
+    assert shape(K.resolve_tile(32))[:2] == (5, 16)
+    assert shape(K.batch_tile(32, 1024)) == (1, 64, 65)
+    assert shape(K.batch_tile(32, 64 * 4 * 4 * 256))[:2] == (5, 16)
+    for hb in (2, 7, 31, 32, 33, 64, 100, 255, 256, 300, 751, 1023):
+        t = K.narrow_tile(hb)
+        assert t is not None and t.lanes_per_pair == 64, hb
+        assert t.lanes_stored * t.cells_per_lane >= 2 * hb + 1, hb
