@@ -177,10 +177,14 @@ SERVE_SLOTS = (512, 1024, 2048)  # persistent wavefronts of the short / long / n
 # admitted when its planes' pages are free); per-slot slabs for each class's
 # longest request would take 0.02 / 6.6 / 156 GB
 SERVE_PLANE_BUDGET = (256 << 20, 8 << 30, 32 << 30)
-# requests of the noisy class with at least this many query rows (the longest
-# ~8% of its lognormal(8 kb, 1.0) lengths) run at raised wave priority: the
-# class's tail latency is its longest requests' own run time
-SERVE_PRIO_ROWS = 32768
+# staged serving, --serve-prio-rows R: requests of the noisy class with at
+# least R query rows run at raised wave priority.  Off by default: priority
+# moves the tail between classes rather than shortening it.  R = 32,768 (the
+# longest ~8% of the noisy lengths), two runs each: noisy p99 146-148 -> 120
+# ms and capacity 301-311 k -> 338 k req/s, but long-class p99 18 -> 39 ms,
+# short-class p99 0.17 -> 0.34 ms, overall p99 51.9 -> 52.5 ms; online form
+# (not applied there): noisy p99 161 -> 173 ms.
+SERVE_PRIO_ROWS = 0
 
 
 def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO_ROWS):
@@ -261,7 +265,7 @@ def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO
         "completed": r.completed,
         "timed_out": r.timed_out,
         "slots": dict(zip([p.name for p in mix.pools], SERVE_SLOTS)),
-        "noisy_prio_rows": prio_rows,
+        "noisy_prio_rows": prio_rows,  # staged form only
         "planes": pages,
         "workload": f"{n_req} Poisson requests, 60% short (150 bp, half_band 32) / 30% long "
                     "(lognormal 12 kb, half_band 256) / 10% noisy (lognormal 8 kb, 6% errors, "
@@ -276,7 +280,7 @@ def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO
     torch.cuda.empty_cache()
     if online:
         res["online"] = online_serving_extra(torch, S, mix, ref, cls, idx, total_cells, load, sc,
-                                             prio_rows)
+                                             prio_rows=0)
     return res
 
 
@@ -629,6 +633,130 @@ def single_batch(torch, K, WL, dev, args):
                                INT32_PEAK_TOPS, 4)}
 
 
+def chains_extra(torch, K, WL, dev, threads):
+    """The chaining configuration (4,096 reads, Poisson(400) anchors): the DP
+    in the integer and float64 profiles with their VALU / HBM fractions, the
+    oracle's DP on every host core, and anchors -> chains on the device (sort,
+    DP, backtrack, select) against the oracle's per-read chain_read."""
+    from oracle import oracle as O
+
+    wc = WL.chains()
+    re_ = torch.from_numpy(wc.read_end).to(dev)
+    fe_ = torch.from_numpy(wc.ref_end).to(dev)
+    wt = torch.from_numpy(wc.weight).to(dev)
+    so = torch.from_numpy(wc.seg_off).to(dev)
+
+    def chain_step():
+        K.chain_dp(re_, fe_, wt, so, wc.max_lookback, wc.max_gap, gap_open=wc.gap_open,
+                   gap_extend=wc.gap_extend)
+
+    chain_step()
+    cms = time_events(torch, chain_step, 5)
+    from graphmambaformer_amd.alignment.chaining import penalty_table
+    from graphmambaformer_amd.config import ChainingConfig
+
+    lut = penalty_table(ChainingConfig(max_lookback=wc.max_lookback, max_gap=wc.max_gap), dev)
+    wt64 = wt.to(torch.float64)
+
+    def chain_step64():
+        K.chain_dp(re_, fe_, wt64, so, wc.max_lookback, wc.max_gap, pen_lut=lut)
+
+    chain_step64()
+    cms64 = time_events(torch, chain_step64, 5)
+    # CPU: the oracle's C DP per read, reads fanned over every host core
+    # (chaining.py:657-666 runs reads in a thread pool; the C call releases
+    # the interpreter lock)
+    from concurrent.futures import ThreadPoolExecutor
+
+    def cpu_read(s_):
+        a, b = wc.seg_off[s_], wc.seg_off[s_ + 1]
+        O.chain_dp_i32(wc.read_end[a:b], wc.ref_end[a:b], wc.weight[a:b], wc.max_lookback,
+                       wc.max_gap, wc.gap_open, wc.gap_extend)
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(cpu_read, range(64)))
+        t1 = time.perf_counter()
+        list(ex.map(cpu_read, range(wc.n_reads)))
+        ccpu = time.perf_counter() - t1
+    # anchors -> chains on the device: sort into DP order, DP (float64 reference
+    # defaults), backtrack, select; host arrays in, chain records out
+    ccfg = ChainingConfig(max_lookback=wc.max_lookback, max_gap=wc.max_gap)
+    nrd = wc.n_reads
+    seg_of = np.repeat(np.arange(nrd), np.diff(wc.seg_off))
+    shuf = np.lexsort((np.random.default_rng(0).random(len(seg_of)), seg_of))  # unsorted input
+    rp_ = wc.read_end[shuf] - 15
+    fp_ = wc.ref_end[shuf] - 15
+    ids = (np.arange(len(seg_of)) - wc.seg_off[seg_of])[shuf]
+    read_seg = np.stack([np.arange(nrd), np.full(nrd, -1)], axis=1).astype(np.int32)
+    lut64 = penalty_table(ccfg, dev)
+
+    def pipeline():
+        return K.chain_pipeline(rp_ + 15, fp_ + 15, rp_, fp_, np.full(len(rp_), 15.0), ids,
+                                wc.seg_off, np.ones(nrd, np.int32), read_seg, wc.seg_off[:-1],
+                                ccfg.max_lookback, ccfg.max_gap, lut64, ccfg.min_chain_anchors,
+                                ccfg.min_chain_score, ccfg.secondary_score_ratio,
+                                ccfg.secondary_overlap, ccfg.max_chains, dev)
+
+    pipeline()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(3):
+        pres = pipeline()
+    pms_host = (time.perf_counter() - t1) / 3 * 1e3
+    up_ = K.chain_upload(rp_ + 15, fp_ + 15, rp_, fp_, np.full(len(rp_), 15.0), ids, wc.seg_off,
+                         read_seg, wc.seg_off[:-1], dev)
+
+    def device_run():
+        return K.chain_run(up_, ccfg.max_lookback, ccfg.max_gap, lut64, ccfg.min_chain_anchors,
+                           ccfg.min_chain_score, ccfg.secondary_score_ratio,
+                           ccfg.secondary_overlap, ccfg.max_chains)
+
+    device_run()
+    pms = time_events(torch, device_run, 5)
+    sub = np.arange(0, nrd, 16)
+    t1 = time.perf_counter()
+    for r in sub:
+        m = seg_of[shuf] == r
+        O.chain_read(rp_[m], fp_[m], np.full(int(m.sum()), 15), np.ones(int(m.sum()), np.int64),
+                     max_lookback=ccfg.max_lookback, max_gap=ccfg.max_gap)
+    cpu_pipe = (time.perf_counter() - t1) / len(sub)
+    evals = wc.pair_evaluations()
+    n_anchor = int(wc.seg_off[-1])
+    res = {
+        "reads_per_s": round(wc.n_reads / (cms / 1e3), 1),
+        "pair_evals_per_s": round(evals / (cms / 1e3), 1),
+        "roofline": {
+            "bound": "valu_int32",
+            "frac": round(evals * CHAIN_OPS_PER_EVAL / (cms / 1e3) / 1e12 / INT32_PEAK_TOPS, 4),
+            "ops_per_eval": CHAIN_OPS_PER_EVAL,
+            "hbm_frac": round(n_anchor * CHAIN_BYTES_PER_ANCHOR / (cms / 1e3) / 1e9 /
+                              HBM_PEAK_GBS, 5),
+            "bytes_per_anchor": CHAIN_BYTES_PER_ANCHOR,
+            "note": "integer profile DP; the DP is sequential in the anchor index, so one "
+                    "wavefront per read runs at its latency, not at the VALU or HBM roof",
+        },
+        "ms": round(cms, 4),
+        "f64_ref_default_ms": round(cms64, 4),
+        "f64_reads_per_s": round(wc.n_reads / (cms64 / 1e3), 1),
+        "cpu_reads_per_s": round(wc.n_reads / ccpu, 1),
+        "cpu_threads": threads,
+        "anchors_to_chains_ms": round(pms, 3),
+        "anchors_to_chains_reads_per_s": round(nrd / (pms / 1e3), 1),
+        "anchors_to_chains_with_host_ms": round(pms_host, 3),
+        "anchors_to_chains_kept": int(pres.n_kept.sum()),
+        "cpu_anchors_to_chains_reads_per_s": round(1.0 / cpu_pipe, 1),
+        "workload": "4096 reads, Poisson(400) anchors, lookback 50, max_gap 5000, "
+                    "int profile 6/1 (and float64 6/0.05/0.5), seed 1; CPU: whole batch, "
+                    "int profile, the oracle's C DP per read on every host core; "
+                    "anchors_to_chains: unsorted anchors resident on "
+                    "the device, sort + float64 DP + backtrack + select (with_host adds the "
+                    "uploads and the readback of the chain records); CPU: the oracle's "
+                    "chain_read on every 16th read",
+    }
+    log(f"[bench] chains: {res}")
+    return res
+
+
 def host_info():
     model = "unknown"
     try:
@@ -663,16 +791,18 @@ def main():
                     help="skip the online-session serving leg (keeps a profiled run short: "
                          "its Python front end makes ~10^5 library calls)")
     ap.add_argument("--serve-prio-rows", type=int, default=SERVE_PRIO_ROWS,
-                    help="noisy-class requests with at least this many query rows run at "
-                         "raised wave priority (0: none)")
+                    help="staged serving: noisy-class requests with at least this many query "
+                         "rows run at raised wave priority (0: none)")
     ap.add_argument("--serve-load", type=float, default=0.65,
                     help="offered load of the paced serving run, as a fraction of the "
                          "saturation capacity")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
-    ap.add_argument("--workload", choices=("noisy", "long", "short"), default="noisy",
+    ap.add_argument("--workload", choices=("noisy", "long", "short", "serving", "chains"),
+                    default="noisy",
                     help="long / short: only time the long-read or the short-read batch's step "
-                         "(profiling runs; --tile pins its tile, else the library's default)")
+                         "(profiling runs; --tile pins its tile, else the library's default); "
+                         "serving / chains: only that measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -697,6 +827,14 @@ def main():
         return
 
     threads = os.cpu_count() or 1
+    if args.workload == "serving":
+        print(json.dumps(serving_extra(torch, WL, dev, args.serve_requests, args.serve_load,
+                                       online=not args.no_online,
+                                       prio_rows=args.serve_prio_rows)), flush=True)
+        return
+    if args.workload == "chains":
+        print(json.dumps(chains_extra(torch, K, WL, dev, threads)), flush=True)
+        return
     if args.workload != "noisy":
         print(json.dumps(single_batch(torch, K, WL, dev, args)), flush=True)
         return
@@ -949,122 +1087,7 @@ def main():
         torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ chains
-    from oracle import oracle as O
-
-    wc = WL.chains()
-    re_ = torch.from_numpy(wc.read_end).to(dev)
-    fe_ = torch.from_numpy(wc.ref_end).to(dev)
-    wt = torch.from_numpy(wc.weight).to(dev)
-    so = torch.from_numpy(wc.seg_off).to(dev)
-
-    def chain_step():
-        K.chain_dp(re_, fe_, wt, so, wc.max_lookback, wc.max_gap, gap_open=wc.gap_open,
-                   gap_extend=wc.gap_extend)
-
-    chain_step()
-    cms = time_events(torch, chain_step, 5)
-    from graphmambaformer_amd.alignment.chaining import penalty_table
-    from graphmambaformer_amd.config import ChainingConfig
-
-    lut = penalty_table(ChainingConfig(max_lookback=wc.max_lookback, max_gap=wc.max_gap), dev)
-    wt64 = wt.to(torch.float64)
-
-    def chain_step64():
-        K.chain_dp(re_, fe_, wt64, so, wc.max_lookback, wc.max_gap, pen_lut=lut)
-
-    chain_step64()
-    cms64 = time_events(torch, chain_step64, 5)
-    # CPU: the oracle's C DP per read, reads fanned over every host core
-    # (chaining.py:657-666 runs reads in a thread pool; the C call releases
-    # the interpreter lock)
-    from concurrent.futures import ThreadPoolExecutor
-
-    def cpu_read(s_):
-        a, b = wc.seg_off[s_], wc.seg_off[s_ + 1]
-        O.chain_dp_i32(wc.read_end[a:b], wc.ref_end[a:b], wc.weight[a:b], wc.max_lookback,
-                       wc.max_gap, wc.gap_open, wc.gap_extend)
-
-    with ThreadPoolExecutor(threads) as ex:
-        list(ex.map(cpu_read, range(64)))
-        t1 = time.perf_counter()
-        list(ex.map(cpu_read, range(wc.n_reads)))
-        ccpu = time.perf_counter() - t1
-    # anchors -> chains on the device: sort into DP order, DP (float64 reference
-    # defaults), backtrack, select; host arrays in, chain records out
-    ccfg = ChainingConfig(max_lookback=wc.max_lookback, max_gap=wc.max_gap)
-    nrd = wc.n_reads
-    seg_of = np.repeat(np.arange(nrd), np.diff(wc.seg_off))
-    shuf = np.lexsort((np.random.default_rng(0).random(len(seg_of)), seg_of))  # unsorted input
-    rp_ = wc.read_end[shuf] - 15
-    fp_ = wc.ref_end[shuf] - 15
-    ids = (np.arange(len(seg_of)) - wc.seg_off[seg_of])[shuf]
-    read_seg = np.stack([np.arange(nrd), np.full(nrd, -1)], axis=1).astype(np.int32)
-    lut64 = penalty_table(ccfg, dev)
-
-    def pipeline():
-        return K.chain_pipeline(rp_ + 15, fp_ + 15, rp_, fp_, np.full(len(rp_), 15.0), ids,
-                                wc.seg_off, np.ones(nrd, np.int32), read_seg, wc.seg_off[:-1],
-                                ccfg.max_lookback, ccfg.max_gap, lut64, ccfg.min_chain_anchors,
-                                ccfg.min_chain_score, ccfg.secondary_score_ratio,
-                                ccfg.secondary_overlap, ccfg.max_chains, dev)
-
-    pipeline()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(3):
-        pres = pipeline()
-    pms_host = (time.perf_counter() - t1) / 3 * 1e3
-    up_ = K.chain_upload(rp_ + 15, fp_ + 15, rp_, fp_, np.full(len(rp_), 15.0), ids, wc.seg_off,
-                         read_seg, wc.seg_off[:-1], dev)
-
-    def device_run():
-        return K.chain_run(up_, ccfg.max_lookback, ccfg.max_gap, lut64, ccfg.min_chain_anchors,
-                           ccfg.min_chain_score, ccfg.secondary_score_ratio,
-                           ccfg.secondary_overlap, ccfg.max_chains)
-
-    device_run()
-    pms = time_events(torch, device_run, 5)
-    sub = np.arange(0, nrd, 16)
-    t1 = time.perf_counter()
-    for r in sub:
-        m = seg_of[shuf] == r
-        O.chain_read(rp_[m], fp_[m], np.full(int(m.sum()), 15), np.ones(int(m.sum()), np.int64),
-                     max_lookback=ccfg.max_lookback, max_gap=ccfg.max_gap)
-    cpu_pipe = (time.perf_counter() - t1) / len(sub)
-    evals = wc.pair_evaluations()
-    n_anchor = int(wc.seg_off[-1])
-    result["chains"] = {
-        "reads_per_s": round(wc.n_reads / (cms / 1e3), 1),
-        "pair_evals_per_s": round(evals / (cms / 1e3), 1),
-        "roofline": {
-            "bound": "valu_int32",
-            "frac": round(evals * CHAIN_OPS_PER_EVAL / (cms / 1e3) / 1e12 / INT32_PEAK_TOPS, 4),
-            "ops_per_eval": CHAIN_OPS_PER_EVAL,
-            "hbm_frac": round(n_anchor * CHAIN_BYTES_PER_ANCHOR / (cms / 1e3) / 1e9 /
-                              HBM_PEAK_GBS, 5),
-            "bytes_per_anchor": CHAIN_BYTES_PER_ANCHOR,
-            "note": "integer profile DP; the DP is sequential in the anchor index, so one "
-                    "wavefront per read runs at its latency, not at the VALU or HBM roof",
-        },
-        "ms": round(cms, 4),
-        "f64_ref_default_ms": round(cms64, 4),
-        "f64_reads_per_s": round(wc.n_reads / (cms64 / 1e3), 1),
-        "cpu_reads_per_s": round(wc.n_reads / ccpu, 1),
-        "cpu_threads": threads,
-        "anchors_to_chains_ms": round(pms, 3),
-        "anchors_to_chains_reads_per_s": round(nrd / (pms / 1e3), 1),
-        "anchors_to_chains_with_host_ms": round(pms_host, 3),
-        "anchors_to_chains_kept": int(pres.n_kept.sum()),
-        "cpu_anchors_to_chains_reads_per_s": round(1.0 / cpu_pipe, 1),
-        "workload": "4096 reads, Poisson(400) anchors, lookback 50, max_gap 5000, "
-                    "int profile 6/1 (and float64 6/0.05/0.5), seed 1; CPU: whole batch, "
-                    "int profile, the oracle's C DP per read on every host core; "
-                    "anchors_to_chains: unsorted anchors resident on "
-                    "the device, sort + float64 DP + backtrack + select (with_host adds the "
-                    "uploads and the readback of the chain records); CPU: the oracle's "
-                    "chain_read on every 16th read",
-    }
-    log(f"[bench] chains: {result['chains']}")
+    result["chains"] = chains_extra(torch, K, WL, dev, threads)
 
     # ----------------------------------------------------------------- seeding
     result["seeding"] = seeding_extra(torch, WL, dev)

@@ -80,10 +80,13 @@ struct SortArgs {
 };
 
 // Segments of at most this many anchors sort (and backtrack) with their keys
-// and permutations in LDS: the merge sort's binary searches are chains of
-// dependent loads, ~100 cycles each from LDS against ~1 us from L2/HBM.
-// Larger segments (any size) use global scratch.
-constexpr int kLdsAnchors = 1024;
+// and permutations in LDS: the merge sort's binary searches and the claimed
+// walk are chains of dependent loads, ~100 cycles each from LDS against ~1 us
+// from L2/HBM.  Larger segments (any size) use global scratch.  The bound
+// sets the LDS of a workgroup (backtrack: 33 B per anchor, 16.5 KB), so 9
+// segments walk per CU at once (4 at a bound of 1,024); Poisson(400) anchor
+// sets stay under it.
+constexpr int kLdsAnchors = 512;
 
 __global__ __launch_bounds__(256) void chain_sort_kernel(SortArgs a)
 {
@@ -183,9 +186,9 @@ __device__ __forceinline__ void chain_record(const BtArgs &a, int64_t base, int6
 __global__ __launch_bounds__(256) void chain_backtrack_kernel(BtArgs a)
 {
     __shared__ double l_f[kLdsAnchors];
-    __shared__ int32_t l_par[kLdsAnchors], p_a[kLdsAnchors], p_b[kLdsAnchors];
+    // l_pc[i]: parent(i) + 1 in bits 0-30, bit 31 set once i is claimed
+    __shared__ int32_t l_pc[kLdsAnchors], p_a[kLdsAnchors], p_b[kLdsAnchors];
     __shared__ int32_t l_walk[kLdsAnchors], l_start[kLdsAnchors], l_cnt[kLdsAnchors];
-    __shared__ uint8_t l_claim[kLdsAnchors];
     __shared__ int l_nc;
     const int s = blockIdx.x;
     const int64_t base = a.seg_off[s];
@@ -204,36 +207,59 @@ __global__ __launch_bounds__(256) void chain_backtrack_kernel(BtArgs a)
                 atomicOr(a.bad_parent, 1);
                 pi = -1;
             }
-            l_par[i] = (int32_t)pi;
-            l_claim[i] = 0;
+            l_pc[i] = (int32_t)(pi < 0 ? 0 : pi + 1);
         }
         __syncthreads();
         const int32_t *order =
             merge_sort_perm(n, p_a, p_b, [&](int32_t x, int32_t y) { return l_f[x] > l_f[y]; });
-        if (threadIdx.x == 0) {
-            int nc = 0, nm = 0;
-            for (int r = 0; r < n; ++r) {
-                const int i = order[r];
-                if (l_claim[i]) continue;
+        if (threadIdx.x < 64) {
+            // wavefront 0: the lanes look at 64 anchors of the order at once
+            // for the next unclaimed one (claims only change inside a walk,
+            // and the next look starts right after the anchor walked); the
+            // walk itself is wave-uniform, one dependent LDS load per member
+            const int lane = (int)threadIdx.x;
+            int nc = 0, nm = 0, r = 0;
+            while (r < n) {
+                const int rr = r + lane;
+                const int oi = rr < n ? order[rr] : 0;
+                const uint64_t m = __builtin_amdgcn_ballot_w64(rr < n && l_pc[oi] >= 0);
+                if (m == 0ull) {
+                    r += 64;
+                    continue;
+                }
+                const int k = (int)__builtin_ctzll(m);
+                r += k;
+                const int i = __builtin_amdgcn_readlane(oi, k);
                 int node = i;
                 const int start = nm;
-                while (node >= 0 && !l_claim[node]) {
-                    l_claim[node] = 1;
-                    l_walk[nm++] = node;
-                    node = l_par[node];
+                int w = l_pc[node];
+                while (w >= 0) {  // node unclaimed
+                    if (lane == 0) {
+                        l_pc[node] = (int32_t)((uint32_t)w | 0x80000000u);
+                        l_walk[nm] = node;
+                    }
+                    ++nm;
+                    node = w - 1;
+                    if (node < 0) break;
+                    w = l_pc[node];
                 }
                 const double sc = l_f[i] - (node >= 0 ? l_f[node] : 0.0);
+                ++r;
                 if (nm - start < a.min_anchors || sc < a.min_score) {
                     nm = start;
                     continue;
                 }
-                a.c_score[base + nc] = sc;
-                l_start[nc] = start;
-                l_cnt[nc] = nm - start;
+                if (lane == 0) {
+                    a.c_score[base + nc] = sc;
+                    l_start[nc] = start;
+                    l_cnt[nc] = nm - start;
+                }
                 ++nc;
             }
-            l_nc = nc;
-            a.n_chains[s] = nc;
+            if (lane == 0) {
+                l_nc = nc;
+                a.n_chains[s] = nc;
+            }
         }
         __syncthreads();
         const int nc = l_nc;

@@ -744,15 +744,18 @@ def chain_run(u: dict, max_lookback: int, max_gap: int, pen_lut: torch.Tensor, m
                                 ptr(o_re), ptr(o_fe), ptr(o_w), ptr(o_rp), ptr(o_fp), ptr(o_id),
                                 s), "gmfa_chain_sort")
     f, parent = chain_dp(o_re, o_fe, o_w, u["seg_off"], max_lookback, max_gap, pen_lut=pen_lut)
+    # every output is written where it is read (n_chains and n_kept for every
+    # segment / read, records and members for the kept chains): no zero fill
+    # except the parent-error flag
     r = {"claimed": torch.empty(max(n, 1), dtype=torch.uint8, device=dev),
-         "n_chains": torch.zeros(max(ns, 1), dtype=torch.int32, device=dev),
-         "score": torch.zeros(max(n, 1), dtype=torch.float64, device=dev),
-         "span": torch.zeros((max(n, 1), 4), dtype=torch.int64, device=dev),
-         "mem": torch.zeros((max(n, 1), 2), dtype=torch.int64, device=dev),
-         "members": torch.zeros(max(n, 1), dtype=torch.int64, device=dev),
+         "n_chains": torch.empty(max(ns, 1), dtype=torch.int32, device=dev),
+         "score": torch.empty(max(n, 1), dtype=torch.float64, device=dev),
+         "span": torch.empty((max(n, 1), 4), dtype=torch.int64, device=dev),
+         "mem": torch.empty((max(n, 1), 2), dtype=torch.int64, device=dev),
+         "members": torch.empty(max(n, 1), dtype=torch.int64, device=dev),
          "bad": torch.zeros(1, dtype=torch.int32, device=dev),
-         "n_kept": torch.zeros(max(nr, 1), dtype=torch.int32, device=dev),
-         "kept": torch.zeros((max(nr, 1), int(max_chains)), dtype=torch.int64, device=dev)}
+         "n_kept": torch.empty(max(nr, 1), dtype=torch.int32, device=dev),
+         "kept": torch.empty((max(nr, 1), int(max_chains)), dtype=torch.int64, device=dev)}
     check(lib().gmfa_chain_backtrack_device(ptr(u["seg_off"]), ns, n, ptr(f), ptr(parent),
                                             ptr(o_rp), ptr(o_re), ptr(o_fp), ptr(o_fe), ptr(o_id),
                                             int(min_anchors), float(min_score), ptr(scratch),
