@@ -117,18 +117,12 @@ __device__ __forceinline__ uint32_t seq_bits(const uint32_t *src, int64_t words,
 // lanes load many dwords at once and each step picks its dword with a
 // readlane, so a dependent load is paid once per window instead of once per
 // plane block or 16-base word.  Plane window: 4 blocks x 16 diagonals (lane
-// l holds block top - (l >> 4), diagonal d0 + (l & 15)), plus the
-// kWinAhead windows above it, loaded ahead: a window is loaded 4 * kWinAhead
-// block moves (at least as many steps) before the walk enters it, so the
-// load's HBM round trip overlaps the steps in between (with one window
-// ahead, the longest noisy pair's walk waited on nearly every window).
-// Sequence window: 64 words (lane l holds word top - l); the walk only moves
-// towards lower words.
-constexpr int kWinAhead = 3;
+// l holds block top - (l >> 4), diagonal d0 + (l & 15)), plus the 4 blocks
+// above it loaded one window ahead.  Sequence window: 64 words (lane l holds
+// word top - l); the walk only moves towards lower words.
 struct PlaneWin {
     int top = INT_MIN / 2, d0 = 0;
-    uint32_t v = 0u;               // this window
-    uint32_t ahead[kWinAhead] = {};  // ahead[k]: the 4 blocks at top - 4 (k + 1)
+    uint32_t v = 0u, pv = 0u;  // this window, the 4 blocks above it
 };
 struct SeqWin {
     int top = -1;  // -1: nothing loaded
@@ -211,22 +205,14 @@ __device__ __forceinline__ void walk_steps(const WalkCtx &c, int &i, int &j, int
             int rb = pw.top - blk, rd = d - pw.d0;
             if ((uint32_t)rb > 3u || (uint32_t)rd > 15u) {
                 if ((uint32_t)(rb - 4) <= 3u && (uint32_t)rd <= 15u) {
-                    // up into the next window: shift the ones loaded ahead
-                    pw.v = pw.ahead[0];
-#pragma unroll
-                    for (int k = 0; k + 1 < kWinAhead; ++k) pw.ahead[k] = pw.ahead[k + 1];
+                    pw.v = pw.pv;
                     pw.top -= 4;
-                    pw.ahead[kWinAhead - 1] =
-                        plane_win_load(tb, pw.top - 4 * kWinAhead, pw.d0, blkstride, c.W);
                 } else {
-                    // sideways (or the first step): everything from here up
                     pw.top = blk;
                     pw.d0 = d - 8;
                     pw.v = plane_win_load(tb, pw.top, pw.d0, blkstride, c.W);
-#pragma unroll
-                    for (int k = 0; k < kWinAhead; ++k)
-                        pw.ahead[k] = plane_win_load(tb, pw.top - 4 * (k + 1), pw.d0, blkstride, c.W);
                 }
+                pw.pv = plane_win_load(tb, pw.top - 4, pw.d0, blkstride, c.W);
                 rb = pw.top - blk;
                 rd = d - pw.d0;
             }
