@@ -187,7 +187,23 @@ SERVE_PLANE_BUDGET = (256 << 20, 8 << 30, 32 << 30)
 SERVE_PRIO_ROWS = 0
 
 
-def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO_ROWS):
+# staged serving, --serve-split R,K,G,N: noisy-class requests with at least R
+# query rows go to a class of their own whose G workgroups run each request as
+# K band slices (K wavefronts); the noisy class keeps the shorter ones on N
+# wavefronts (R = 0: no split class).  Every class's persistent kernel must be
+# resident at once: per SIMD, the long class's wavefront (~104 VGPRs), half a
+# short one and the noisy ones (~224 each) leave room for split wavefronts
+# (~152 each) only if the noisy class gives some up; the split class launches
+# first so its workgroups are placed before the others fill the CUs.
+# Measured at 65% load, two runs: (32768, 3, 256, 1024) against none: noisy
+# p99 137.5 -> 88.5-90.7 ms, overall p99 48.8 -> 45.1-45.6 ms, long-class p99
+# 16.8 -> 22.2 ms, saturation 314 k -> 304-305 k req/s; (16384, 3, 256, 1024):
+# saturation 212 k; (32768, 3, 128, 1280): 198 k, noisy p99 114 ms.
+SERVE_SPLIT = (32768, 3, 256, 1024)
+
+
+def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO_ROWS,
+                  split=SERVE_SPLIT):
     """p50 / p99 latency of a Poisson request stream at a fixed rate.
 
     A resident 46 Mbp reference; requests 60% short (150 bp, half_band 32), 30%
@@ -212,28 +228,57 @@ def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO
     log(f"[bench] serving stream: {n_req} requests, {total_cells:.3e} cells, "
         f"{time.time() - t0:.1f}s")
 
+    # the staged run's classes: the three bands, and the long noisy requests
+    # in a split class when asked
+    split_rows, split_k, split_groups, split_noisy = (tuple(split) + (0,))[:4]
+    qlen_req = np.array([len(mix.pools[c].queries[k]) for c, k in zip(cls, idx)], np.int64)
+    scls = cls.copy()
+    noisy = [p.name for p in mix.pools].index("noisy")
+    if split_rows > 0:
+        scls[(cls == noisy) & (qlen_req >= split_rows)] = len(mix.pools)
+    slots = list(SERVE_SLOTS)
+    if split_rows > 0:
+        slots[noisy] = split_noisy or max(1, slots[noisy] - split_groups * split_k)
+    # plane budget of the split class: the part of the noisy class's budget
+    # its requests' planes take
+    split_frac = (float(req_cells[scls == len(mix.pools)].sum()) /
+                  max(float(req_cells[cls == noisy].sum()), 1.0)) if split_rows > 0 else 0.0
+    budgets = list(SERVE_PLANE_BUDGET)
+
     def build(arrivals):
+        """The classes in launch order, each with its class id (the index into
+        scls): a split class first, so its multi-wavefront workgroups are
+        placed before the one-wavefront classes take the remaining room."""
         out = []
+        if split_rows > 0:
+            sel = np.flatnonzero(scls == len(mix.pools))
+            out.append((len(mix.pools), S.make_class(
+                "noisy_split", ref, pools[noisy], idx[sel], arrivals[sel], sc, split_groups,
+                split_slices=split_k,
+                plane_budget_bytes=int(budgets[noisy] * split_frac) + (4 << 30))))
         for c, p in enumerate(mix.pools):
-            sel = np.flatnonzero(cls == c)
-            out.append(S.make_class(p.name, ref, pools[c], idx[sel], arrivals[sel], sc,
-                                    SERVE_SLOTS[c], plane_budget_bytes=SERVE_PLANE_BUDGET[c],
-                                    prio_rows=prio_rows if p.name == "noisy" else 0))
+            sel = np.flatnonzero(scls == c)
+            budget = budgets[c]
+            if c == noisy and split_rows > 0:
+                budget = int(budget * (1.0 - split_frac)) + (4 << 30)
+            out.append((c, S.make_class(p.name, ref, pools[c], idx[sel], arrivals[sel], sc,
+                                        slots[c], plane_budget_bytes=budget,
+                                        prio_rows=prio_rows if p.name == "noisy" else 0)))
         return out
 
     def gather(classes):
         comp = np.empty(n_req)
-        for c, k in enumerate(classes):
-            comp[np.flatnonzero(cls == c)] = k.complete_s
+        for c, k in classes:
+            comp[np.flatnonzero(scls == c)] = k.complete_s
         return comp
 
     sat = build(np.zeros(n_req))
     slab_gb = lambda k: round(k.n_slots * k.slab_words * 4 / 2**30, 2)  # noqa: E731
-    pages = {p.name: ({"page_blocks": k.pages.page_blocks, "pages": k.pages.n_pages,
+    pages = {k.name: ({"page_blocks": k.pages.page_blocks, "pages": k.pages.n_pages,
                        "pool_gb": round(k.pages.pool_bytes / 2**30, 2),
                        "slab_equivalent_gb": slab_gb(k)} if k.pages else {"slab_gb": slab_gb(k)})
-             for p, k in zip(mix.pools, sat)}
-    r_sat = S.run(sat, timeout_s=300.0)
+             for _, k in sat}
+    r_sat = S.run([k for _, k in sat], timeout_s=300.0)
     capacity = n_req / r_sat.wall_s
     del sat
     torch.cuda.empty_cache()
@@ -243,7 +288,7 @@ def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO
     rate = load * capacity
     arr = WL.poisson_arrivals(n_req, rate)
     paced = build(arr)
-    r = S.run(paced, timeout_s=300.0)
+    r = S.run([k for _, k in paced], timeout_s=300.0)
     comp = gather(paced)
     skip_t = arr[n_req // 10]
     keep = arr >= skip_t
@@ -252,6 +297,10 @@ def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO
     for c, p in enumerate(mix.pools):
         m = keep & (cls == c)
         per_class[p.name] = S.latency_summary(arr[m], comp[m])
+    if split_rows > 0:
+        for label, m in (("noisy_split", scls == len(mix.pools)),
+                         ("noisy_unsplit", (cls == noisy) & (scls == noisy))):
+            per_class[label] = S.latency_summary(arr[keep & m], comp[keep & m])
     del paced
     res = {
         "qps": round(rate, 1),
@@ -264,8 +313,12 @@ def serving_extra(torch, WL, dev, n_req, load, online=True, prio_rows=SERVE_PRIO
         "latency_by_class": per_class,
         "completed": r.completed,
         "timed_out": r.timed_out,
-        "slots": dict(zip([p.name for p in mix.pools], SERVE_SLOTS)),
+        "slots": dict(zip([p.name for p in mix.pools], slots)),
         "noisy_prio_rows": prio_rows,  # staged form only
+        "noisy_split": ({"rows": split_rows, "slices": split_k, "workgroups": split_groups,
+                         "noisy_waves": slots[noisy],
+                         "requests": int((scls == len(mix.pools)).sum()),
+                         "cell_frac_of_noisy": round(split_frac, 4)} if split_rows > 0 else None),
         "planes": pages,
         "workload": f"{n_req} Poisson requests, 60% short (150 bp, half_band 32) / 30% long "
                     "(lognormal 12 kb, half_band 256) / 10% noisy (lognormal 8 kb, 6% errors, "
@@ -773,7 +826,7 @@ def host_info():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--tile", default=None,
@@ -793,6 +846,10 @@ def main():
     ap.add_argument("--serve-prio-rows", type=int, default=SERVE_PRIO_ROWS,
                     help="staged serving: noisy-class requests with at least this many query "
                          "rows run at raised wave priority (0: none)")
+    ap.add_argument("--serve-split", default=",".join(str(x) for x in SERVE_SPLIT),
+                    help="R,K,G,N: staged serving routes noisy requests of at least R query "
+                         "rows to a class of G workgroups that run each as K band slices, the "
+                         "noisy class keeping N wavefronts (R = 0: none)")
     ap.add_argument("--serve-load", type=float, default=0.65,
                     help="offered load of the paced serving run, as a fraction of the "
                          "saturation capacity")
@@ -830,7 +887,9 @@ def main():
     if args.workload == "serving":
         print(json.dumps(serving_extra(torch, WL, dev, args.serve_requests, args.serve_load,
                                        online=not args.no_online,
-                                       prio_rows=args.serve_prio_rows)), flush=True)
+                                       prio_rows=args.serve_prio_rows,
+                                       split=tuple(int(x) for x in args.serve_split.split(",")))),
+              flush=True)
         return
     if args.workload == "chains":
         print(json.dumps(chains_extra(torch, K, WL, dev, threads)), flush=True)
@@ -1003,8 +1062,22 @@ def main():
         if launch["split_rows"] else dict(launch)
     one.step()
     ms1l = time_events(torch, one.step, 2)
+    # what its plain step is made of, and its time over each compiled split
+    one_parts = {}
+    for label, kw in (("scores_only", dict(traceback=False)), ("with_planes", dict(traceback=True))):
+        fn = lambda kw=kw: K.sw_forward(one.inp, *scoring(wn), wn.half_band, order=one.order,  # noqa: E731
+                                        tile=one.tile, workspace=one.ws, **kw)
+        fn()
+        one_parts[label] = round(time_events(torch, fn, 2), 3)
+    one_parts["with_fused_walk"] = round(ms1, 3)
+    one_split = {}
+    for _, sl in K.split_variants(wn.half_band, rn.tile.cells_per_lane):
+        one.launch = {"split_rows": 1, "split_slices": sl}
+        one.step()
+        one_split[f"{sl}_slices"] = round(time_events(torch, one.step, 2), 3)
     del one
-    # each length bin as a batch of its own (its autotuned tile, plain launch)
+    # each length bin as a batch of its own (its autotuned tile): plain launch,
+    # and the library's default launch for that batch
     for b, idx in zip(bins, [i for i in bin_idx if len(i)]):
         rb = BatchRun(torch, K, WL.subsample(wn, idx), dev, True,
                       tile=K.resolve_tile(wn.half_band, *b["tile"]))
@@ -1012,6 +1085,14 @@ def main():
         bms = time_events(torch, rb.step, 2)
         b["ms"] = round(bms, 3)
         b["gcups"] = round(b["cells"] / (bms / 1e3) / 1e9, 2)
+        dl = K.default_launch(rb.inp, wn.half_band, rb.tile)
+        if dl["split_rows"]:
+            rb.launch = dl
+            rb.step()
+            bms = time_events(torch, rb.step, 2)
+            b["default_launch"] = dl
+            b["default_launch_ms"] = round(bms, 3)
+            b["default_launch_gcups"] = round(b["cells"] / (bms / 1e3) / 1e9, 2)
         del rb
     del rn
     torch.cuda.empty_cache()
@@ -1040,7 +1121,10 @@ def main():
         "longest_pair_launch_ms": round(ms1l, 3),
         "longest_pair_cells": int(ncells[longest]),
         "longest_pair_frac_of_batch": round(ms1l / ms_step, 3),
-        "length_bins_note": "each bin also run as its own batch (its tile, plain launch)",
+        "longest_pair_parts_ms": one_parts,
+        "longest_pair_split_ms": one_split,
+        "length_bins_note": "each bin also run as its own batch (its tile): plain launch, and "
+                            "the library's default launch where it splits",
     }
     log(f"[bench] sw_noisy extras: {result['sw_noisy']}, per-pair {ppn:.3f}, CPU {cgn:.3f} "
         f"GCUPS on {threads} threads")
@@ -1060,6 +1144,24 @@ def main():
     gr = K.SwGraph(rs.inp, scoring(ws_), ws_.half_band, order=rs.order, tile=rs.tile,
                    workspace=rs.ws, cigar=True)
     gms = time_events(torch, gr.replay, 20)
+    # what the step is made of: the forward pass without planes, with planes,
+    # with the fused walk (medians of 20 launches each)
+    def part_ms(**kw):
+        fn = lambda: K.sw_forward(rs.inp, *scoring(ws_), ws_.half_band, order=rs.order,  # noqa: E731
+                                  tile=rs.tile, workspace=rs.ws, **kw)
+        fn()
+        s_ = torch.cuda.current_stream()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(20)]
+        for a_, b_ in ev:
+            a_.record(s_)
+            fn()
+            b_.record(s_)
+        torch.cuda.synchronize()
+        return round(float(np.median([a_.elapsed_time(b_) for a_, b_ in ev])), 4)
+
+    parts = {"scores_only": part_ms(traceback=False), "with_planes": part_ms(traceback=True),
+             "with_fused_walk": part_ms(traceback=True, cigar=True)}
     pps = per_pair_gcups(torch, K, ws_, np.arange(ws_.n_pairs), dev)
     cgs, css = cpu_gcups(ws_, np.arange(ws_.n_pairs), threads)
     result["sw_short"] = {
@@ -1072,6 +1174,7 @@ def main():
         "cpu_threads": threads,
         "tile": [rs.tile.cells_per_lane, rs.tile.lanes_per_pair],
         "autotune_ms": tile_timings(tune_s),
+        "parts_ms": parts,
         "workload": "1024 x 150 bp, half_band 32 at offset 32, seed 0 (CPU: whole batch, one C "
                     "call)",
     }
@@ -1083,7 +1186,8 @@ def main():
     if args.serve_requests > 0:
         result["serving"] = serving_extra(torch, WL, dev, args.serve_requests, args.serve_load,
                                           online=not args.no_online,
-                                          prio_rows=args.serve_prio_rows)
+                                          prio_rows=args.serve_prio_rows,
+                                          split=tuple(int(x) for x in args.serve_split.split(",")))
         torch.cuda.empty_cache()
 
     # ------------------------------------------------------------------ chains

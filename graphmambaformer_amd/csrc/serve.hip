@@ -392,6 +392,9 @@ extern "C" int gmfa_serve_open(const uint32_t *ref_seq, const uint32_t *ref_mask
                      "class %d: page_wait_s must be in [0, 3600]", c);
         GMFA_REQUIRE(!(k.params.flags & GMFA_SW_NMASK) || ref_mask,
                      "class %d: GMFA_SW_NMASK needs ref_mask", c);
+        GMFA_REQUIRE(k.params.split_rows == 0,
+                     "class %d: online sessions take no split classes (split_rows must be 0; "
+                     "gmfa_serve_run serves them)", c);
     }
     int dev = 0, khz = 0;
     GMFA_CHECK_HIP(hipGetDevice(&dev));

@@ -704,6 +704,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                  "GMFA_SW_ROWKEYS needs rowkey and rowkey_off");
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, 2 * P.half_band + 1);
     SwKernel k = serve ? t->serve(P.flags) : t->get(P.flags);
+    int block = 64;  // threads per workgroup of the launch
     const SplitEntry *split = nullptr;
     int split_groups = 0;
 
@@ -832,6 +833,25 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                      "serving needs 64 bytes of 64-byte aligned device control words");
         a.dctl = serve->dctl;
         waves = serve->n_slots;
+        if (P.split_rows > 0) {
+            // a class of split requests: every request runs as band slices
+            // on a workgroup of several wavefronts (n_slots workgroups)
+            GMFA_REQUIRE(!online, "online sessions take no split classes (split_rows must be 0)");
+            GMFA_REQUIRE(!nm, "split serving classes take no N masks");
+            const SplitEntry *se = nullptr;
+            for (const auto &e : splits()) {
+                if (!split_serves(e, 2 * P.half_band + 1, tile)) continue;
+                if (P.split_slices > 0 && e.K != P.split_slices) continue;
+                se = &e;
+                break;
+            }
+            GMFA_REQUIRE(se != nullptr,
+                         "no compiled split variant for half_band %d on the %dx%d tile with %d "
+                         "slices", P.half_band, tile.cells_per_lane, tile.lanes_per_pair,
+                         P.split_slices);
+            k = se->serve;
+            block = 64 * se->K;
+        }
         if (online) {
             GMFA_REQUIRE(serve->ring_tail && serve->ring_desc && serve->ring_q &&
                              serve->ring_res && serve->ring_cigar && serve->ring_done &&
@@ -871,7 +891,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
     // other offline launches: the kernel's last wave leaves the caller's
     // counter at zero, so it is zero here (the caller zeroes it once, before
     // its first launch)
-    hipLaunchKernelGGL(k, dim3(waves), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k, dim3(waves), dim3(block), 0, s, a);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }

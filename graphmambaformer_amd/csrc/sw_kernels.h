@@ -675,10 +675,13 @@ constexpr int kXw = 8;
 // Forward pass of one pair on slice s of K.  Returns the wavefront's best-cell key
 // (H << 40 | (2^20-1-i) << 12 | (4095-d)), 0 if no positive cell.
 // KST: compile-time register of diagonal W-1 in its lane (-1: runtime).
+// pl != nullptr: the planes are paged (serving): block b of the pair lives in
+// page pl[b >> a.page_shift] of the pool at a.tb, tb_base unused.
 template <int C, int K, int KST>
 __device__ __forceinline__ uint64_t slice_forward(const SwKArgs &a, int s, int qlen, int tlen,
                                                   int off, int64_t qo, int64_t to,
-                                                  int64_t tb_base, int *xch)
+                                                  int64_t tb_base, int *xch,
+                                                  const int32_t *pl = nullptr)
 {
     constexpr int NCW = (C + 15) / 16;  // target words feeding the cells
     constexpr int NA = NCW + 1;         // aligned window words
@@ -920,6 +923,11 @@ __device__ __forceinline__ uint64_t slice_forward(const SwKArgs &a, int s, int q
         }
         if (lane_live) {
             uint32_t *dst = a.tb + tb_base + (int64_t)b * a.blkstride + gbase;
+            if (pl) {
+                const int32_t pmask = (1 << a.page_shift) - 1;
+                dst = a.tb + ((int64_t)page_of(pl, b >> a.page_shift) << a.page_shift) * a.blkstride +
+                      (int64_t)(b & pmask) * a.blkstride + gbase;
+            }
 #pragma unroll
             for (int k = 0; k < C; ++k) dst[k] = acc[k];
         }
@@ -1018,6 +1026,123 @@ __device__ __forceinline__ uint32_t low_fields(int n)
 {
     n = min(max(n, 0), 16);
     return n >= 16 ? ~0u : ((1u << (2 * n)) - 1u);
+}
+
+// Serving with split requests (a staged class of gmfa_serve_run whose params
+// ask for a split): a persistent grid of KSL-wavefront workgroups, one
+// request at a time per workgroup, every request run as KSL band slices
+// (slice_forward), so a class of long requests has each one's rows spread
+// over KSL wavefronts.  Wavefront 0 claims the next request of the class
+// (the class's work counter), waits until the host has published it, takes
+// its traceback pages (or uses the workgroup's slab), and after the slices
+// walks it and raises its completion flag.  Every wait is bounded as in the
+// one-wavefront serving kernel.
+template <int CS, int KSL, int KSS, int CW>
+__global__ __launch_bounds__(64 * KSL) void sw_split_serve_kernel(SwKArgs a)
+{
+    __shared__ int xch[2 * (KSL + 2) * kXw];
+    __shared__ int claim[3];  // item, published requests, planes ready
+    __shared__ unsigned long long keys[KSL];
+    __shared__ int32_t page_list[kMaxPages];
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = lane_id();
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.serve_started)
+        __hip_atomic_store(a.serve_started, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const bool paged = a.page_shift >= 0;
+    for (;;) {
+        __syncthreads();  // the previous request is complete
+        if (wave == 0) {
+            int item = 0;
+            if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
+            item = __builtin_amdgcn_readfirstlane(item);
+            int n = 0, ready = 1;
+            if (item < a.n_pairs) {
+                n = serve_wait(a.dctl, a.serve_tail, a.serve_closed, a.serve_gave_up,
+                               a.deadline_ticks, a.poll_ticks, item);
+                if (item < n && paged) {
+                    const int ql = __builtin_amdgcn_readfirstlane(a.qlen[item]);
+                    const int need = (((ql + 7) >> 3) + (1 << a.page_shift) - 1) >> a.page_shift;
+                    if (!take_pages(a.page_owner, a.n_pages, need, &a.dctl->page_hint, page_list,
+                                    a.alloc_ticks)) {
+                        if (lane == 0)
+                            __hip_atomic_store(a.serve_gave_up, 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_SYSTEM);
+                        ready = 0;
+                    }
+                }
+            }
+            if (lane == 0) {
+                claim[0] = item;
+                claim[1] = n;
+                claim[2] = ready;
+            }
+        }
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(claim[0]);
+        if (item >= __builtin_amdgcn_readfirstlane(claim[1])) return;
+        const bool ready = __builtin_amdgcn_readfirstlane(claim[2]) != 0;
+        const int p = item;  // staged requests in arrival order
+        const int qlen = __builtin_amdgcn_readfirstlane(a.qlen[p]);
+        const int tlen = __builtin_amdgcn_readfirstlane(a.tlen[p]);
+        const int off = __builtin_amdgcn_readfirstlane(a.boff[p]);
+        const int64_t qo = uni64(a.q_off[p]), to = uni64(a.t_off[p]);
+        const int64_t tb_base = paged ? 0 : (int64_t)blockIdx.x * a.slab_words;
+        const int need = paged ? (((qlen + 7) >> 3) + (1 << a.page_shift) - 1) >> a.page_shift : 0;
+        uint64_t key = 0;
+        if (ready) {
+            for (int x = threadIdx.x; x < 2 * (KSL + 2) * kXw; x += 64 * KSL)
+                xch[x] = ((x % kXw) == 4) ? kNeg : 0;
+            __syncthreads();
+            key = slice_forward<CS, KSL, KSS>(a, wave, qlen, tlen, off, qo, to, tb_base, xch,
+                                              paged ? page_list : nullptr);
+        }
+        if (lane == 0) keys[wave] = key;
+        __syncthreads();  // every slice's planes and key
+        if (wave == 0) {
+            uint64_t best = 0;
+#pragma unroll
+            for (int w = 0; w < KSL; ++w) best = keys[w] > best ? keys[w] : best;
+            best = (uint64_t)uni64((int64_t)best);
+            int res_s = 0, res_i = 0, res_j = 0;
+            if (best != 0) {
+                res_s = (int)(best >> 40);
+                res_i = (int)(0xFFFFFu - (uint32_t)((best >> 12) & 0xFFFFFu));
+                res_j = res_i + off - a.hb + (int)(0xFFFu - (uint32_t)(best & 0xFFFu));
+            }
+            if (lane == 0) {
+                a.score[p] = res_s;
+                a.qend[p] = res_i;
+                a.tend[p] = res_j;
+            }
+            if (paged) {
+                const int64_t page_words = (int64_t)a.blkstride << a.page_shift;
+                const int32_t pmask = (1 << a.page_shift) - 1;
+                int cur_pg = -1;
+                const uint32_t *pbase = a.tb;
+                auto ldp = [&](int blk, int d) {
+                    const int pgi = blk >> a.page_shift;
+                    if (pgi != cur_pg) {
+                        cur_pg = pgi;
+                        pbase = a.tb + (int64_t)page_of(page_list, pgi) * page_words;
+                    }
+                    return (uint32_t)__builtin_amdgcn_readfirstlane(
+                        (int)pbase[(int64_t)(blk & pmask) * a.blkstride + d]);
+                };
+                walk_pair_paged<CW>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to, ldp);
+            } else {
+                walk_pair<CW, false>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to,
+                                     a.tb + tb_base);
+            }
+            if (!ready && lane == 0) a.wo.ncig[p] = -1;
+        }
+        __syncthreads();  // the walk's stores by every lane of wavefront 0, then the flag
+        if (wave == 0) {
+            if (lane == 0)
+                __hip_atomic_store(a.serve_done + p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            // after that release (dirty plane lines written back), free the pages
+            if (paged && ready) free_pages(a.page_owner, page_list, need);
+        }
+    }
 }
 
 // SV: serving variant (persistent queue fed by the host; staged requests of
@@ -1724,9 +1849,11 @@ using SplitKernel = void (*)(SwKArgs, uint32_t *);
 struct SplitEntry {
     int CS, K, KSS, C, KS;
     SplitKernel k;
+    SwKernel serve;  // staged serving class of split requests
 };
 
-#define GMFA_SPLIT(CS, K, KSS, C, KS) {CS, K, KSS, C, KS, &sw_split_kernel<CS, K, KSS, C>}
+#define GMFA_SPLIT(CS, K, KSS, C, KS) \
+    {CS, K, KSS, C, KS, &sw_split_kernel<CS, K, KSS, C>, &sw_split_serve_kernel<CS, K, KSS, C>}
 
 // The compiled variants, one group per translation unit (n = entries).
 const TileEntry *sw_tiles_24(int *n);      // sw_tiles_24.hip: 24 x 64

@@ -147,6 +147,7 @@ class ServeClassRun:
     page_owner: Optional[torch.Tensor] = None
     page_wait_s: float = 0.0
     prio_rows: int = 0  # requests with at least this many query rows run at raised priority
+    split_slices: int = 0  # > 0: every request runs as this many band slices (workgroup slots)
 
     @property
     def n(self) -> int:
@@ -158,7 +159,8 @@ class ServeClassRun:
         ma, mi, go, ge = self.scoring
         flags = _lib.SW_TRACEBACK | _lib.SW_CIGAR
         s.params = kernels.make_params(ma, mi, go, ge, self.half_band, flags, self.tile,
-                                       prio_rows=self.prio_rows)
+                                       split_rows=1 if self.split_slices else 0,
+                                       split_slices=self.split_slices, prio_rows=self.prio_rows)
         o = s.out
         o.score, o.query_end, o.target_end = ptr(self.score), ptr(self.qend), ptr(self.tend)
         o.tb, o.tb_off = ptr(self.tb), None
@@ -183,12 +185,16 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
                tile: Optional[SwTile] = None,
                plane_budget_bytes: Optional[int] = None,
                page_blocks: Optional[int] = None,
-               page_wait_s: float = 0.0, prio_rows: int = 0) -> ServeClassRun:
+               page_wait_s: float = 0.0, prio_rows: int = 0,
+               split_slices: int = 0) -> ServeClassRun:
     """Request table for pool entries `picks` (in arrival order).  With
     `plane_budget_bytes`, the traceback planes come from a pool of pages of
     that size and requests are admitted by their planes' footprint; without,
     every slot holds a slab for the class's longest query.  Requests with at
-    least `prio_rows` query rows (> 0) run at raised wave priority."""
+    least `prio_rows` query rows (> 0) run at raised wave priority.
+    `split_slices` > 0 (a band with a compiled split variant, no N masks):
+    every request of the class runs as that many band slices on a workgroup
+    of as many wavefronts; `n_slots` then counts workgroups."""
     dev = reference.words.device
     picks = np.asarray(picks, np.int64)
     n = len(picks)
@@ -235,7 +241,8 @@ def make_class(name: str, reference: ResidentReference, pool: RequestPool, picks
         tb=torch.empty(pages.n_pages * pages.page_words if pages else int(n_slots) * slab,
                        dtype=torch.int32, device=dev),
         counter=torch.zeros(16, dtype=torch.int32, device=dev), walk=walk, picks=picks,
-        pages=pages, page_owner=owner, page_wait_s=page_wait_s, prio_rows=int(prio_rows))
+        pages=pages, page_owner=owner, page_wait_s=page_wait_s, prio_rows=int(prio_rows),
+        split_slices=int(split_slices))
 
 
 @dataclass

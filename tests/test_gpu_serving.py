@@ -503,3 +503,37 @@ def test_paged_budget_below_one_request_is_rejected(mix):
     with pytest.raises(ValueError):
         S.make_class("x", ref_dev, pools_dev[1], np.arange(3), np.zeros(3), SC, 4,
                      plane_budget_bytes=budgets[1])
+
+
+@pytest.mark.parametrize("paged", [False, True])
+@pytest.mark.parametrize("slices", [2, 3, 4])
+def test_split_class_matches_oracle_alone_and_in_mix(mix, slices, paged):
+    """A staged class whose every request runs as band slices on a workgroup
+    (the noisy band, half_band 751): bit-identical to the oracle alone, and in
+    a mixed run beside the short and long classes and a one-wavefront noisy
+    class holding the same requests."""
+    from graphmambaformer_amd import serving as S
+
+    ref_dev, pools_dev, exp = mix
+    n = len(exp[2])
+    budget = dict(plane_budget_bytes=64 << 20, page_blocks=4) if paged else {}
+    _say("case: split serving class alone", slices, "paged", paged)
+    alone = S.make_class("noisy-split", ref_dev, pools_dev[2], np.arange(n),
+                         np.linspace(0.0, 0.005, n), SC, 3, split_slices=slices, **budget)
+    res = S.run([alone], timeout_s=30.0)
+    assert res.completed == n and not res.timed_out and not res.waves_gave_up
+    _check(alone, exp[2], f"split{slices}-alone")
+    _say("case: split serving class in a mix", slices, "paged", paged)
+    rng = np.random.default_rng(slices)
+    picks = [rng.integers(0, len(e), size=m) for e, m in zip(exp, (60, 20, 8))]
+    arrivals = [np.sort(rng.uniform(0.0, 0.02, size=len(p))) for p in picks]
+    classes = _build(ref_dev, pools_dev, picks, arrivals)
+    sp = rng.permutation(n)
+    classes.append(S.make_class("noisy-split", ref_dev, pools_dev[2], sp,
+                                np.sort(rng.uniform(0.0, 0.02, size=n)), SC, 4,
+                                split_slices=slices, **budget))
+    res = S.run(classes, timeout_s=30.0)
+    assert res.completed == sum(c.n for c in classes)
+    assert not res.timed_out and not res.waves_gave_up
+    for c, e in zip(classes, exp + [exp[2]]):
+        _check(c, e, c.name + f"/mix{slices}")

@@ -271,3 +271,26 @@ def test_narrow_and_batch_tiles_on_the_host():
         t = K.narrow_tile(hb)
         assert t is not None and t.lanes_per_pair == 64, hb
         assert t.lanes_stored * t.cells_per_lane >= 2 * hb + 1, hb
+
+
+
+def test_online_session_rejects_a_split_class():
+    """Online sessions take no split classes: gmfa_serve_open refuses one with
+    GMFA_E_INVALID while validating its arguments, before any device call."""
+    import ctypes
+
+    from graphmambaformer_amd import _lib, kernels as K
+    from graphmambaformer_amd._lib import lib
+
+    cfg = (_lib.ServeClassCfg * 1)()
+    c = cfg[0]
+    c.params = K.make_params(2, 4, 4, 2, 751, _lib.SW_TRACEBACK | _lib.SW_CIGAR,
+                             K.resolve_tile(751), split_rows=1, split_slices=3)
+    c.max_qlen, c.ring, c.n_slots, c.cigar_cap = 1000, 8, 2, 100
+    dummy = ctypes.c_void_p(0x1000)  # never dereferenced: validation fails first
+    c.tb, c.stage, c.head = dummy, dummy, dummy
+    sess = ctypes.c_void_p()
+    rc = lib().gmfa_serve_open(dummy, None, 1000, cfg, 1, 1.0, ctypes.byref(sess))
+    assert rc == _lib.GMFA_E_INVALID
+    assert b"split" in lib().gmfa_last_error()
+    assert not sess.value
