@@ -338,7 +338,7 @@ static const std::vector<TileEntry> &tiles()
             {5, 16, 4},   {9, 64, 8},   {8, 64, 8},   {24, 64, 14}, {2, 16, -1},  {5, 16, -1},
             {8, 16, -1},  {4, 32, -1},  {8, 32, -1},  {2, 64, -1},  {4, 64, -1},  {8, 64, -1},
             {9, 64, -1},  {12, 64, -1}, {16, 64, -1}, {24, 64, -1}, {32, 64, -1},
-            {1, 64, 1},
+            {1, 64, 1},   {2, kSkewLanes, -1},
         };
         std::vector<TileEntry> pool, v;
         for (auto part : {sw_tiles_24, sw_tiles_wide, sw_tiles_mid, sw_tiles_small}) {
@@ -387,17 +387,20 @@ static const TileEntry *find_tile(int C, int G, int W)
     return gen;
 }
 
+// Latency tiles, for batches too small to fill the device and never the
+// default: a one-cell lane (1 x 64 + the scalar cell for W = 65) and the
+// skewed tile (2 x 63).  A large batch of short pairs runs faster packed
+// several pairs to a wavefront.
+static bool latency_tile(const TileEntry &t) { return t.C == 1 || t.G == kSkewLanes; }
+
 // default: the tile with the least padding (live cells / lane slots), then
-// specialised before general, then more cells per lane.  A one-cell lane
-// (1 x 64 + the scalar cell for W = 65) is a latency tile for batches too
-// small to fill the device and never the default: a large batch of short
-// pairs runs faster packed several pairs to a wavefront.
+// specialised before general, then more cells per lane, latency tiles aside
 static const TileEntry *default_tile(int W)
 {
     const TileEntry *best = nullptr;
     double best_eff = -1.0;
     for (const auto &t : tiles()) {
-        if (!tile_serves(t, W) || t.C == 1) continue;
+        if (!tile_serves(t, W) || latency_tile(t)) continue;
         const double eff = (double)W / (double)tile_capacity(t) + (t.KS >= 0 ? 1e-3 : 0.0);
         if (eff > best_eff + 1e-9) { best = &t; best_eff = eff; }
     }
@@ -630,7 +633,9 @@ extern "C" int gmfa_sw_resident_waves(const gmfa_sw_params *params, int32_t *wav
     if (rc != GMFA_OK) return rc;
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair,
                                    2 * params->half_band + 1);
-    *waves = resident_waves(t->get(params->flags));
+    SwKernel k = t->get(params->flags);
+    GMFA_REQUIRE(k != nullptr, "the %dx%d tile has no form with per-row keys", t->C, t->G);
+    *waves = resident_waves(k);
     return GMFA_OK;
 }
 
@@ -704,6 +709,7 @@ int sw_launch(const gmfa_sw_batch *batch, const gmfa_sw_params *params, const gm
                  "GMFA_SW_ROWKEYS needs rowkey and rowkey_off");
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, 2 * P.half_band + 1);
     SwKernel k = serve ? t->serve(P.flags) : t->get(P.flags);
+    GMFA_REQUIRE(serve || k != nullptr, "the %dx%d tile has no form with per-row keys", t->C, t->G);
     int block = 64;  // threads per workgroup of the launch
     const SplitEntry *split = nullptr;
     int split_groups = 0;

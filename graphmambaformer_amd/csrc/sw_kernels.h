@@ -1799,6 +1799,256 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 }
 
 // -------------------------------------------------------------------------
+// Skewed tile: 2 diagonals per lane, 63 lanes per pair (bands up to 126)
+// -------------------------------------------------------------------------
+// The latency tile of batches of short-band pairs too small to fill the
+// device (one pair per wavefront, about one wavefront per SIMD).  Lane l owns
+// diagonals 2l ("even") and 2l+1 ("odd"); cell (i, d) is computed at step
+// 2(i-1) + d, so in double step s lane l computes row s - l + 1 (its even
+// cell, then its odd cell), lane l-1 is a row ahead and lane l+1 a row
+// behind.  Every input of a cell is a register of the lane or one DPP shift
+// away:
+//   even (i, 2l):    left (i, 2l-1)   lane l-1's odd cell of double step s-1
+//                    up   (i-1, 2l+1) own odd cell of s-1
+//                    diag (i-1, 2l)   own even cell of s-1
+//   odd  (i, 2l+1):  left (i, 2l)     own even cell of s
+//                    up   (i-1, 2l+2) lane l+1's even cell of s
+//                    diag (i-1, 2l+1) own odd cell of s-1
+// E and F follow the recurrences themselves (extension.py:139-176), so a
+// double step is a short dependency chain with no prefix scan across the
+// lanes: the row-sweep tiles pay a 6-step DPP scan per row, which a lone
+// wavefront on its SIMD cannot overlap with anything.  A pair of qlen rows
+// takes 8*ceil(qlen/8) + ceil(W/2) - 1 double steps.
+//
+// Cells outside the matrix (rows < 1 or > qlen, columns < 1 or > tlen) are
+// computed like the others with their bases forced to mismatch, as the edge
+// blocks of the row-sweep kernel: a cell above row 1 or left of column 1
+// then holds H = 0 (its inputs are such cells or the boundary), F <= -o and
+// E <= -o, which is what the recurrences read there; a cell below row qlen or
+// right of column tlen never exceeds a cell before it in (row, diagonal)
+// order.  Cells past the band (d >= W) are forced to H = 0, F = -inf (the
+// band guard).  Left of diagonal 0 the DPP shift brings H = 0, E = -inf.
+//
+// Planes: the layout of a 2-cell tile with ceil(W/2) lanes stored (dword d of
+// block b at b * blkstride + d, rows in nibbles from the top), the same
+// decision bits as the row-sweep kernel; a lane stores its two dwords of a
+// block in the double step that completes the block's last row.  The first
+// maximum in (row, diagonal) order: keys H << 4 | (15 - cell) over a window
+// of 8 double steps (cells 2k + 0/1 in lane order, which is (row, diagonal)
+// order), a strict improvement across windows, then the wave reduction of
+// the row-sweep kernel.
+template <bool TB, bool NM>
+__global__ __launch_bounds__(64) void sw_skew_kernel(SwKArgs a)
+{
+    const int l = (int)threadIdx.x;
+    const int W = a.W, hb = a.hb, ma = a.ma, mi = a.mi, go = a.go, ge = a.ge;
+    const int nl = (W + 1) >> 1;  // lanes holding a band diagonal
+    const bool eout = 2 * l >= W, oout = 2 * l + 1 >= W;
+    const uint32_t mms = (uint32_t)(ma + mi);
+    const int64_t t_hi = a.t_words - 1, q_hi = a.q_words - 1;
+    // rows of this lane that a window's end leaves past its last complete block
+    const int bsh = (8 - (l & 7)) & 7;
+    // per-lane forms of the band guard: H capped at 0 and F - e at -inf past
+    // the band
+    const int hcap_e = eout ? 0 : INT_MAX, hcap_o = oout ? 0 : INT_MAX;
+    const int gcap_e = eout ? (1 << 29) : ge, gcap_o = oout ? (1 << 29) : ge;
+    const int ebias = (int)(0x80000000u - (uint32_t)ge);
+    // loop constants held in VGPRs: the DPP forms of sub and xor read their
+    // second operand from a VGPR
+    int vgo = go, vzero = 0;
+    uint32_t vsign = 0x80000000u;
+    asm volatile("" : "+v"(vgo), "+v"(vsign), "+v"(vzero));
+
+    for (;;) {
+        int item = 0;
+        if (l == 0) item = (int)atomicAdd(a.counter, 1u);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= a.n_items) {
+            // the launch's last claim leaves the counter at zero
+            if (l == 0 && item == a.n_items + (int)gridDim.x - 1) *a.counter = 0u;
+            break;
+        }
+        const int p = uni<64>(a.order ? a.order[item] : item);
+        const int qlen = uni<64>(a.qlen[p]), tlen = uni<64>(a.tlen[p]), off = uni<64>(a.boff[p]);
+        const int64_t qo = uni64(a.q_off[p]), to = uni64(a.t_off[p]);
+        const bool prio = a.prio_rows > 0 && qlen >= a.prio_rows;
+        if (prio) __builtin_amdgcn_s_setprio(2);
+        const int nblk = (qlen + 7) >> 3;
+        const int nwin = (8 * nblk + nl - 1 + 7) >> 3;
+        const int64_t tb_base =
+            TB ? (a.slab_words > 0 ? (int64_t)blockIdx.x * a.slab_words : a.tb_off[p]) : 0;
+
+        // double steps s of this lane's cells inside the matrix: row s-l+1 in
+        // [1, qlen], even column s+l+1+off-hb in [1, tlen] (odd: one more)
+        const int c0 = hb - off - l, c1 = tlen - 1 + hb - off - l;
+        const int lo_e = max(l, c0), hi_e = min(qlen - 1 + l, c1);
+        const int lo_o = max(l, c0 - 1), hi_o = min(qlen - 1 + l, c1 - 1);
+
+        // 2-bit bases of the window's 8 double steps (16 loaded): query from
+        // row s0-l+1, target from the even column s0+l+1+off-hb (the odd cell
+        // reads one base further); words before a stream's start are clamped,
+        // their cells lie outside the matrix
+        int64_t eq = qo - l, et = to + l + off - hb;
+        uint32_t q0, q1, t0, t1, qm0 = 0u, qm1 = 0u, tm0 = 0u, tm1 = 0u;
+        auto ld2 = [&](const uint32_t *src, int64_t e, int64_t hi, uint32_t &w0, uint32_t &w1)
+            __attribute__((always_inline)) {
+            const int64_t wb = e >> 4;
+            w0 = src[clampi64(wb, hi)];
+            w1 = src[clampi64(wb + 1, hi)];
+        };
+        ld2(a.qseq, eq, q_hi, q0, q1);
+        ld2(a.tseq, et, t_hi, t0, t1);
+        if (NM) {
+            ld2(a.qmask, eq, q_hi, qm0, qm1);
+            ld2(a.tmask, et, t_hi, tm0, tm1);
+        }
+
+        // Registers between double steps: the lane's cells of the previous
+        // row (H for the diagonal terms, the odd cell's F - e, and its E - e
+        // with the sign bit flipped: the zero that the DPP shift brings into
+        // lane 0 reads as E = INT_MIN left of diagonal 0, as H reads 0).
+        int He = 0, Ho = 0, Fgo = kNeg, Ebx = 0;
+        uint32_t acc_e = 0u, acc_o = 0u, prev_e = 0u, prev_o = 0u;
+        int bestK = 0, bestS = 0;
+        for (int w = 0; w < nwin; ++w) {
+            const int s0 = 8 * w;
+            uint32_t ze, zo;
+            {
+                const uint32_t shq = (uint32_t)(eq & 15) * 2u, sht = (uint32_t)(et & 15) * 2u;
+                const uint32_t qw = __builtin_amdgcn_alignbit(q1, q0, shq);
+                const uint32_t tw = __builtin_amdgcn_alignbit(t1, t0, sht);
+                uint32_t ye = qw ^ tw, yo = qw ^ (tw >> 2);
+                ye |= ye >> 1;
+                yo |= yo >> 1;
+                if (NM) {
+                    const uint32_t qn = __builtin_amdgcn_alignbit(qm1, qm0, shq);
+                    const uint32_t tn = __builtin_amdgcn_alignbit(tm1, tm0, sht);
+                    ye |= qn | tn;
+                    yo |= qn | (tn >> 2);
+                }
+                // bit 2k: cell k of the window matches and lies inside the matrix
+                ze = ~ye & (low_fields(hi_e - s0 + 1) & ~low_fields(lo_e - s0)) & 0x55555555u;
+                zo = ~yo & (low_fields(hi_o - s0 + 1) & ~low_fields(lo_o - s0)) & 0x55555555u;
+            }
+            eq += 8;
+            et += 8;
+            if (w + 1 < nwin) {
+                ld2(a.qseq, eq, q_hi, q0, q1);
+                ld2(a.tseq, et, t_hi, t0, t1);
+                if (NM) {
+                    ld2(a.qmask, eq, q_hi, qm0, qm1);
+                    ld2(a.tmask, et, t_hi, tm0, tm1);
+                }
+            }
+
+            int bk = 0;  // the window's best key, H << 4 | (15 - cell)
+            unroll_ic<8>([&](auto kc) __attribute__((always_inline)) {
+                constexpr int k = decltype(kc)::value;
+                // even cell (i, 2l): left from lane l-1, up and diagonal own
+                const int ae = __builtin_amdgcn_update_dpp(0, Ho, kWaveShr1, 0xf, 0xf, true) - vgo;
+                const int be = (int)((uint32_t)__builtin_amdgcn_update_dpp(0, Ebx, kWaveShr1, 0xf,
+                                                                           0xf, true) ^ vsign);
+                const int Ee = max(ae, be);
+                const int Hgo = Ho - go;
+                const int Fe = max(Hgo, Fgo);
+                const int De = mad_u24s(__builtin_amdgcn_ubfe(ze, 2 * k, 1), mms, He - mi);
+                // H >= 0, and 0 past the band
+                const int He2 = med3i(max(max(De, Ee), Fe), vzero, hcap_e);
+                if (TB) {
+                    // "continue the gap" bits as open candidate < max: the
+                    // maxima are computed anyway, and no operand of the
+                    // difference comes through a DPP shift
+                    acc_e = insert_sign(acc_e, Hgo - Fe);   // f_cont
+                    acc_e = insert_sign(acc_e, De - He2);   // not_diag
+                    acc_e = insert_sign(acc_e, Fe - He2);   // not_f
+                    acc_e = insert_sign(acc_e, ae - Ee);    // e_cont
+                }
+                const int ke = lshl4_or<15 - 2 * k>(He2);
+                // F - e past the band: -inf
+                const int Hge = He2 - go, Ege = Ee - ge, Fge = Fe - gcap_e;
+                // odd cell (i, 2l+1): left own even, up from lane l+1's even
+                const int Eo = max(Hge, Ege);
+                const int fa = __builtin_amdgcn_update_dpp(0, Hge, kWaveShl1, 0xf, 0xf, true);
+                const int fb = __builtin_amdgcn_update_dpp(0, Fge, kWaveShl1, 0xf, 0xf, true);
+                const int Fo = max(fa, fb);
+                const int Do = mad_u24s(__builtin_amdgcn_ubfe(zo, 2 * k, 1), mms, Ho - mi);
+                const int Ho2 = med3i(max(max(Do, Eo), Fo), vzero, hcap_o);
+                if (TB) {
+                    acc_o = insert_sign(acc_o, fa - Fo);    // f_cont
+                    acc_o = insert_sign(acc_o, Do - Ho2);   // not_diag
+                    acc_o = insert_sign(acc_o, Fo - Ho2);   // not_f
+                    acc_o = insert_sign(acc_o, Hge - Eo);   // e_cont
+                }
+                bk = max(max(bk, ke), lshl4_or<14 - 2 * k>(Ho2));
+                He = He2;
+                Ho = Ho2;
+                Ebx = Eo + ebias;
+                Fgo = Fo - gcap_o;
+            });
+            if constexpr (TB) {
+                // this lane's block completed in this window: its rows end at
+                // row s0 - l + 8 - bsh (a multiple of 8), the last 16 rows are
+                // this window's and the previous window's accumulators
+                const int sblk = (s0 - l + 7 - bsh) >> 3;
+                if (l < nl && sblk >= 0 && sblk < nblk) {
+                    uint32_t *dst = a.tb + tb_base + (int64_t)sblk * a.blkstride + 2 * l;
+                    dst[0] = __builtin_amdgcn_alignbit(prev_e, acc_e, 4 * bsh);
+                    dst[1] = __builtin_amdgcn_alignbit(prev_o, acc_o, 4 * bsh);
+                }
+                prev_e = acc_e;
+                prev_o = acc_o;
+            }
+            // strict improvement in H over the lane's best so far
+            const bool up = (bk >> 4) > (bestK >> 4);
+            bestK = up ? bk : bestK;
+            bestS = up ? s0 : bestS;
+        }
+        int bestH = bestK >> 4, besti = 0, bestd = 0;
+        {
+            const int c = 15 - (bestK & 15);  // cell 2k + (0 even, 1 odd)
+            besti = bestS + (c >> 1) - l + 1;
+            bestd = 2 * l + (c & 1);
+        }
+
+        // wave reduction of the lane-first maxima: (max H, min i, min d)
+        uint64_t key = bestH > 0 ? (((uint64_t)(uint32_t)bestH << 40) |
+                                    ((uint64_t)(0xFFFFFu - (uint32_t)besti) << 12) |
+                                    (uint64_t)(0xFFFu - (uint32_t)bestd))
+                                 : 0ull;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const uint32_t lo32 = (uint32_t)__shfl_xor((int)(uint32_t)key, o);
+            const uint32_t hi32 = (uint32_t)__shfl_xor((int)(uint32_t)(key >> 32), o);
+            const uint64_t other = ((uint64_t)hi32 << 32) | lo32;
+            key = other > key ? other : key;
+        }
+        int res_s = 0, res_i = 0, res_j = 0;
+        if (key != 0) {
+            res_s = (int)(key >> 40);
+            res_i = (int)(0xFFFFFu - (uint32_t)((key >> 12) & 0xFFFFFu));
+            res_j = res_i + off - hb + (int)(0xFFFu - (uint32_t)(key & 0xFFFu));
+        }
+        res_s = uni<64>(res_s);
+        res_i = uni<64>(res_i);
+        res_j = uni<64>(res_j);
+        if (l == 0) {
+            a.score[p] = res_s;
+            a.qend[p] = res_i;
+            a.tend[p] = res_j;
+        }
+        if constexpr (TB) {
+            if (a.walk) {
+                // this wave's plane stores must be visible to its own loads
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                walk_pair<2, NM>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to, a.tb + tb_base);
+            }
+        }
+        if (prio) __builtin_amdgcn_s_setprio(0);
+    }
+}
+
+// -------------------------------------------------------------------------
 // host side: variant table
 // -------------------------------------------------------------------------
 using SwKernel = void (*)(SwKArgs);
@@ -1841,6 +2091,27 @@ struct TileEntry {
 #define GMFA_TILE(C, G, KS)                                                                   \
     {C, G, KS, &pick_kernel<C, G, KS>, &sw_traceback_kernel<C>, &pick_serve_kernel<C, G, KS>, \
      &sw_traceback_wave_kernel<C>}
+
+// The skewed tile (sw_skew_kernel): offline batches only (no row keys, no
+// serving form); its planes are those of a 2-cell tile, walked by the C = 2
+// walkers.
+template <int = 0>
+SwKernel pick_skew_kernel(uint32_t flags)
+{
+    if (flags & GMFA_SW_ROWKEYS) return nullptr;
+    const bool tb = flags & GMFA_SW_TRACEBACK, nm = flags & GMFA_SW_NMASK;
+    if (tb && nm) return sw_skew_kernel<true, true>;
+    if (tb) return sw_skew_kernel<true, false>;
+    if (nm) return sw_skew_kernel<false, true>;
+    return sw_skew_kernel<false, false>;
+}
+template <int = 0>
+SwKernel no_serve_kernel(uint32_t) { return nullptr; }
+
+constexpr int kSkewLanes = 63;  // lanes_per_pair of the skewed tile (lane 63 idle)
+#define GMFA_SKEW_TILE                                                                       \
+    {2, kSkewLanes, -1, &pick_skew_kernel<>, &sw_traceback_kernel<2>, &no_serve_kernel<>, \
+     &sw_traceback_wave_kernel<2>}
 
 // Split launches: (slice cells per lane CS, slices K, compile-time register
 // of diagonal W-1 in a slice tile) over a regular 64-lane tile (C, KS)

@@ -6,6 +6,7 @@
 #include "common.h"
 
 #include <climits>
+#include <utility>
 
 namespace gmfa {
 
@@ -68,6 +69,45 @@ __device__ __forceinline__ int mad_u24(uint32_t a, uint32_t b, int c)
     int r;
     asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
     return r;
+}
+
+// the same with a wave-uniform b read from an SGPR (no VGPR copy of it)
+__device__ __forceinline__ int mad_u24s(uint32_t a, uint32_t b, int c)
+{
+    int r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
+// (x << 4) | CC in one VALU op (CC an inline constant, 0 .. 15)
+template <int CC>
+__device__ __forceinline__ int lshl4_or(int x)
+{
+    static_assert(CC >= 0 && CC < 16, "4-bit constant");
+    int r;
+    asm("v_lshl_or_b32 %0, %1, 4, %2" : "=v"(r) : "v"(x), "i"(CC));
+    return r;
+}
+
+// the median of x, lo and hi in one VALU op (x clamped to [lo, hi] when
+// lo <= hi); asm, so the compiler does not merge lo into a max3 before it
+__device__ __forceinline__ int med3i(int x, int lo, int hi)
+{
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+    return r;
+}
+
+// f(std::integral_constant<int, K>{}) for K = 0 .. N-1, in order
+template <typename F, int... K>
+__device__ __forceinline__ void unroll_ic_seq(F &f, std::integer_sequence<int, K...>)
+{
+    (f(std::integral_constant<int, K>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void unroll_ic(F &&f)
+{
+    unroll_ic_seq(f, std::make_integer_sequence<int, N>{});
 }
 
 __device__ __forceinline__ int64_t readlane64(int64_t v, int l)

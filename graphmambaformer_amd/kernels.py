@@ -230,14 +230,31 @@ def narrow_tile(half_band: int) -> Optional[SwTile]:
     return None
 
 
-def batch_tile(half_band: int, n_pairs: int, device=None) -> SwTile:
+SKEW_LANES = 63  # lanes_per_pair of the skewed tile (two diagonals per lane)
+
+
+def latency_tile(half_band: int) -> Optional[SwTile]:
+    """The tile that finishes one pair soonest when each pair has a SIMD to
+    itself: for bands up to 126 diagonals the skewed tile (2 x 63: two
+    diagonals per lane, cells along anti-diagonals, no prefix scan per row),
+    else ``narrow_tile``."""
+    if 2 * int(half_band) + 1 <= 2 * SKEW_LANES:
+        try:
+            return resolve_tile(half_band, 2, SKEW_LANES)
+        except (RuntimeError, ValueError):
+            pass
+    return narrow_tile(half_band)
+
+
+def batch_tile(half_band: int, n_pairs: int, device=None, rowkeys: bool = False) -> SwTile:
     """Default tile for a batch of ``n_pairs``.  The library's default packs
     several short-band pairs into one wavefront (16- or 32-lane groups), the
     least padding per cell; when that leaves fewer than 4 wavefronts per SIMD,
-    a 64-lane tile (one pair per wavefront, the fewest cells per lane:
-    ``narrow_tile``) spreads the pairs over the idle SIMDs and finishes sooner
-    (1,024 pairs of 150 bp at half_band 32: 0.078 ms on 2x64 against 0.13 ms
-    on 5x16)."""
+    a latency tile (one pair per wavefront: ``latency_tile``) spreads the
+    pairs over the idle SIMDs and finishes sooner (1,024 pairs of 150 bp at
+    half_band 32: 0.078 ms on 2x64 and 0.069 ms on 1x64 + the scalar cell
+    against 0.13 ms on 5x16).  ``rowkeys``: the launch writes per-row keys,
+    which the skewed tile has no form for (a 64-lane tile then)."""
     t = resolve_tile(half_band)
     if t.lanes_per_pair == 64:
         return t
@@ -245,7 +262,7 @@ def batch_tile(half_band: int, n_pairs: int, device=None) -> SwTile:
     waves = -(-int(n_pairs) // (64 // t.lanes_per_pair))
     if waves >= 4 * 4 * cus:
         return t
-    return narrow_tile(half_band) or t
+    return (narrow_tile(half_band) if rowkeys else latency_tile(half_band)) or t
 
 
 def list_tiles() -> list[tuple[int, int]]:
@@ -407,7 +424,7 @@ def sw_forward(inp: SwInputs, ma: int, mi: int, go: int, ge: int, half_band: int
     ``max_waves > 0`` caps the persistent launch's wavefronts."""
     dev = inp.device
     n = inp.n_pairs
-    tile = tile or batch_tile(half_band, n, dev)
+    tile = tile or batch_tile(half_band, n, dev, rowkeys=rowkeys)
     flags = 0
     if traceback:
         flags |= _lib.SW_TRACEBACK

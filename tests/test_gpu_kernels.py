@@ -211,6 +211,28 @@ def test_sw_scalar_cell_tile_matches_oracle_and_9x64(fused):
     assert np.array_equal(keys[0][1], keys[1][1])
 
 
+@pytest.mark.parametrize("scoring", [(2, 4, 4, 2), (2, 4, 2, 2), (3, 2, 9, 1)])
+def test_sw_skew_tile_edges(scoring):
+    """The skewed tile (2 diagonals per lane, cells along anti-diagonals) on
+    bands from one diagonal to its 125-diagonal limit: pairs of 1 base, pairs
+    shorter than the band, offsets that put the band's edge far left or right
+    of the matrix, N codes; both walks, against the oracle."""
+    rng = np.random.default_rng(9 + sum(scoring))
+    for hb in (0, 1, 4, 31, 32, 62):
+        qs, ts, offs = [], [], []
+        for p in range(20):
+            n = 1 if p == 0 else (int(rng.integers(1, 12)) if p % 4 == 1 else
+                                  int(rng.integers(1, 400)))
+            q, t = _random_pair(rng, n, with_n=(p % 5 == 2))
+            qs.append(q)
+            ts.append(t)
+            offs.append(int(rng.integers(-hb - 20, hb + 40)))
+        for fused in (False, True):
+            _say("case: skew tile", scoring, "hb", hb, "fused", fused)
+            got = _gpu_align(qs, ts, offs, hb, scoring, tile=(2, 63), fused=fused)
+            _check_against_oracle(qs, ts, offs, hb, scoring, got, f"skew{hb}")
+
+
 def test_sw_short_config_full():
     """1,024 x 150 bp pairs, hb 32 at offset 32, scoring 2/-4/-4/-2 (whole config)."""
     from graphmambaformer_amd import workloads

@@ -263,14 +263,25 @@ def test_narrow_and_batch_tiles_on_the_host():
     assert shape(K.narrow_tile(751)) == (24, 64, 63)
     assert shape(K.narrow_tile(2)) == (2, 64, 3)
     # the default for W = 65 packs four pairs to a wavefront; a small batch
-    # (fewer than 4 wavefronts per SIMD of a 256-CU device) takes the narrow tile
+    # (fewer than 4 wavefronts per SIMD of a 256-CU device) takes the latency
+    # tile: the skewed 2 x 63 tile up to 126 diagonals, else the narrow tile
     assert shape(K.resolve_tile(32))[:2] == (5, 16)
-    assert shape(K.batch_tile(32, 1024)) == (1, 64, 65)
+    assert shape(K.batch_tile(32, 1024)) == (2, 63, 33)
     assert shape(K.batch_tile(32, 64 * 4 * 4 * 256))[:2] == (5, 16)
+    assert shape(K.latency_tile(62)) == (2, 63, 63)    # W = 125
+    assert shape(K.latency_tile(63)) == (2, 64, 64)    # W = 127: past the skewed tile
+    assert shape(K.latency_tile(256)) == (8, 64, 65)
     for hb in (2, 7, 31, 32, 33, 64, 100, 255, 256, 300, 751, 1023):
         t = K.narrow_tile(hb)
         assert t is not None and t.lanes_per_pair == 64, hb
         assert t.lanes_stored * t.cells_per_lane >= 2 * hb + 1, hb
+    # the skewed tile is never a default, holds up to 126 diagonals and has no
+    # form with per-row keys
+    assert (2, 63) in K.list_tiles()
+    for hb in range(0, 1024, 7):
+        assert K.resolve_tile(hb).lanes_per_pair != 63, hb
+    with pytest.raises((RuntimeError, ValueError)):
+        K.resolve_tile(63, 2, 63)
 
 
 
