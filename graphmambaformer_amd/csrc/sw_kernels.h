@@ -1193,6 +1193,7 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
         if (blockIdx.x == 0 && threadIdx.x == 0 && a.serve_started)
             __hip_atomic_store(a.serve_started, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+    bool first = true;
     for (;;) {
         int item = 0;
         int n_items = a.n_items;
@@ -1254,8 +1255,21 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         } else {
-            if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
-            item = __builtin_amdgcn_readfirstlane(item);
+            // offline launches without a split: a wave's first item is its
+            // workgroup index, later ones are claimed after the grid's; a
+            // grid that covers every item claims nothing (a claim round trip
+            // per wave, all at once at the start, costs more than a short
+            // pair's forward pass)
+            const bool stat = !SV && a.split_rows == 0;
+            if (stat && first) {
+                item = (int)blockIdx.x;
+            } else if (stat && (int)gridDim.x >= n_items) {
+                item = n_items + (int)gridDim.x;  // nothing left; the counter stays untouched
+            } else {
+                if (lane == 0) item = (int)atomicAdd(a.counter, 1u);
+                item = __builtin_amdgcn_readfirstlane(item) + (stat ? (int)gridDim.x : 0);
+            }
+            first = false;
             // serving: one pair per wavefront (G == 64), bounded by what the host
             // has published
             if constexpr (SV) {
@@ -1265,9 +1279,10 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
             }
             if (item >= n_items) {
                 // every wave ends on one failing claim, so the claim that
-                // returns n_items + grid - 1 is the launch's last: that wave
-                // leaves the counter at zero for the next launch (no reset
-                // kernel per launch; split launches set it up anew)
+                // returns n_items + grid - 1 (after the grid's first items:
+                // n_items - 1) is the launch's last: that wave leaves the
+                // counter at zero for the next launch (no reset kernel per
+                // launch; split launches set it up anew)
                 if constexpr (!SV) {
                     if (lane == 0 && item == n_items + (int)gridDim.x - 1) *a.counter = 0u;
                 }
@@ -1859,10 +1874,18 @@ __global__ __launch_bounds__(64) void sw_skew_kernel(SwKArgs a)
     uint32_t vsign = 0x80000000u;
     asm volatile("" : "+v"(vgo), "+v"(vsign), "+v"(vzero));
 
-    for (;;) {
+    for (bool first = true;; first = false) {
+        // the first item is the workgroup index, later ones are claimed after
+        // the grid's (a grid that covers every item claims nothing)
         int item = 0;
-        if (l == 0) item = (int)atomicAdd(a.counter, 1u);
-        item = __builtin_amdgcn_readfirstlane(item);
+        if (first) {
+            item = (int)blockIdx.x;
+        } else if ((int)gridDim.x >= a.n_items) {
+            break;
+        } else {
+            if (l == 0) item = (int)atomicAdd(a.counter, 1u);
+            item = __builtin_amdgcn_readfirstlane(item) + (int)gridDim.x;
+        }
         if (item >= a.n_items) {
             // the launch's last claim leaves the counter at zero
             if (l == 0 && item == a.n_items + (int)gridDim.x - 1) *a.counter = 0u;
