@@ -1900,6 +1900,27 @@ __global__ __launch_bounds__(64) void sw_skew_kernel(SwKArgs a)
         const int nwin = (8 * nblk + nl - 1 + 7) >> 3;
         const int64_t tb_base =
             TB ? (a.slab_words > 0 ? (int64_t)blockIdx.x * a.slab_words : a.tb_off[p]) : 0;
+        // sequence windows for the fused walk, loaded now (each 64 words
+        // ending at the pair's last word, when that covers the pair): the walk
+        // then starts without two dependent loads
+        const bool walk = TB && a.walk;
+        SeqWin qsw, tsw;
+        if (walk) {
+            const int qtop = (int)(((qo & 15) + qlen - 1) >> 4) + 1;
+            const int ttop = (int)(((to & 15) + tlen - 1) >> 4) + 1;
+            const int64_t qw0 = qo >> 4, tw0 = to >> 4;
+            // lane x holds word top - x of the pair's stream
+            if (qlen > 0 && qtop <= 63) {
+                const int x = qtop - l;
+                qsw.top = qtop;
+                qsw.v = (x >= 0 && qw0 + x < a.q_words) ? a.qseq[qw0 + x] : 0u;
+            }
+            if (tlen > 0 && ttop <= 63) {
+                const int x = ttop - l;
+                tsw.top = ttop;
+                tsw.v = (x >= 0 && tw0 + x < a.t_words) ? a.tseq[tw0 + x] : 0u;
+            }
+        }
 
         // double steps s of this lane's cells inside the matrix: row s-l+1 in
         // [1, qlen], even column s+l+1+off-hb in [1, tlen] (odd: one more)
@@ -2060,11 +2081,15 @@ __global__ __launch_bounds__(64) void sw_skew_kernel(SwKArgs a)
             a.tend[p] = res_j;
         }
         if constexpr (TB) {
-            if (a.walk) {
+            if (walk) {
                 // this wave's plane stores must be visible to its own loads
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                walk_pair<2, NM>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to, a.tb + tb_base);
+                const uint32_t *tbp = a.tb + tb_base;
+                const int bs = a.blkstride;
+                auto pld = [&](int top, int d0) { return plane_win_load(tbp, top, d0, bs, W); };
+                walk_pair_with<2, NM>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to, pld, qsw,
+                                      tsw);
             }
         }
         if (prio) __builtin_amdgcn_s_setprio(0);

@@ -233,6 +233,34 @@ def test_sw_skew_tile_edges(scoring):
             _check_against_oracle(qs, ts, offs, hb, scoring, got, f"skew{hb}")
 
 
+def test_sw_skew_fused_walk_paths():
+    """The skewed tile's fused walk on both sides of its limits: sequence
+    windows loaded with the forward pass (pairs up to ~62 words) or by the
+    walk, CIGARs of up to 64 runs kept in a register or spilled to the output
+    and reversed there, walk steps of up to 64 cells; against the oracle."""
+    rng = np.random.default_rng(31)
+    scoring = (2, 4, 4, 2)
+    for hb in (16, 32, 62):
+        qs, ts, offs = [], [], []
+        for n in (150, 191, 192, 193, 367, 368, 369, 990, 1300):
+            q = rng.integers(1, 5, size=n).astype(np.int16)
+            t = q.copy()
+            # a substitution every ~4 bases: well over 64 runs
+            sub = rng.choice(n, size=max(1, n // 4), replace=False)
+            t[sub] = (t[sub] % 4) + 1
+            qs.append(q)
+            ts.append(t)
+            offs.append(hb)
+        for n in (150, 368, 369, 1300):
+            q, t = _random_pair(rng, n)
+            qs.append(q)
+            ts.append(t)
+            offs.append(int(rng.integers(0, hb + 1)))
+        _say("case: skew fused walk paths hb", hb)
+        got = _gpu_align(qs, ts, offs, hb, scoring, tile=(2, 63), fused=True)
+        _check_against_oracle(qs, ts, offs, hb, scoring, got, f"skewpaths{hb}")
+
+
 def test_sw_short_config_full():
     """1,024 x 150 bp pairs, hb 32 at offset 32, scoring 2/-4/-4/-2 (whole config)."""
     from graphmambaformer_amd import workloads
