@@ -280,125 +280,19 @@ __device__ __forceinline__ uint32_t page_of(const int32_t *pl, int page)
     return (uint32_t)__builtin_amdgcn_readfirstlane(pl[page]);
 }
 
-// The same walk reading the planes through a plane-load functor ldp(block,
-// diagonal): serving with paged planes.
-template <int C, typename Ld, typename Ldp, typename Put>
-__device__ __forceinline__ void walk_steps_paged(const WalkCtx &c, int &i, int &j, int h, int off,
-                                           int64_t qo, int64_t to, Ldp ldp, Ld ld,
-                                           Put put, int &nr, int (&cnt)[4])
+// Plane window of the wavefront walk (lane l: block top - (l >> 4),
+// diagonal d0 + (l & 15); 0 outside the planes) from paged planes: the page
+// list `pl` in LDS (each lane reads its block's page; the walk runs with
+// every lane active), pages of 2^shift blocks of blkstride dwords.
+__device__ __forceinline__ uint32_t plane_win_load_paged(const uint32_t *tb, const int32_t *pl,
+                                                         int shift, int64_t blkstride, int top,
+                                                         int d0, int W)
 {
-    const bool econt_ok = c.go != c.ge;
-    const int blkstride = c.Gs * C;  // dwords between vertically adjacent blocks
-    int state = 0;  // 0 H, 1 F (I), 2 E (D)
-    int run_op = -1, run_len = 0;
-    int64_t cur = -1, pf_addr = -1;
-    uint32_t v = 0u, pf = 0u;
-    SeqWindow qwin, twin, qmwin, tmwin;
-    auto emit = [&](int op, int n) {
-        if (op == run_op) {
-            run_len += n;
-        } else {
-            if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
-            run_op = op;
-            run_len = n;
-        }
-    };
-    while (i > 0 && j > 0) {
-        const int d = j - i - off + c.hb;
-        if (d < 0 || d >= c.W) break;  // unreachable for planes this library wrote
-        // plane word of block (i-1)/8 = blk * blkstride + d
-        const int64_t addr = plane_word<C>(i, d, c.Gs);
-        if (addr != cur) {
-            const int blk = (i - 1) >> 3;
-            v = (addr == pf_addr) ? pf : ldp(blk, d);
-            cur = addr;
-            pf_addr = (i > 8) ? addr - blkstride : -1;
-            if (pf_addr >= 0) pf = ldp(blk - 1, d);
-        }
-        const int r0 = (i - 1) & 7;
-        const uint32_t nib = (v >> (28 - 4 * r0)) & 15u;
-        if (state == 0) {
-            if (h <= 0) break;
-            if (!(nib & 4u)) {
-                // rows r0, r0-1, ... of this dword while not_diag stays 0
-                const uint32_t y = (v & 0x44444444u) & (~0u << (30 - 4 * r0));
-                int m = y ? r0 - ((30 - (int)__builtin_ctz(y)) >> 2) : r0 + 1;
-                m = min(m, min(i, j));
-                if (h <= (m - 1) * c.ma) m = 1;
-                const uint32_t lenmask = (m >= 16) ? ~0u : ((1u << (2 * m)) - 1u);
-                const int64_t eq = qo + i - m, et = to + j - m;
-                uint32_t x = seq_bits(c.qseq, c.q_words, eq, qwin, ld) ^
-                             seq_bits(c.tseq, c.t_words, et, twin, ld);
-                x = (x | (x >> 1)) & 0x55555555u;
-                if (c.nm)
-                    x |= (seq_bits(c.qmask, c.q_words, eq, qmwin, ld) |
-                          seq_bits(c.tmask, c.t_words, et, tmwin, ld)) & 0x55555555u;
-                x &= lenmask;  // bit 2t: base i-m+t mismatches
-                const int nmis = __builtin_popcount(x);
-                h -= c.ma * (m - nmis) - c.mi * nmis;
-                cnt[0] += m - nmis;
-                cnt[1] += nmis;
-                if (x == 0u) {
-                    emit(GMFA_OP_EQ, m);
-                } else {
-                    for (int t = m - 1; t >= 0; --t)
-                        emit(((x >> (2 * t)) & 1u) ? GMFA_OP_X : GMFA_OP_EQ, 1);
-                }
-                i -= m;
-                j -= m;
-            } else {
-                state = (nib & 2u) ? 2 : 1;
-            }
-        } else if (state == 1) {
-            emit(GMFA_OP_I, 1);
-            cnt[2]++;
-            --i;
-            if (nib & 8u) h += c.ge;
-            else { h += c.go; state = 0; }
-        } else {
-            emit(GMFA_OP_D, 1);
-            cnt[3]++;
-            --j;
-            if (econt_ok && (nib & 1u)) h += c.ge;
-            else { h += c.go; state = 0; }
-        }
-    }
-    if (run_len) put(nr++, ((uint32_t)run_len << 4) | (uint32_t)run_op);
-}
-
-// walk_pair for paged planes.
-template <int C, typename Ldp>
-__device__ __forceinline__ void walk_pair_paged(const WalkCtx &c, const WalkOut &o, int p, int i, int j,
-                                          int h, int off, int64_t qo, int64_t to, Ldp ldp)
-{
-    const int lane = (int)(threadIdx.x & 63);
-    uint32_t *out = o.cigar + o.cig_off[p];
-    int nr = 0;
-    int cnt[4] = {0, 0, 0, 0};
-    auto ld = [&](const uint32_t *src, int64_t idx) {
-        return (uint32_t)__builtin_amdgcn_readfirstlane((int)src[idx]);
-    };
-    auto put = [&](int x, uint32_t vrun) {
-        if (lane == 0) out[x] = vrun;
-    };
-    walk_steps_paged<C>(c, i, j, h, off, qo, to, ldp, ld, put, nr, cnt);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int x = lane; x < nr / 2; x += 64) {
-        const int y = nr - 1 - x;
-        const uint32_t va = out[x], vb = out[y];
-        out[x] = vb;
-        out[y] = va;
-    }
-    if (lane == 0) {
-        o.ncig[p] = nr;
-        o.rstart[p] = i;
-        o.tstart[p] = j;
-        o.counts[4 * p + 0] = cnt[0];
-        o.counts[4 * p + 1] = cnt[1];
-        o.counts[4 * p + 2] = cnt[2];
-        o.counts[4 * p + 3] = cnt[3];
-    }
+    const int l = (int)(threadIdx.x & 63);
+    const int b = top - (l >> 4), d = d0 + (l & 15);
+    if (b < 0 || d < 0 || d >= W) return 0u;
+    const int64_t pg = (int64_t)(uint32_t)pl[b >> shift];
+    return tb[((pg << shift) + (b & ((1 << shift) - 1))) * blkstride + d];
 }
 
 // A store into host-coherent memory (the online rings): system scope, so it
@@ -420,10 +314,10 @@ __device__ __forceinline__ void host_store(void *p, uint32_t v)
 // in lane x & 63) and stored by the whole wavefront at once.  Runs past cap
 // are counted but not stored.  The outputs record follows as one store per
 // field lane.
-template <int C, bool NM>
+template <int C, bool NM, typename PLd>
 __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, int cap,
                                                ServeRes *res, int i, int j, int h, int off,
-                                               int64_t qo, int64_t to, const uint32_t *tb,
+                                               int64_t qo, int64_t to, PLd pld,
                                                uint64_t claim_tick)
 {
     int lane = (int)(threadIdx.x & 63);
@@ -431,9 +325,6 @@ __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, 
     int nr = 0;
     int cnt[4] = {0, 0, 0, 0};
     uint32_t buf = 0u;
-    auto ld = [&](const uint32_t *src, int64_t idx) {
-        return (uint32_t)__builtin_amdgcn_readfirstlane((int)src[idx]);
-    };
     auto put = [&](int x, uint32_t vrun) {
         if (lane == (x & 63)) buf = vrun;
         if ((x & 63) == 63) {
@@ -442,44 +333,7 @@ __device__ __forceinline__ void walk_pair_ring(const WalkCtx &c, uint32_t *cig, 
         }
     };
     const int score = h, qend = i, tend = j;
-    walk_steps<C, true, NM>(c, i, j, h, off, qo, to, tb, ld, put, nr, cnt);
-    if (lane < (nr & 63)) {
-        const int y = cap - 1 - ((nr & ~63) + lane);
-        if (y >= 0) host_store(cig + y, buf);
-    }
-    const uint64_t done_tick = (uint64_t)wall_clock64();
-    const int v = lane == 0 ? score : lane == 1 ? qend : lane == 2 ? tend : lane == 3 ? i
-                : lane == 4 ? j : lane == 5 ? nr : lane == 6 ? cnt[0] : lane == 7 ? cnt[1]
-                : lane == 8 ? cnt[2] : lane == 9 ? cnt[3]
-                : lane == 10 ? (int)(uint32_t)claim_tick : lane == 11 ? (int)(claim_tick >> 32)
-                : lane == 12 ? (int)(uint32_t)done_tick : (int)(done_tick >> 32);
-    if (lane < 14) host_store(reinterpret_cast<uint32_t *>(res) + lane, (uint32_t)v);
-}
-
-// walk_pair_ring for paged planes.
-template <int C, typename Ldp>
-__device__ __forceinline__ void walk_pair_ring_paged(const WalkCtx &c, uint32_t *cig, int cap,
-                                               ServeRes *res, int i, int j, int h, int off,
-                                               int64_t qo, int64_t to, Ldp ldp,
-                                               uint64_t claim_tick)
-{
-    int lane = (int)(threadIdx.x & 63);
-    asm volatile("" : "+v"(lane));  // keep lane-derived addresses inside the request loop
-    int nr = 0;
-    int cnt[4] = {0, 0, 0, 0};
-    uint32_t buf = 0u;
-    auto ld = [&](const uint32_t *src, int64_t idx) {
-        return (uint32_t)__builtin_amdgcn_readfirstlane((int)src[idx]);
-    };
-    auto put = [&](int x, uint32_t vrun) {
-        if (lane == (x & 63)) buf = vrun;
-        if ((x & 63) == 63) {
-            const int y = cap - 1 - (x - 63 + lane);
-            if (y >= 0) host_store(cig + y, buf);
-        }
-    };
-    const int score = h, qend = i, tend = j;
-    walk_steps_paged<C>(c, i, j, h, off, qo, to, ldp, ld, put, nr, cnt);
+    walk_steps_vec<C, NM>(c, i, j, h, off, qo, to, put, nr, cnt, pld, SeqWin{}, SeqWin{});
     if (lane < (nr & 63)) {
         const int y = cap - 1 - ((nr & ~63) + lane);
         if (y >= 0) host_store(cig + y, buf);
@@ -1115,20 +969,13 @@ __global__ __launch_bounds__(64 * KSL) void sw_split_serve_kernel(SwKArgs a)
                 a.tend[p] = res_j;
             }
             if (paged) {
-                const int64_t page_words = (int64_t)a.blkstride << a.page_shift;
-                const int32_t pmask = (1 << a.page_shift) - 1;
-                int cur_pg = -1;
-                const uint32_t *pbase = a.tb;
-                auto ldp = [&](int blk, int d) {
-                    const int pgi = blk >> a.page_shift;
-                    if (pgi != cur_pg) {
-                        cur_pg = pgi;
-                        pbase = a.tb + (int64_t)page_of(page_list, pgi) * page_words;
-                    }
-                    return (uint32_t)__builtin_amdgcn_readfirstlane(
-                        (int)pbase[(int64_t)(blk & pmask) * a.blkstride + d]);
+                const int sh = a.page_shift, Wb = a.W;
+                const int64_t bs = a.blkstride;
+                auto pld = [&](int top, int d0) {
+                    return plane_win_load_paged(a.tb, page_list, sh, bs, top, d0, Wb);
                 };
-                walk_pair_paged<CW>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to, ldp);
+                walk_pair_with<CW, false>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to, pld,
+                                          SeqWin{}, SeqWin{});
             } else {
                 walk_pair<CW, false>(a.wc, a.wo, p, res_i, res_j, res_s, off, qo, to,
                                      a.tb + tb_base);
@@ -1713,27 +1560,21 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 if (paged) {
-                    const int64_t blkstride = (int64_t)(lstar + 1) * C;
-                    // the walk moves up a block at a time: look the page up
-                    // only when it changes
-                    int cur_pg = -1;
-                    const uint32_t *pbase = a.tb;
-                    auto ldp = [&](int blk, int d) {
-                        const int pgi = blk >> a.page_shift;
-                        if (pgi != cur_pg) {
-                            cur_pg = pgi;
-                            pbase = a.tb + (int64_t)page_of(pl, pgi) * page_words;
-                        }
-                        return (uint32_t)__builtin_amdgcn_readfirstlane(
-                            (int)pbase[(int64_t)(blk & page_mask) * blkstride + d]);
+                    const int64_t bs = (int64_t)(lstar + 1) * C;
+                    const int sh = a.page_shift;
+                    auto pld = [&](int top, int d0) {
+                        return plane_win_load_paged(a.tb, pl, sh, bs, top, d0, W);
                     };
-                    walk_pair_ring_paged<C>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap,
-                                            a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s,
-                                            off, qo, to, ldp, claim_tick);
-                } else {
                     walk_pair_ring<C, NM>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap,
-                                      a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s, off,
-                                      qo, to, a.tb + tb_base, claim_tick);
+                                          a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s,
+                                          off, qo, to, pld, claim_tick);
+                } else {
+                    const uint32_t *tbp = a.tb + tb_base;
+                    const int bs = a.wc.Gs * C;
+                    auto pld = [&](int top, int d0) { return plane_win_load(tbp, top, d0, bs, W); };
+                    walk_pair_ring<C, NM>(a.wc, a.ring_cigar + (int64_t)ring_r * a.cigar_cap,
+                                          a.cigar_cap, a.ring_res + ring_r, res_i, res_j, res_s,
+                                          off, qo, to, pld, claim_tick);
                 }
                 if (nopages && lane == 0) host_store(&a.ring_res[ring_r].n_cigar, ~0u);
                 // every output of the request (stored by the wave's lanes) before
@@ -1774,22 +1615,15 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
                     const int64_t qog = readlane64(qo, L0), tog = readlane64(to, L0);
                     const int64_t tbg = readlane64(tb_base, L0);
                     if (paged) {
-                        const int64_t blkstride = (int64_t)wc.Gs * C;
-                        int cur_pg = -1;
-                        const uint32_t *pbase = a.tb;
-                        auto ldp = [&](int blk, int d) {
-                            const int pgi = blk >> a.page_shift;
-                            if (pgi != cur_pg) {
-                                cur_pg = pgi;
-                                pbase = a.tb + (int64_t)page_of(pl, pgi) * page_words;
-                            }
-                            return (uint32_t)__builtin_amdgcn_readfirstlane(
-                                (int)pbase[(int64_t)(blk & page_mask) * blkstride + d]);
+                        const int64_t bs = (int64_t)wc.Gs * C;
+                        const int sh = a.page_shift;
+                        auto pld = [&](int top, int d0) {
+                            return plane_win_load_paged(a.tb, pl, sh, bs, top, d0, W);
                         };
-                        walk_pair_paged<C>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
-                                           __builtin_amdgcn_readlane(res_j, L0),
-                                           __builtin_amdgcn_readlane(res_s, L0), offg, qog, tog,
-                                           ldp);
+                        walk_pair_with<C, NM>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
+                                              __builtin_amdgcn_readlane(res_j, L0),
+                                              __builtin_amdgcn_readlane(res_s, L0), offg, qog,
+                                              tog, pld, SeqWin{}, SeqWin{});
                     } else {
                         walk_pair<C, NM>(wc, wo, pg, __builtin_amdgcn_readlane(res_i, L0),
                                      __builtin_amdgcn_readlane(res_j, L0),
