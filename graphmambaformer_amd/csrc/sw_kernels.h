@@ -1139,7 +1139,11 @@ __global__ __launch_bounds__(64) void sw_forward_kernel(SwKArgs a)
 
         const int pidx = item * (64 / G) + grp;
         const bool has = online || pidx < (SV ? n_items : a.n_pairs);
-        const int p = (has && !online) ? (a.order ? a.order[pidx] : pidx) : 0;
+        // a one-pair-per-wavefront grid covering the batch needs no
+        // processing order (one dependent load less before the pair's fields)
+        const bool use_order = a.order && (G < 64 || SV || a.split_rows > 0 ||
+                                           (int)gridDim.x < n_items);
+        const int p = (has && !online) ? (use_order ? a.order[pidx] : pidx) : 0;
         const int qlen = online ? r_qlen : uni<G>(has ? a.qlen[p] : 0);
         const int tlen = online ? r_tlen : uni<G>(has ? a.tlen[p] : 0);
         const int off = online ? r_off : uni<G>(has ? a.boff[p] : 0);
@@ -1725,7 +1729,10 @@ __global__ __launch_bounds__(64) void sw_skew_kernel(SwKArgs a)
             if (l == 0 && item == a.n_items + (int)gridDim.x - 1) *a.counter = 0u;
             break;
         }
-        const int p = uni<64>(a.order ? a.order[item] : item);
+        // a grid with a wavefront for every pair needs no processing order
+        // (one dependent load less before the pair's fields)
+        const int p =
+            uni<64>((a.order && (int)gridDim.x < a.n_items) ? a.order[item] : item);
         const int qlen = uni<64>(a.qlen[p]), tlen = uni<64>(a.tlen[p]), off = uni<64>(a.boff[p]);
         const int64_t qo = uni64(a.q_off[p]), to = uni64(a.t_off[p]);
         const bool prio = a.prio_rows > 0 && qlen >= a.prio_rows;
@@ -1888,27 +1895,21 @@ __global__ __launch_bounds__(64) void sw_skew_kernel(SwKArgs a)
             bestd = 2 * l + (c & 1);
         }
 
-        // wave reduction of the lane-first maxima: (max H, min i, min d)
-        uint64_t key = bestH > 0 ? (((uint64_t)(uint32_t)bestH << 40) |
-                                    ((uint64_t)(0xFFFFFu - (uint32_t)besti) << 12) |
-                                    (uint64_t)(0xFFFu - (uint32_t)bestd))
-                                 : 0ull;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            const uint32_t lo32 = (uint32_t)__shfl_xor((int)(uint32_t)key, o);
-            const uint32_t hi32 = (uint32_t)__shfl_xor((int)(uint32_t)(key >> 32), o);
-            const uint64_t other = ((uint64_t)hi32 << 32) | lo32;
-            key = other > key ? other : key;
-        }
+        // wave reduction of the lane-first maxima: (max H, min i, min d), as
+        // two DPP max-scans: the largest H, then, among the lanes holding it,
+        // the smallest (i << 7 | d) (d < 127 here)
+        const int bh = max(bestH, 0);
+        const int hmax = __builtin_amdgcn_readlane(seg_scan_max<64>(bh), 63);
         int res_s = 0, res_i = 0, res_j = 0;
-        if (key != 0) {
-            res_s = (int)(key >> 40);
-            res_i = (int)(0xFFFFFu - (uint32_t)((key >> 12) & 0xFFFFFu));
-            res_j = res_i + off - hb + (int)(0xFFFu - (uint32_t)(key & 0xFFFu));
+        if (hmax > 0) {
+            const uint32_t pos = ((uint32_t)besti << 7) | (uint32_t)bestd;
+            const int ck = bh == hmax ? (int)(0x7FFFFFFFu - pos) : INT_MIN;
+            const uint32_t pmin =
+                0x7FFFFFFFu - (uint32_t)__builtin_amdgcn_readlane(seg_scan_max<64>(ck), 63);
+            res_s = hmax;
+            res_i = (int)(pmin >> 7);
+            res_j = res_i + off - hb + (int)(pmin & 127u);
         }
-        res_s = uni<64>(res_s);
-        res_i = uni<64>(res_i);
-        res_j = uni<64>(res_j);
         if (l == 0) {
             a.score[p] = res_s;
             a.qend[p] = res_i;
