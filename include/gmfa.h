@@ -131,7 +131,10 @@ typedef struct gmfa_sw_batch {
     const int32_t *qlen;   /* [n_pairs] */
     const int32_t *tlen;   /* [n_pairs] */
     const int32_t *band_offset; /* [n_pairs] diagonal of the band centre */
-    const int32_t *order;  /* [n_pairs] processing order (longest first) or NULL */
+    const int32_t *order;  /* [n_pairs] processing order (longest first: a
+                            * permutation of 0..n_pairs-1) or NULL; a launch
+                            * that gives every pair a wavefront of its own
+                            * does not read it */
 } gmfa_sw_batch;
 
 typedef struct gmfa_sw_out {
