@@ -70,6 +70,13 @@ __device__ __forceinline__ double wave_max_to63(double x)
     return x;
 }
 
+__device__ __forceinline__ int64_t readlane_i64(int64_t x, int l)
+{
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)x, l);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), l);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int l)
 {
     const int64_t x = __double_as_longlong(v);
@@ -92,9 +99,34 @@ __global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
     int64_t wre = 0, wfe = 0;
     double wfd = 0.0;
     int wfi = 0;
+    // anchors 64 at a time (lane k: anchor i0 + k), the next block loaded a
+    // block ahead: a step reads its anchor with readlane instead of waiting
+    // on a load
+    int64_t bre = 0, bfe = 0, nre = 0, nfe = 0;
+    double bwd = 0.0, nwd = 0.0;
+    int bwi = 0, nwi = 0;
+    auto load_blk = [&](int i0, int64_t &re, int64_t &fe, double &wd, int &wi) {
+        const int k = i0 + lane;
+        if (k < n) {
+            re = a.read_end[base + k];
+            fe = a.ref_end[base + k];
+            if constexpr (F64) wd = a.wd[base + k];
+            else wi = a.wi[base + k];
+        }
+    };
+    load_blk(0, bre, bfe, bwd, bwi);
+    load_blk(64, nre, nfe, nwd, nwi);
     for (int i = 0; i < n; ++i) {
-        const int64_t re_i = a.read_end[base + i];
-        const int64_t fe_i = a.ref_end[base + i];
+        const int t = i & 63;
+        if (t == 0 && i > 0) {
+            bre = nre;
+            bfe = nfe;
+            bwd = nwd;
+            bwi = nwi;
+            load_blk(i + 64, nre, nfe, nwd, nwi);
+        }
+        const int64_t re_i = readlane_i64(bre, t);
+        const int64_t fe_i = readlane_i64(bfe, t);
         const int lo = i - L > 0 ? i - L : 0;
         const int nvalid = i - lo;  // predecessors lo..i-1 sit in lanes 64-nvalid..63
         const int j = i - 64 + lane;
@@ -107,7 +139,7 @@ __global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
         const uint64_t okmask = __builtin_amdgcn_ballot_w64(ok);
         int par = -1;
         if constexpr (F64) {
-            const double w_i = a.wd[base + i];
+            const double w_i = readlane_f64(bwd, t);
             double s = -__builtin_inf();
             if (ok) {
                 const double adv = (double)mn < w_i ? (double)mn : w_i;
@@ -125,7 +157,7 @@ __global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
             if (lane == 0) { a.fd[base + i] = fi; a.parent[base + i] = par; }
             wfd = shl_f64(fi, wfd);
         } else {
-            const int w_i = a.wi[base + i];
+            const int w_i = __builtin_amdgcn_readlane(bwi, t);
             int key = INT_MIN;
             if (ok) {
                 const int adv = (int)(mn < (int64_t)w_i ? mn : (int64_t)w_i);
