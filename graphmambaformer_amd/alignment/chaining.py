@@ -19,8 +19,8 @@ sarakh1999/GraphMambaFormer:
       reads and strands of a batch in one launch), host backtrack (libgmfa)
       and primary/secondary selection.
 
-Coordinates must be integer valued; ``max_lookback`` <= 64 (one wavefront
-holds the window).  A context carrying a pangenome distance oracle or learned
+Coordinates must be integer valued; ``max_lookback`` <= 256 (up to 4 windows
+of 64 predecessors, one per lane of a wavefront).  A context carrying a pangenome distance oracle or learned
 transitions raises NotImplementedError (its bonus can still be passed to
 chain_dp_numpy directly).
 """
@@ -62,17 +62,18 @@ def penalty_table(cfg, device) -> torch.Tensor:
     return t
 
 
-MAX_LOOKBACK = 64  # one predecessor per lane of a wavefront
+MAX_LOOKBACK = 256  # up to 4 windows of 64 predecessors (one per lane)
 
 
 def _lookback(cfg) -> int:
     """The reference takes any max_lookback (chaining.py:154; its default is
-    64, config.py:681); the chaining kernel holds up to 64.  Larger windows
-    are rejected, before any device work."""
+    64, config.py:681); the chaining kernels hold up to 256 predecessors (one
+    per lane in up to 4 windows of a 64-lane wavefront).  Larger windows are
+    rejected, before any device work."""
     L = max(1, int(cfg.max_lookback))
     if L > MAX_LOOKBACK:
-        raise ValueError(f"max_lookback {L} above {MAX_LOOKBACK}: the chaining kernel holds one "
-                         "predecessor per lane of a 64-lane wavefront")
+        raise ValueError(f"max_lookback {L} above {MAX_LOOKBACK}: the chaining kernels hold up "
+                         "to 4 windows of 64 predecessors")
     return L
 
 

@@ -10,7 +10,8 @@
 // (read_end, ref_end, f) in registers; each step evaluates all candidates,
 // reduces (max score, first j) across the wave with DPP, writes f[i] and
 // parent[i], and slides the window one lane left (DPP wave_shl:1) with anchor
-// i entering at lane 63.  Lookback is therefore capped at 64.
+// i entering at lane 63.  Lookbacks up to 256 take 2 or 4 such windows
+// (chain_dp_wide_kernel).
 //
 // The float64 profile reproduces numpy exactly: the penalty comes from a
 // caller-supplied table computed with the reference's own numpy expression
@@ -180,16 +181,178 @@ __global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
     }
 }
 
+// Lookback beyond one wavefront (64 < L <= 64 K): K windows of 64
+// predecessors, window w (0 the oldest) holding j = i - 64 (K - w) + lane.
+// The windows slide together, lane 63 of window w taking lane 0 of window
+// w+1 and the newest taking anchor i.  The best predecessor is the largest
+// score, then the smallest j (chain_dp_numpy's argmax): the wave maximum,
+// then the oldest window with a lane at it, its first such lane.
+template <bool F64, bool BONUS, int K>
+__global__ __launch_bounds__(64) void chain_dp_wide_kernel(ChainArgs a)
+{
+    const int seg = blockIdx.x;
+    if (seg >= a.n_segments) return;
+    const int lane = (int)threadIdx.x;
+    const int64_t base = a.seg_off[seg];
+    const int n = (int)(a.seg_off[seg + 1] - base);
+    const int L = a.L;
+    const int64_t G = a.max_gap;
+
+    int64_t wre[K], wfe[K];
+    double wfd[K];
+    int wfi[K];
+#pragma unroll
+    for (int w = 0; w < K; ++w) {
+        wre[w] = 0;
+        wfe[w] = 0;
+        wfd[w] = 0.0;
+        wfi[w] = 0;
+    }
+    int64_t bre = 0, bfe = 0, nre = 0, nfe = 0;
+    double bwd = 0.0, nwd = 0.0;
+    int bwi = 0, nwi = 0;
+    auto load_blk = [&](int i0, int64_t &re, int64_t &fe, double &wd, int &wi) {
+        const int k = i0 + lane;
+        if (k < n) {
+            re = a.read_end[base + k];
+            fe = a.ref_end[base + k];
+            if constexpr (F64) wd = a.wd[base + k];
+            else wi = a.wi[base + k];
+        }
+    };
+    load_blk(0, bre, bfe, bwd, bwi);
+    load_blk(64, nre, nfe, nwd, nwi);
+    for (int i = 0; i < n; ++i) {
+        const int t = i & 63;
+        if (t == 0 && i > 0) {
+            bre = nre;
+            bfe = nfe;
+            bwd = nwd;
+            bwi = nwi;
+            load_blk(i + 64, nre, nfe, nwd, nwi);
+        }
+        const int64_t re_i = readlane_i64(bre, t);
+        const int64_t fe_i = readlane_i64(bfe, t);
+        const int lo = i - L > 0 ? i - L : 0;
+        const int nvalid = i - lo;
+        bool ok[K];
+        double sd[K];
+        int si[K];
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < K; ++w) {
+            const int j = i - 64 * (K - w) + lane;
+            const int64_t dq = re_i - wre[w];
+            const int64_t dr = fe_i - wfe[w];
+            ok[w] = j >= lo && dq > 0 && dr > 0 && dq <= G && dr <= G;
+            any = any || ok[w];
+            const int64_t mn = dq < dr ? dq : dr;
+            int64_t g = dr - dq;
+            g = g < 0 ? -g : g;
+            if constexpr (F64) {
+                const double w_i = readlane_f64(bwd, t);
+                double sv = -__builtin_inf();
+                if (ok[w]) {
+                    const double adv = (double)mn < w_i ? (double)mn : w_i;
+                    sv = __dsub_rn(__dadd_rn(wfd[w], adv), a.lut[g]);
+                    if (BONUS) sv = __dadd_rn(sv, a.bonus[(base + i) * L + (L - nvalid) + (j - lo)]);
+                }
+                sd[w] = sv;
+            } else {
+                const int w_i = __builtin_amdgcn_readlane(bwi, t);
+                int sv = INT_MIN;
+                if (ok[w]) {
+                    const int adv = (int)(mn < (int64_t)w_i ? mn : (int64_t)w_i);
+                    const int pen = g > 0 ? a.go + a.ge * (int)g : 0;
+                    sv = wfi[w] + adv - pen;
+                }
+                si[w] = sv;
+            }
+        }
+        const bool anyok = __builtin_amdgcn_ballot_w64(any) != 0ull;
+        int64_t par = -1;
+        if constexpr (F64) {
+            const double w_i = readlane_f64(bwd, t);
+            double m = sd[0];
+#pragma unroll
+            for (int w = 1; w < K; ++w) m = fmax(m, sd[w]);
+            const double best = readlane_f64(wave_max_to63(m), 63);
+            double fi = w_i;
+            if (anyok && best > w_i) {
+#pragma unroll
+                for (int w = 0; w < K; ++w) {
+                    const uint64_t hit = __builtin_amdgcn_ballot_w64(ok[w] && sd[w] == best);
+                    if (par < 0 && hit != 0ull)
+                        par = i - 64 * (K - w) + (int)__builtin_ctzll(hit);
+                }
+                fi = best;
+            }
+            if (lane == 0) { a.fd[base + i] = fi; a.parent[base + i] = par; }
+#pragma unroll
+            for (int w = 0; w < K; ++w) {
+                const double in = w + 1 < K ? readlane_f64(wfd[w + 1 < K ? w + 1 : w], 0) : fi;
+                wfd[w] = shl_f64(in, wfd[w]);
+            }
+        } else {
+            const int w_i = __builtin_amdgcn_readlane(bwi, t);
+            int m = si[0];
+#pragma unroll
+            for (int w = 1; w < K; ++w) m = max(m, si[w]);
+            const int best = __builtin_amdgcn_readlane(seg_scan_max<64>(m), 63);
+            int fi = w_i;
+            if (anyok && best > w_i) {
+#pragma unroll
+                for (int w = 0; w < K; ++w) {
+                    const uint64_t hit = __builtin_amdgcn_ballot_w64(ok[w] && si[w] == best);
+                    if (par < 0 && hit != 0ull)
+                        par = i - 64 * (K - w) + (int)__builtin_ctzll(hit);
+                }
+                fi = best;
+            }
+            if (lane == 0) { a.fi[base + i] = fi; a.parent[base + i] = par; }
+#pragma unroll
+            for (int w = 0; w < K; ++w) {
+                const int in = w + 1 < K ? __builtin_amdgcn_readlane(wfi[w + 1 < K ? w + 1 : w], 0)
+                                         : fi;
+                wfi[w] = from_next_lane(in, wfi[w]);
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < K; ++w) {
+            const int64_t inr = w + 1 < K ? readlane_i64(wre[w + 1 < K ? w + 1 : w], 0) : re_i;
+            const int64_t inf = w + 1 < K ? readlane_i64(wfe[w + 1 < K ? w + 1 : w], 0) : fe_i;
+            wre[w] = shl_i64(inr, wre[w]);
+            wfe[w] = shl_i64(inf, wfe[w]);
+        }
+    }
+}
+
 }  // namespace gmfa
 
 using namespace gmfa;
+
+constexpr int kMaxLookback = 256;  // 4 windows of 64 predecessors
+
+// the kernel for a lookback: one window up to 64, else 2 or 4
+template <bool F64, bool BONUS>
+static void launch_chain_dp(const ChainArgs &a, hipStream_t s)
+{
+    const dim3 g((unsigned)a.n_segments), b(64);
+    if (a.L <= 64)
+        hipLaunchKernelGGL((chain_dp_kernel<F64, BONUS>), g, b, 0, s, a);
+    else if (a.L <= 128)
+        hipLaunchKernelGGL((chain_dp_wide_kernel<F64, BONUS, 2>), g, b, 0, s, a);
+    else
+        hipLaunchKernelGGL((chain_dp_wide_kernel<F64, BONUS, 4>), g, b, 0, s, a);
+}
 
 static int chain_common_checks(const int64_t *read_end, const int64_t *ref_end,
                                const int64_t *seg_off, int32_t n_segments, int32_t L,
                                int64_t max_gap)
 {
     GMFA_REQUIRE(n_segments >= 0, "negative segment count");
-    GMFA_REQUIRE(L >= 1 && L <= 64, "max_lookback %d outside [1, 64]", L);
+    GMFA_REQUIRE(L >= 1 && L <= kMaxLookback, "max_lookback %d outside [1, %d]", L,
+                 kMaxLookback);
     GMFA_REQUIRE(max_gap >= 0, "negative max_gap");
     if (n_segments == 0) return GMFA_OK;
     GMFA_REQUIRE(read_end && ref_end && seg_off, "NULL pointer");
@@ -211,11 +374,9 @@ extern "C" int gmfa_chain_dp_f64(const int64_t *read_end, const int64_t *ref_end
     a.n_segments = n_segments; a.L = max_lookback; a.max_gap = max_gap;
     a.lut = pen_lut; a.bonus = bonus; a.fd = f; a.parent = parent;
     if (bonus)
-        hipLaunchKernelGGL((chain_dp_kernel<true, true>), dim3(n_segments), dim3(64), 0,
-                           (hipStream_t)stream, a);
+        launch_chain_dp<true, true>(a, (hipStream_t)stream);
     else
-        hipLaunchKernelGGL((chain_dp_kernel<true, false>), dim3(n_segments), dim3(64), 0,
-                           (hipStream_t)stream, a);
+        launch_chain_dp<true, false>(a, (hipStream_t)stream);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }
@@ -235,8 +396,7 @@ extern "C" int gmfa_chain_dp_i32(const int64_t *read_end, const int64_t *ref_end
     a.read_end = read_end; a.ref_end = ref_end; a.wi = weight; a.seg_off = seg_off;
     a.n_segments = n_segments; a.L = max_lookback; a.max_gap = max_gap;
     a.go = gap_open; a.ge = gap_extend; a.fi = f; a.parent = parent;
-    hipLaunchKernelGGL((chain_dp_kernel<false, false>), dim3(n_segments), dim3(64), 0,
-                       (hipStream_t)stream, a);
+    launch_chain_dp<false, false>(a, (hipStream_t)stream);
     GMFA_CHECK_HIP(hipGetLastError());
     return GMFA_OK;
 }

@@ -273,7 +273,8 @@ int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_params *params,
  * seg_off[s] .. seg_off[s+1]-1, already in DP order).  Float64 profile:
  * pen_lut[g] = penalty of gap g for g < max_gap (device array); bonus is an
  * optional (n_anchors, max_lookback) row-major graph bonus.
- * Requires 1 <= max_lookback <= 64. */
+ * Requires 1 <= max_lookback <= 256 (chaining.py:154 has no cap; one
+ * wavefront holds 64 predecessors per window, up to 4 windows). */
 int gmfa_chain_dp_f64(const int64_t *read_end, const int64_t *ref_end, const double *weight,
                       const int64_t *seg_off, int32_t n_segments, int32_t max_lookback,
                       int64_t max_gap, const double *pen_lut, const double *bonus,

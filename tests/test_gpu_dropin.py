@@ -275,7 +275,7 @@ def _anchor_read(rng, n_anchors, read_len=8000, with_ties=False):
 
 @pytest.mark.parametrize("min_score,lookback,ratio,max_chains",
                          [(20.0, 64, 0.6, 8), (0.0, 17, 0.0, 64), (200.0, 50, 0.6, 8),
-                          (10.0, 33, 0.1, 3)])
+                          (10.0, 33, 0.1, 3), (20.0, 200, 0.6, 8)])
 def test_affine_chainer_device_pipeline_vs_oracle(min_score, lookback, ratio, max_chains):
     """Sort, DP, backtrack and select all on the device for a batch of reads
     (ragged, both strands, ties, one read with 20,000 anchors) equal the

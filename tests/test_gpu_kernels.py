@@ -360,6 +360,43 @@ def test_chain_dp_i32_segments_vs_oracle():
         assert np.array_equal(par[a:b], ep), s
 
 
+@pytest.mark.parametrize("lookback", [65, 100, 128, 129, 200, 256])
+def test_chain_dp_wide_lookback_vs_oracle(lookback):
+    """Lookbacks beyond one wavefront (2 or 4 windows of 64 predecessors):
+    both profiles against the oracle, read by read.  Equal weights make many
+    ties, which must go to the smallest predecessor index; the float64
+    profile adds random weights and a graph bonus (chaining.py:129-180)."""
+    from graphmambaformer_amd import workloads
+
+    _say("case: chain wide lookback", lookback)
+    k = K()
+    w = workloads.chains(n_reads=48, seed=lookback)
+    d = torch.device(DEV)
+    re_, fe = torch.from_numpy(w.read_end).to(d), torch.from_numpy(w.ref_end).to(d)
+    so = torch.from_numpy(w.seg_off).to(d)
+    f, par = k.chain_dp(re_, fe, torch.from_numpy(w.weight).to(d), so, lookback, w.max_gap,
+                        gap_open=w.gap_open, gap_extend=w.gap_extend)
+    f, par = f.cpu().numpy(), par.cpu().numpy()
+    rng = np.random.default_rng(lookback)
+    w64 = rng.integers(5, 30, size=len(w.read_end)).astype(np.float64)
+    bonus = rng.normal(0.0, 2.0, size=(len(w.read_end), lookback))
+    lut = O.penalty_lut(6, 0.05, 0.5, w.max_gap)
+    f64, p64 = k.chain_dp(re_, fe, torch.from_numpy(w64).to(d), so, lookback, w.max_gap,
+                          pen_lut=torch.from_numpy(lut).to(d),
+                          bonus=torch.from_numpy(np.ascontiguousarray(bonus)).to(d))
+    f64, p64 = f64.cpu().numpy(), p64.cpu().numpy()
+    for s in range(w.n_reads):
+        a, b = w.seg_off[s], w.seg_off[s + 1]
+        ef, ep = O.chain_dp_i32(w.read_end[a:b], w.ref_end[a:b], w.weight[a:b], lookback,
+                                w.max_gap, w.gap_open, w.gap_extend)
+        assert np.array_equal(f[a:b], ef), s
+        assert np.array_equal(par[a:b], ep), s
+        ef, ep = O.chain_dp_f64(w.read_end[a:b], w.ref_end[a:b], w64[a:b], lookback, w.max_gap,
+                                lut, bonus=bonus[a:b])
+        assert np.array_equal(f64[a:b], ef), s
+        assert np.array_equal(p64[a:b], ep), s
+
+
 def test_sw_edge_cases_vs_oracle():
     """Empty query or target, a band of one diagonal, bands lying entirely
     beside the target, all-N sequences, one-base pairs."""

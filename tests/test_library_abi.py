@@ -219,7 +219,7 @@ def test_noalloc_validates_before_touching_the_device():
 def test_drop_ins_reject_bands_and_lookbacks_beyond_the_kernels():
     """The reference takes any half_band (extension.py:105) and max_lookback
     (chaining.py:154); the kernels here hold half_band <= 1023 and
-    max_lookback <= 64 (the reference defaults are 512 and 64, config.py:681,
+    max_lookback <= 256 (the reference defaults are 512 and 64, config.py:681,
     760).  Beyond the caps the drop-ins raise ValueError before any device
     work; at the caps they accept."""
     import torch
@@ -239,11 +239,11 @@ def test_drop_ins_reject_bands_and_lookbacks_beyond_the_kernels():
         E.banded_affine_sw_batch(q, q, n, n, ExtensionConfig(), half_band=1024)
     with pytest.raises(ValueError, match="half_band 1100"):
         E.banded_affine_sw_batch(q, q, n, n, ExtensionConfig(half_band=1100))
-    with pytest.raises(ValueError, match="max_lookback 65"):
-        C.AffineChainer(ChainingConfig(max_lookback=65))
+    with pytest.raises(ValueError, match="max_lookback 257"):
+        C.AffineChainer(ChainingConfig(max_lookback=257))
     x = np.arange(5, dtype=np.int64)
-    with pytest.raises(ValueError, match="max_lookback 65"):
-        C.chain_dp_numpy(x, x, np.ones(5), ChainingConfig(max_lookback=65))
+    with pytest.raises(ValueError, match="max_lookback 257"):
+        C.chain_dp_numpy(x, x, np.ones(5), ChainingConfig(max_lookback=257))
     C.AffineChainer(ChainingConfig(max_lookback=C.MAX_LOOKBACK))
 
 
