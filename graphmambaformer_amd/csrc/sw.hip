@@ -971,20 +971,23 @@ extern "C" int gmfa_sw_traceback(const gmfa_sw_batch *batch, const gmfa_sw_param
     a.score = score; a.qend = query_end; a.tend = target_end;
     a.tb = tb; a.tb_off = tb_off;
     const TileEntry *t = find_tile(tile.cells_per_lane, tile.lanes_per_pair, a.c.W);
-    // As few pairs per wavefront as still lets every wavefront be resident
-    // at once (8 per SIMD): each walk is a serial chain of dependent loads,
-    // so a lane's chain is shortest when its wavefront shares the SIMD's
-    // issue slots with the fewest other chains, but a second round of
-    // wavefronts waits for the first (16,384 long pairs: 8,192 wavefronts of
-    // 2; 8,192 noisy pairs: one pair per wavefront)
+    // A wavefront per pair (the 64-cell wavefront walk), except for batches
+    // of many short pairs: pairs per wavefront of the lane walk are as few
+    // as still let every wavefront be resident at once (8 per SIMD), and the
+    // lane walk (each lane a serial chain of dependent loads) takes over when
+    // that packs at least 8 pairs of at most 1,024 rows per wavefront.
+    // Measured, standalone walk: 16,384 long-read pairs 5.97 ms on lanes (2
+    // per wavefront) against 1.63 ms on wavefronts; 16,384 150 bp pairs 0.058
+    // against 0.030 ms; 65,536 150 bp pairs 0.078 ms on lanes (8 per
+    // wavefront) against 0.090 ms.
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
     const int64_t slots = (int64_t)8 * 4 * cus;  // 8 wavefronts on each SIMD
     a.per_wave = (int)std::max<int64_t>(1, std::min<int64_t>(64, (n + slots - 1) / slots));
-    // one pair per wavefront anyway: let the whole wavefront walk it (windows)
-    if (a.per_wave == 1)
+    const bool lanes = a.per_wave >= 8 && batch->max_qlen <= 1024;
+    if (!lanes)
         hipLaunchKernelGGL(t->walk_wave, dim3(n), dim3(64), 0, (hipStream_t)stream, a);
     else
         hipLaunchKernelGGL(t->walk, dim3((n + a.per_wave - 1) / a.per_wave), dim3(64), 0,
