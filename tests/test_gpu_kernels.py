@@ -261,6 +261,39 @@ def test_sw_skew_fused_walk_paths():
         _check_against_oracle(qs, ts, offs, hb, scoring, got, f"skewpaths{hb}")
 
 
+@pytest.mark.parametrize("tile", [(2, 63), (1, 64), (8, 64), (5, 16)])
+def test_sw_grid_smaller_than_batch_claims_the_rest(tile):
+    """A persistent grid capped below the batch (max_waves): each wavefront's
+    first item is its workgroup index, the rest come from the work counter
+    after the grid's; a processing order is read.  Two launches in a row on
+    one workspace (the counter must be back at zero after the first), both
+    against the oracle."""
+    k = K()
+    rng = np.random.default_rng(71 + tile[0])
+    hb = 32 if tile != (8, 64) else 100
+    qs, ts, offs = [], [], []
+    for p in range(300):
+        q, t = _random_pair(rng, int(rng.integers(1, 250)))
+        qs.append(q)
+        ts.append(t)
+        offs.append(int(rng.integers(0, hb + 10)))
+    inp = k.upload_pairs(qs, ts, offs, DEV, alphabet="ref")
+    tl = k.resolve_tile(hb, *tile)
+    ws = k.make_workspace(inp, tl, cigar=True)
+    cells = np.array([len(q) for q in qs])
+    order = torch.from_numpy(np.argsort(-cells, kind="stable").astype(np.int32)).to(DEV)
+    for rep in range(2):
+        _say("case: capped grid, tile", tile, "launch", rep)
+        fwd = k.sw_forward(inp, 2, 4, 4, 2, hb, traceback=True, tile=tl, workspace=ws,
+                           cigar=True, order=order, max_waves=37)
+        got = dict(score=fwd.score.cpu().numpy(), qend=fwd.query_end.cpu().numpy(),
+                   tend=fwd.target_end.cpu().numpy(), cigar=k.decode_cigars(fwd.walk),
+                   rs=fwd.walk.read_start.cpu().numpy(), ts=fwd.walk.ref_start.cpu().numpy(),
+                   counts=fwd.walk.counts.cpu().numpy())
+        _check_against_oracle(qs, ts, offs, hb, (2, 4, 4, 2), got, f"capped{tile}-{rep}")
+        assert int(ws["counter"].abs().sum()) == 0, "work counter left at zero"
+
+
 def test_sw_short_config_full():
     """1,024 x 150 bp pairs, hb 32 at offset 32, scoring 2/-4/-4/-2 (whole config)."""
     from graphmambaformer_amd import workloads
