@@ -117,35 +117,46 @@ def chain_dp_numpy(read_end, ref_end, weight, cfg: ChainingConfig,
     return _run_segments(re_, fe, w, np.array([0, n], np.int64), cfg, bonus)
 
 
+def _int_coords(x: torch.Tensor, name: str) -> torch.Tensor:
+    """Integer-valued coordinates as int64 on their device (ValueError otherwise)."""
+    if x.is_floating_point():
+        if not bool(torch.all(torch.isfinite(x) & (x == torch.trunc(x)))):
+            raise ValueError(f"{name} must hold integer coordinates")
+    return x.to(torch.int64)
+
+
 def chain_dp_batched(read_end: torch.Tensor, ref_end: torch.Tensor, weight: torch.Tensor,
                      n_anchors: torch.Tensor, cfg: ChainingConfig,
                      bonus: Optional[torch.Tensor] = None):
-    """Padded (B, A) batch; returns (f float32, parent int64) of shape (B, A)."""
+    """Padded (B, A) batch; returns (f float32, parent int64) of shape (B, A).
+
+    The batch stays on the device: the valid slots of every row (the first
+    n_anchors[b], row-major) are gathered into back-to-back segments, one DP
+    launch runs them all, and the results are scattered back (padded slots 0
+    / -1); each row equals chain_dp_numpy on it (chaining.py:183-258)."""
     B, A = read_end.shape
     out_dev = read_end.device
-    counts = torch.as_tensor(n_anchors).detach().cpu().numpy().astype(np.int64).reshape(B)
-    re_h = read_end.detach().cpu().numpy()
-    fe_h = ref_end.detach().cpu().numpy()
-    w_h = weight.detach().cpu().numpy().astype(np.float64)
-    seg = np.zeros(B + 1, np.int64)
-    np.cumsum(counts, out=seg[1:])
-    flat_re = np.concatenate([re_h[b, :counts[b]] for b in range(B)]) if B else np.zeros(0)
-    flat_fe = np.concatenate([fe_h[b, :counts[b]] for b in range(B)]) if B else np.zeros(0)
-    flat_w = np.concatenate([w_h[b, :counts[b]] for b in range(B)]) if B else np.zeros(0)
-    flat_b = None
-    if bonus is not None:
-        bh = bonus.detach().cpu().numpy().astype(np.float64)
-        flat_b = np.concatenate([bh[b, :counts[b]] for b in range(B)])
-    f, par = _run_segments(_as_int64(flat_re, "read_end"), _as_int64(flat_fe, "ref_end"), flat_w,
-                           seg, cfg, flat_b)
-    fo = np.zeros((B, A), np.float32)
-    po = np.full((B, A), NO_PREDECESSOR, np.int64)
-    for b in range(B):
-        a, z = seg[b], seg[b + 1]
-        fo[b, :z - a] = f[a:z]
+    L = _lookback(cfg)
+    dev = _device()
+    counts = torch.as_tensor(n_anchors).to(dev).reshape(B).to(torch.int64).clamp(0, A)
+    mask = torch.arange(A, device=dev)[None, :] < counts[:, None]
+    re_ = _int_coords(read_end.to(dev)[mask], "read_end")
+    fe = _int_coords(ref_end.to(dev)[mask], "ref_end")
+    w = weight.to(dev, torch.float64)[mask]
+    seg = torch.zeros(B + 1, dtype=torch.int64, device=dev)
+    torch.cumsum(counts, 0, out=seg[1:])
+    fo = torch.zeros((B, A), dtype=torch.float32, device=dev)
+    po = torch.full((B, A), NO_PREDECESSOR, dtype=torch.int64, device=dev)
+    if re_.numel():
+        b = None
+        if bonus is not None:
+            b = bonus.to(dev, torch.float64)[mask].reshape(-1, L).contiguous()
+        f, par = K.chain_dp(re_.contiguous(), fe.contiguous(), w.contiguous(), seg, L,
+                            int(cfg.max_gap), pen_lut=penalty_table(cfg, dev), bonus=b)
         # parents are segment-local already (indices within the read)
-        po[b, :z - a] = par[a:z]
-    return (torch.from_numpy(fo).to(out_dev), torch.from_numpy(po).to(out_dev))
+        fo[mask] = f.to(torch.float32)
+        po[mask] = par
+    return fo.to(out_dev), po.to(out_dev)
 
 
 class AffineChainer:

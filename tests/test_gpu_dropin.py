@@ -135,6 +135,48 @@ def test_chain_dp_numpy_and_batched_golden():
     assert p[1].tolist() == [-1, 0, -1] and float(f[1, 2]) == 0.0
 
 
+def test_chain_dp_batched_rows_vs_oracle():
+    """A padded (B, A) batch on the device (the accelerator tier's shape,
+    chaining.py:183-258): every row equals the oracle's float64 DP on that
+    read (f rounded to float32), padded slots 0 / -1, with and without a
+    (B, A, L) graph bonus."""
+    from graphmambaformer_amd import workloads
+    from graphmambaformer_amd.alignment import chain_dp_batched
+    from graphmambaformer_amd.config import ChainingConfig
+
+    _say("case: chain_dp_batched rows vs oracle")
+    w = workloads.chains(n_reads=40, seed=9)
+    cfg = ChainingConfig(max_lookback=64, max_gap=5000)
+    B = w.n_reads
+    cnt = np.diff(w.seg_off)
+    A = int(cnt.max()) + 3
+    rng = np.random.default_rng(9)
+    wt = rng.integers(5, 30, size=len(w.read_end)).astype(np.float64)
+    re_ = np.zeros((B, A), np.float32)
+    fe = np.zeros((B, A), np.float64)
+    ww = np.zeros((B, A), np.float64)
+    for b in range(B):
+        a, z = w.seg_off[b], w.seg_off[b + 1]
+        re_[b, :z - a] = w.read_end[a:z]
+        fe[b, :z - a] = w.ref_end[a:z]
+        ww[b, :z - a] = wt[a:z]
+    bonus = rng.normal(0.0, 1.5, size=(B, A, cfg.max_lookback))
+    lut = O.penalty_lut(cfg.gap_open, cfg.gap_extend, cfg.log_coeff, cfg.max_gap)
+    for use_bonus in (False, True):
+        f, p = chain_dp_batched(torch.from_numpy(re_).to(DEV), torch.from_numpy(fe).to(DEV),
+                                torch.from_numpy(ww).to(DEV), torch.from_numpy(cnt).to(DEV), cfg,
+                                bonus=torch.from_numpy(bonus).to(DEV) if use_bonus else None)
+        f, p = f.cpu().numpy(), p.cpu().numpy()
+        for b in range(B):
+            a, z = w.seg_off[b], w.seg_off[b + 1]
+            ef, ep = O.chain_dp_f64(w.read_end[a:z], w.ref_end[a:z], wt[a:z], cfg.max_lookback,
+                                    cfg.max_gap, lut,
+                                    bonus=bonus[b, :z - a] if use_bonus else None)
+            assert np.array_equal(f[b, :z - a], ef.astype(np.float32)), (use_bonus, b)
+            assert np.array_equal(p[b, :z - a], ep), (use_bonus, b)
+            assert not f[b, z - a:].any() and (p[b, z - a:] == -1).all(), (use_bonus, b)
+
+
 def test_affine_chainer_golden():
     from graphmambaformer_amd.alignment import AffineChainer, AnchorSet
     from graphmambaformer_amd.config import ChainingConfig
