@@ -514,13 +514,12 @@ __device__ __forceinline__ void scalar_cell(int &Hs, int &Fs, uint32_t &acc, int
 // the one-wavefront result; planes keep the tile's layout (dword d of a block
 // = band diagonal d), so the walkers are unchanged.
 //
-// A split launch is one persistent grid of K-wavefront workgroups.  Phase 1: a
-// workgroup claims the next pair from the longest-first work counter and, if
-// its query has at least split_rows rows, runs it as K slices (the slices
-// hand their best keys to wavefront 0, which then walks the CIGAR).  Then, or
-// at once for a shorter pair, the workgroup's wavefronts go their own way
-// (phase 2): each claims pairs from the same counter and runs them alone with
-// the regular tile, the rest of sw_forward_kernel.
+// A split launch is a persistent grid of K-wavefront workgroups beside the
+// plain kernel: a workgroup claims the next of the split pairs (the
+// longest-first prefix of the batch whose queries have at least split_rows
+// rows) and runs it as K slices (the slices hand their best keys to
+// wavefront 0, which then walks the CIGAR); when the split pairs run out it
+// exits, and the plain kernel, launched beside it, takes the other pairs.
 
 // LDS words per slice and parity slot: P, e_cont difference, H(dN), H(dN+1),
 // F(dN+1) (slot s + 1 holds slice s's, read by slices s - 1 and s + 1)
