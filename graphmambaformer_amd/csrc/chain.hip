@@ -86,31 +86,39 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
     return __longlong_as_double((int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
-template <bool F64, bool BONUS>
-__global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
+__device__ __forceinline__ int32_t shl_t(int32_t old, int32_t v) { return from_next_lane(old, v); }
+__device__ __forceinline__ int64_t shl_t(int64_t old, int64_t v) { return shl_i64(old, v); }
+__device__ __forceinline__ int32_t readlane_t(int32_t v, int l)
 {
-    const int seg = blockIdx.x;
-    if (seg >= a.n_segments) return;
-    const int lane = (int)threadIdx.x;
-    const int64_t base = a.seg_off[seg];
-    const int n = (int)(a.seg_off[seg + 1] - base);
-    const int L = a.L;
-    const int64_t G = a.max_gap;
+    return __builtin_amdgcn_readlane(v, l);
+}
+__device__ __forceinline__ int64_t readlane_t(int64_t v, int l) { return readlane_i64(v, l); }
 
-    int64_t wre = 0, wfe = 0;
+// One read's DP (lookback <= 64) with coordinates of type T: int64 as given,
+// or int32 after subtracting the read's smallest coordinates (the caller
+// takes that form when both spans are below 2^30, so every difference the
+// DP forms is exact in 32 bits; a 32-bit difference and compare per
+// coordinate instead of a 64-bit pair).
+template <bool F64, bool BONUS, typename T>
+__device__ __forceinline__ void chain_segment(const ChainArgs &a, int64_t base, int n,
+                                              int64_t re0, int64_t fe0, T G)
+{
+    const int lane = (int)threadIdx.x;
+    const int L = a.L;
+    T wre = 0, wfe = 0;
     double wfd = 0.0;
     int wfi = 0;
     // anchors 64 at a time (lane k: anchor i0 + k), the next block loaded a
     // block ahead: a step reads its anchor with readlane instead of waiting
     // on a load
-    int64_t bre = 0, bfe = 0, nre = 0, nfe = 0;
+    T bre = 0, bfe = 0, nre = 0, nfe = 0;
     double bwd = 0.0, nwd = 0.0;
     int bwi = 0, nwi = 0;
-    auto load_blk = [&](int i0, int64_t &re, int64_t &fe, double &wd, int &wi) {
+    auto load_blk = [&](int i0, T &re, T &fe, double &wd, int &wi) {
         const int k = i0 + lane;
         if (k < n) {
-            re = a.read_end[base + k];
-            fe = a.ref_end[base + k];
+            re = (T)(a.read_end[base + k] - re0);
+            fe = (T)(a.ref_end[base + k] - fe0);
             if constexpr (F64) wd = a.wd[base + k];
             else wi = a.wi[base + k];
         }
@@ -126,16 +134,16 @@ __global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
             bwi = nwi;
             load_blk(i + 64, nre, nfe, nwd, nwi);
         }
-        const int64_t re_i = readlane_i64(bre, t);
-        const int64_t fe_i = readlane_i64(bfe, t);
+        const T re_i = readlane_t(bre, t);
+        const T fe_i = readlane_t(bfe, t);
         const int lo = i - L > 0 ? i - L : 0;
         const int nvalid = i - lo;  // predecessors lo..i-1 sit in lanes 64-nvalid..63
         const int j = i - 64 + lane;
-        const int64_t dq = re_i - wre;
-        const int64_t dr = fe_i - wfe;
+        const T dq = re_i - wre;
+        const T dr = fe_i - wfe;
         const bool ok = (lane >= 64 - nvalid) && dq > 0 && dr > 0 && dq <= G && dr <= G;
-        const int64_t mn = dq < dr ? dq : dr;
-        int64_t g = dr - dq;
+        const T mn = dq < dr ? dq : dr;
+        T g = dr - dq;
         g = g < 0 ? -g : g;
         const uint64_t okmask = __builtin_amdgcn_ballot_w64(ok);
         int par = -1;
@@ -161,7 +169,7 @@ __global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
             const int w_i = __builtin_amdgcn_readlane(bwi, t);
             int key = INT_MIN;
             if (ok) {
-                const int adv = (int)(mn < (int64_t)w_i ? mn : (int64_t)w_i);
+                const int adv = (int)(mn < (T)w_i ? mn : (T)w_i);
                 const int pen = g > 0 ? a.go + a.ge * (int)g : 0;
                 const int s = wfi + adv - pen;
                 key = (int)(((uint32_t)s << 6) | (uint32_t)(63 - lane));
@@ -176,8 +184,56 @@ __global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
             if (lane == 0) { a.fi[base + i] = fi; a.parent[base + i] = par; }
             wfi = from_next_lane(fi, wfi);
         }
-        wre = shl_i64(re_i, wre);
-        wfe = shl_i64(fe_i, wfe);
+        wre = shl_t(re_i, wre);
+        wfe = shl_t(fe_i, wfe);
+    }
+}
+
+// wave-wide min (MAX = false) or max of an int64, the same in every lane
+template <bool MAX>
+__device__ __forceinline__ int64_t wave_reduce_i64(int64_t x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int64_t y = (int64_t)(((uint64_t)(uint32_t)__shfl_xor((int)((uint64_t)x >> 32), o) << 32) |
+                                    (uint32_t)__shfl_xor((int)(uint32_t)x, o));
+        x = MAX ? (y > x ? y : x) : (y < x ? y : x);
+    }
+    return readlane_i64(x, 0);
+}
+
+template <bool F64, bool BONUS>
+__global__ __launch_bounds__(64) void chain_dp_kernel(ChainArgs a)
+{
+    const int seg = blockIdx.x;
+    if (seg >= a.n_segments) return;
+    const int lane = (int)threadIdx.x;
+    const int64_t base = a.seg_off[seg];
+    const int n = (int)(a.seg_off[seg + 1] - base);
+    // the read's coordinate ranges (one pass, 64 anchors per load)
+    int64_t lo_re = INT64_MAX, lo_fe = INT64_MAX, hi_re = INT64_MIN, hi_fe = INT64_MIN;
+    for (int k = lane; k < n; k += 64) {
+        const int64_t re = a.read_end[base + k], fe = a.ref_end[base + k];
+        lo_re = re < lo_re ? re : lo_re;
+        lo_fe = fe < lo_fe ? fe : lo_fe;
+        hi_re = re > hi_re ? re : hi_re;
+        hi_fe = fe > hi_fe ? fe : hi_fe;
+    }
+    lo_re = wave_reduce_i64<false>(lo_re);
+    lo_fe = wave_reduce_i64<false>(lo_fe);
+    hi_re = wave_reduce_i64<true>(hi_re);
+    hi_fe = wave_reduce_i64<true>(hi_fe);
+    // (spans compared as differences of non-overflowing halves: coordinates
+    // may be any int64)
+    constexpr int64_t kSpan = (int64_t)1 << 30;
+    const bool narrow = n > 0 && lo_re > INT64_MIN / 2 && hi_re < INT64_MAX / 2 &&
+                        lo_fe > INT64_MIN / 2 && hi_fe < INT64_MAX / 2 &&
+                        hi_re - lo_re < kSpan && hi_fe - lo_fe < kSpan;
+    if (narrow) {
+        const int32_t G32 = (int32_t)(a.max_gap < kSpan ? a.max_gap : kSpan);
+        chain_segment<F64, BONUS, int32_t>(a, base, n, lo_re, lo_fe, G32);
+    } else {
+        chain_segment<F64, BONUS, int64_t>(a, base, n, 0, 0, a.max_gap);
     }
 }
 

@@ -393,6 +393,44 @@ def test_chain_dp_i32_segments_vs_oracle():
         assert np.array_equal(par[a:b], ep), s
 
 
+def test_chain_dp_coordinate_ranges_vs_oracle():
+    """Reads whose coordinates sit far from zero (rebased to 32 bits in the
+    kernel) and reads whose anchors span more than 2^30 (the 64-bit form),
+    read_end and ref_end alike, in both profiles against the oracle."""
+    from graphmambaformer_amd import workloads
+
+    _say("case: chain coordinate ranges")
+    k = K()
+    w = workloads.chains(n_reads=32, seed=5)
+    re_, fe = w.read_end.copy(), w.ref_end.copy()
+    for s in range(w.n_reads):
+        a, b = w.seg_off[s], w.seg_off[s + 1]
+        if s % 4 == 1:
+            fe[a:b] += 1 << 40
+            re_[a:b] += 1 << 33
+        elif s % 4 == 2:
+            fe[b - 1] += 3 << 30  # the last anchor (largest ref_end) far away
+        elif s % 4 == 3:
+            re_[a] += 5 << 30
+    d = torch.device(DEV)
+    so = torch.from_numpy(w.seg_off).to(d)
+    tre, tfe = torch.from_numpy(re_).to(d), torch.from_numpy(fe).to(d)
+    f, par = k.chain_dp(tre, tfe, torch.from_numpy(w.weight).to(d), so, w.max_lookback, w.max_gap,
+                        gap_open=w.gap_open, gap_extend=w.gap_extend)
+    w64 = np.random.default_rng(5).integers(5, 30, size=len(re_)).astype(np.float64)
+    lut = O.penalty_lut(6, 0.05, 0.5, w.max_gap)
+    f64, p64 = k.chain_dp(tre, tfe, torch.from_numpy(w64).to(d), so, w.max_lookback, w.max_gap,
+                          pen_lut=torch.from_numpy(lut).to(d))
+    f, par, f64, p64 = f.cpu().numpy(), par.cpu().numpy(), f64.cpu().numpy(), p64.cpu().numpy()
+    for s in range(w.n_reads):
+        a, b = w.seg_off[s], w.seg_off[s + 1]
+        ef, ep = O.chain_dp_i32(re_[a:b], fe[a:b], w.weight[a:b], w.max_lookback, w.max_gap,
+                                w.gap_open, w.gap_extend)
+        assert np.array_equal(f[a:b], ef) and np.array_equal(par[a:b], ep), s
+        ef, ep = O.chain_dp_f64(re_[a:b], fe[a:b], w64[a:b], w.max_lookback, w.max_gap, lut)
+        assert np.array_equal(f64[a:b], ef) and np.array_equal(p64[a:b], ep), s
+
+
 @pytest.mark.parametrize("lookback", [65, 100, 128, 129, 200, 256])
 def test_chain_dp_wide_lookback_vs_oracle(lookback):
     """Lookbacks beyond one wavefront (2 or 4 windows of 64 predecessors):
